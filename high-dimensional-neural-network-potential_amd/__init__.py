@@ -1,0 +1,213 @@
+"""MI355X HDNNP engine: Python host mirror of the reference potential API.
+
+Binds lib/libhdnnp_amd.so (include/hdnnp_capi.h) with ctypes. The names
+mirror the reference Fortran interface (src/potentials/hdnnps.f90):
+
+    h = parseRuNNerInput(path, precision="fp64")        # hdnnps.f90:34
+    E, F, dEdLat = hdnnpEnergyAndForces(h, ats)          # hdnnps.f90:333
+    stress = dEdLat2Stress(lat, dEdLat)                  # atomicStructure.f90:679
+
+Errors raise HdnnpError (status code + message) instead of the reference's
+print + stop. There is no CPU fallback: without the built library every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .workloads import Structure  # noqa: F401  (host mirror of atStruct)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhdnnp_amd.so")
+
+FP64, MIXED = 0, 1
+_PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
+           "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
+           "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
+           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
+
+
+class HdnnpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [("nelements", ctypes.c_int32), ("Z", ctypes.c_int32 * 16), ("nsf", ctypes.c_int32 * 16),
+                ("nlayers", ctypes.c_int32), ("nodes", ctypes.c_int32 * 8), ("act", ctypes.c_char * 8),
+                ("scalesf", ctypes.c_int32), ("centersf", ctypes.c_int32), ("maxrc", ctypes.c_double),
+                ("atom_energy", ctypes.c_double * 16), ("emin", ctypes.c_double),
+                ("emax", ctypes.c_double)]
+
+
+_LIB = None
+
+
+def load_library():
+    """Load lib/libhdnnp_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HdnnpError(5, f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C "
+                                f"{HERE})")
+        lib = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        lib.hdnnp_create.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(vp)]
+        lib.hdnnp_evaluate.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]
+        lib.hdnnp_evaluate_device.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, _dp, _dp, vp, _dp, vp]
+        lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
+        lib.hdnnp_get_symfunctions.argtypes = [vp, ctypes.c_int32, _dp, _dp, _dp]
+        lib.hdnnp_get_model_info.argtypes = [vp, ctypes.POINTER(ModelInfo)]
+        lib.hdnnp_get_sf_table.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
+        lib.hdnnp_get_weights.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, _dp, ctypes.POINTER(ctypes.c_int64)]
+        lib.hdnnp_dEdLat_to_stress.argtypes = [_dp, _dp, _dp]
+        lib.hdnnp_last_error.argtypes = [vp]
+        lib.hdnnp_last_error.restype = ctypes.c_char_p
+        lib.hdnnp_last_timings.argtypes = [vp, _dp]
+        lib.hdnnp_destroy.argtypes = [vp]
+        lib.hdnnp_destroy.restype = None
+        _LIB = lib
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Hdnnp:
+    """Handle of one potential (the reference's type(hdnnp), hdnnps.f90:13-21)."""
+
+    def __init__(self, param_dir: str, precision="fp64"):
+        self.lib = load_library()
+        self.ctx = ctypes.c_void_p()
+        self.precision = _PREC[precision]
+        rc = self.lib.hdnnp_create(os.fsencode(param_dir), self.precision, ctypes.byref(self.ctx))
+        if rc:
+            raise HdnnpError(rc, self.lib.hdnnp_last_error(None).decode())
+        info = ModelInfo()
+        self._check(self.lib.hdnnp_get_model_info(self.ctx, ctypes.byref(info)))
+        self.info = info
+        self.maxsf = max(info.nsf[e] for e in range(info.nelements))
+
+    def _check(self, rc):
+        if rc:
+            raise HdnnpError(rc, self.lib.hdnnp_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.hdnnp_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- model introspection -------------------------------------------------
+    @property
+    def elements(self):
+        return [self.info.Z[e] for e in range(self.info.nelements)]
+
+    @property
+    def maxrc(self) -> float:
+        return self.info.maxrc
+
+    def sf_table(self, e: int) -> np.ndarray:
+        n = self.info.nsf[e]
+        rows = np.zeros((max(n, 1), 11))
+        self._check(self.lib.hdnnp_get_sf_table(self.ctx, e, n, _p(rows, _dp)))
+        return rows[:n]
+
+    def weights(self, e: int) -> np.ndarray:
+        cnt = ctypes.c_int64()
+        self._check(self.lib.hdnnp_get_weights(self.ctx, e, 0, None, ctypes.byref(cnt)))
+        w = np.zeros(cnt.value)
+        self._check(self.lib.hdnnp_get_weights(self.ctx, e, cnt.value, _p(w, _dp), ctypes.byref(cnt)))
+        return w
+
+    # --- evaluation ----------------------------------------------------------
+    def energy_and_forces(self, s, want_dEdLat: bool = True):
+        """hdnnpEnergyAndForces (hdnnps.f90:333-371): returns (E, F[nat,3], dEdLat[3,3]|None).
+        dEdLat uses the reference memory order (row k = d E / d lattice vector k)."""
+        pos = np.ascontiguousarray(s.pos, dtype=np.float64)
+        Z = np.ascontiguousarray(s.Z, dtype=np.int32)
+        lat = np.ascontiguousarray(s.lat, dtype=np.float64)
+        nat = Z.shape[0]
+        E = ctypes.c_double()
+        F = np.zeros((nat, 3))
+        L = np.zeros((3, 3)) if want_dEdLat else None
+        self._check(self.lib.hdnnp_evaluate(self.ctx, nat, _p(Z, _ip), _p(pos, _dp), int(bool(s.periodic)),
+                                            _p(lat, _dp), ctypes.byref(E), _p(F, _dp), _p(L, _dp)))
+        return E.value, F, L
+
+    def evaluate_device(self, Z_dev, pos_dev, force_dev, periodic: bool, lat, stream=None):
+        """Device-resident evaluation: Z_dev/pos_dev/force_dev are device
+        pointers (ints), lat a (3,3) host array. Returns (E, dEdLat)."""
+        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        E = ctypes.c_double()
+        L = np.zeros((3, 3))
+        nat = self._nat_dev
+        self._check(self.lib.hdnnp_evaluate_device(self.ctx, nat, ctypes.c_void_p(Z_dev), ctypes.c_void_p(pos_dev),
+                                                   int(bool(periodic)), _p(lat, _dp), ctypes.byref(E),
+                                                   ctypes.c_void_p(force_dev), _p(L, _dp),
+                                                   ctypes.c_void_p(stream or 0)))
+        return E.value, L
+
+    def evaluate_torch(self, Z, pos, periodic: bool, lat, force=None, stream=None):
+        """Z (int32), pos (float64 [nat,3]) torch tensors on the GPU; returns (E, force, dEdLat)."""
+        import torch
+        nat = int(Z.shape[0])
+        if force is None:
+            force = torch.empty((nat, 3), dtype=torch.float64, device=pos.device)
+        self._nat_dev = nat
+        E, L = self.evaluate_device(Z.data_ptr(), pos.data_ptr(), force.data_ptr(), periodic, lat, stream)
+        return E, force, L
+
+    # --- stage outputs of the last evaluation ------------------------------------
+    def neighbor_entries(self) -> np.ndarray:
+        n = ctypes.c_int64()
+        self._check(self.lib.hdnnp_neighbor_count(self.ctx, ctypes.byref(n)))
+        ent = np.zeros((max(n.value, 1), 5), dtype=np.int32)
+        self._check(self.lib.hdnnp_get_neighbors(self.ctx, n.value, _p(ent, _ip)))
+        return ent[:n.value]
+
+    def symfunctions(self, nat: int):
+        G = np.zeros((nat, self.maxsf))
+        D = np.zeros((nat, self.maxsf))
+        Ei = np.zeros(nat)
+        self._check(self.lib.hdnnp_get_symfunctions(self.ctx, self.maxsf, _p(G, _dp), _p(D, _dp), _p(Ei, _dp)))
+        return G, D, Ei
+
+    def timings(self):
+        t = np.zeros(6)
+        self._check(self.lib.hdnnp_last_timings(self.ctx, _p(t, _dp)))
+        return t
+
+
+def parseRuNNerInput(path: str, precision="fp64") -> Hdnnp:
+    return Hdnnp(path, precision)
+
+
+def hdnnpEnergyAndForces(handle: Hdnnp, ats, want_dEdLat: bool = True):
+    return handle.energy_and_forces(ats, want_dEdLat)
+
+
+def dEdLat2Stress(lat, dEdLat) -> np.ndarray:
+    lib = load_library()
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    d = np.ascontiguousarray(dEdLat, dtype=np.float64)
+    out = np.zeros((3, 3))
+    rc = lib.hdnnp_dEdLat_to_stress(_p(lat, _dp), _p(d, _dp), _p(out, _dp))
+    if rc:
+        raise HdnnpError(rc, "dEdLat2Stress failed")
+    return out

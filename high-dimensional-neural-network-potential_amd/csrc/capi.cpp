@@ -1,0 +1,214 @@
+// C ABI (include/hdnnp_capi.h): thin extern "C" layer over hdnnp::Engine.
+// Exceptions never cross the boundary; they become status codes plus a
+// message retrievable with hdnnp_last_error().
+#include "../../include/hdnnp_capi.h"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+#include "runner.hpp"
+
+struct hdnnp_ctx {
+    hdnnp::Engine *eng = nullptr;
+    std::string err;
+};
+
+static thread_local std::string g_create_error;
+
+template <typename F>
+static int guarded(hdnnp_ctx *ctx, F &&f)
+{
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return HDNNP_OK;
+    } catch (const hdnnp::Error &e) {
+        (ctx ? ctx->err : g_create_error) = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        (ctx ? ctx->err : g_create_error) = "host out of memory";
+        return HDNNP_ERR_CAPACITY;
+    } catch (const std::exception &e) {
+        (ctx ? ctx->err : g_create_error) = e.what();
+        return HDNNP_ERR_DEVICE;
+    }
+}
+
+extern "C" {
+
+int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out)
+{
+    if (!out || !param_dir) {
+        g_create_error = "hdnnp_create: null argument";
+        return HDNNP_ERR_ARG;
+    }
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        hdnnp::HostModel m = hdnnp::read_runner(param_dir);
+        auto *c = new hdnnp_ctx;
+        try {
+            c->eng = new hdnnp::Engine(std::move(m), precision);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int hdnnp_evaluate(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *pos, int32_t periodic,
+                   const double *lat, double *energy, double *force, double *dEdLat)
+{
+    if (!ctx || !energy) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->evaluate_host(nat, Z, pos, periodic, lat, energy, force, dEdLat); });
+}
+
+int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev, const double *pos_dev,
+                          int32_t periodic, const double *lat, double *energy, double *force_dev,
+                          double *dEdLat, void *stream)
+{
+    if (!ctx || !energy) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng->evaluate_device(nat, Z_dev, pos_dev, periodic, lat, energy, force_dev, dEdLat,
+                                  (hipStream_t)stream);
+    });
+}
+
+int hdnnp_neighbor_count(hdnnp_ctx *ctx, int64_t *count)
+{
+    if (!ctx || !count) return HDNNP_ERR_ARG;
+    *count = ctx->eng->neighbor_count();
+    return HDNNP_OK;
+}
+
+int hdnnp_get_neighbors(hdnnp_ctx *ctx, int64_t cap, int32_t *ent)
+{
+    if (!ctx || (!ent && cap > 0)) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->get_neighbors(cap, ent); });
+}
+
+int hdnnp_get_symfunctions(hdnnp_ctx *ctx, int32_t ld, double *G, double *dEdG, double *Ei)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->get_symfunctions(ld, G, dEdG, Ei); });
+}
+
+int hdnnp_get_model_info(hdnnp_ctx *ctx, hdnnp_model_info *info)
+{
+    if (!ctx || !info) return HDNNP_ERR_ARG;
+    const hdnnp::HostModel &m = ctx->eng->model();
+    std::memset(info, 0, sizeof *info);
+    info->nelements = (int32_t)m.el.size();
+    for (size_t e = 0; e < m.el.size() && e < 16; ++e) {
+        info->Z[e] = m.el[e].Z;
+        info->nsf[e] = (int32_t)m.el[e].sfs.size();
+        info->atom_energy[e] = m.el[e].atom_energy;
+    }
+    if (!m.el.empty()) {
+        info->nlayers = (int32_t)m.el[0].layers.size();
+        for (size_t l = 0; l < m.el[0].layers.size() && l < 8; ++l) {
+            info->nodes[l] = m.el[0].layers[l].nout;
+            info->act[l] = m.el[0].layers[l].act;
+        }
+    }
+    info->scalesf = m.scalesf;
+    info->centersf = m.centersf;
+    info->maxrc = m.maxrc;
+    info->emin = m.emin;
+    info->emax = m.emax;
+    return HDNNP_OK;
+}
+
+int hdnnp_get_sf_table(hdnnp_ctx *ctx, int32_t e, int32_t cap_rows, double *rows)
+{
+    if (!ctx || !rows) return HDNNP_ERR_ARG;
+    const hdnnp::HostModel &m = ctx->eng->model();
+    if (e < 0 || e >= (int)m.el.size()) {
+        ctx->err = "element rank out of range";
+        return HDNNP_ERR_ARG;
+    }
+    const auto &sfs = m.el[e].sfs;
+    if (cap_rows < (int)sfs.size()) {
+        ctx->err = "row buffer too small";
+        return HDNNP_ERR_ARG;
+    }
+    for (size_t s = 0; s < sfs.size(); ++s) {
+        const auto &f = sfs[s];
+        double *r = rows + 11 * s;
+        r[0] = f.type;
+        r[1] = f.e1;
+        r[2] = f.type == 3 ? f.e2 : 0;
+        r[3] = f.rcut;
+        r[4] = f.eta;
+        r[5] = f.type == 2 ? f.rs : 0.0;
+        r[6] = f.type == 3 ? f.lambda : 0.0;
+        r[7] = f.type == 3 ? f.zeta : 0.0;
+        r[8] = f.sfmin;
+        r[9] = f.sfmax;
+        r[10] = f.sfmean;
+    }
+    return HDNNP_OK;
+}
+
+int hdnnp_get_weights(hdnnp_ctx *ctx, int32_t e, int64_t cap, double *w, int64_t *count)
+{
+    if (!ctx || !count) return HDNNP_ERR_ARG;
+    const hdnnp::HostModel &m = ctx->eng->model();
+    if (e < 0 || e >= (int)m.el.size()) {
+        ctx->err = "element rank out of range";
+        return HDNNP_ERR_ARG;
+    }
+    int64_t n = 0;
+    for (auto &L : m.el[e].layers) n += (int64_t)L.W.size() + (int64_t)L.b.size();
+    *count = n;
+    if (!w) return HDNNP_OK;
+    if (cap < n) {
+        ctx->err = "weight buffer too small";
+        return HDNNP_ERR_ARG;
+    }
+    int64_t k = 0;
+    for (auto &L : m.el[e].layers) {
+        for (double v : L.W) w[k++] = v;
+        for (double v : L.b) w[k++] = v;
+    }
+    return HDNNP_OK;
+}
+
+int hdnnp_dEdLat_to_stress(const double *lat, const double *dEdLat, double *stress)
+{
+    if (!lat || !dEdLat || !stress) return HDNNP_ERR_ARG;
+    // stress = -1/det3D(lat) * matmul(dEdLat, transpose(lat)); Fortran A(r,c) at [r + 3c]
+    const double det = hdnnp::det3(lat);
+    double out[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += dEdLat[r + 3 * k] * lat[c + 3 * k];
+            out[r + 3 * c] = -1.0 / det * acc;
+        }
+    std::memcpy(stress, out, sizeof out);
+    return HDNNP_OK;
+}
+
+const char *hdnnp_last_error(const hdnnp_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int hdnnp_last_timings(hdnnp_ctx *ctx, double *ms6)
+{
+    if (!ctx || !ms6) return HDNNP_ERR_ARG;
+    ctx->eng->timings(ms6);
+    return HDNNP_OK;
+}
+
+void hdnnp_destroy(hdnnp_ctx *ctx)
+{
+    if (!ctx) return;
+    delete ctx->eng;
+    delete ctx;
+}
+
+}  // extern "C"

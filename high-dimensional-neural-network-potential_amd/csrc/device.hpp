@@ -1,0 +1,122 @@
+// Device-side data structures and kernel launchers of the HDNNP engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hdnnp {
+
+constexpr int kMaxEl = 16;       // elements per model (= kMaxElements)
+constexpr int kMaxLay = 8;       // weight layers
+constexpr int kNonIntegralZeta = -(1 << 30);   // g4_ipow marker
+
+// Model tables in device memory; the struct itself is a kernel argument.
+// G2 entries of centre element e with neighbour element f live at
+// g2_tab[e*kMaxEl+f] = (start, count); G4 entries of centre e with the
+// unordered neighbour pair (f <= g) at g4_tab[(e*kMaxEl+f)*kMaxEl+g].
+// Within a G4 block the entries keep the reference's sorted order, so equal
+// eta are adjacent; g4_neweta marks where exp(-eta*sum r^2) must be redone.
+struct DevModel {
+    int nel;
+    int nsfmax;
+    int nsf[kMaxEl];
+    int sf_off[kMaxEl];
+    double rc[kMaxEl];
+    const int2 *g2_tab;
+    const int2 *g4_tab;
+    const int *g2_s;
+    const double *g2_eta, *g2_rs;
+    const int *g4_s;
+    const double *g4_eta, *g4_lam, *g4_zeta, *g4_pre;
+    const int *g4_ipow;      // zeta-1 when zeta is integral, else kNonIntegralZeta
+    const int *g4_neweta;
+    const double *sf_mean;   // [sf_off[e] + s]; 0 when not centring
+    const double *sf_den;    // max - min, 1 when not scaling
+    int nlayers;
+    int nodes[kMaxLay];
+    char act[kMaxLay];
+    const double *W64[kMaxEl];   // per layer: W(o,j) at [j*nout + o], then bias[nout]
+    const float *W32[kMaxEl];
+    double atom_energy[kMaxEl];
+    const int *zlut;             // Z (0..127) -> element rank, -1 if absent
+};
+
+// Cell grid of one evaluation (host-computed, reference formulas).
+struct Geo {
+    double lat[9];      // list-frame lattice, lat[3k+a] = component a of vector k
+    double inv[9];      // its inverse (inv3DM form)
+    double origin[3];   // artificial-box origin (nonperiodic), else 0
+    double rc2;         // (largest cutoff)^2
+    int periodic;
+    int ng[3];          // grid cells per lattice vector
+    int nc[3];          // stencil half-widths (getNcells of the grid)
+    int nat;
+};
+
+// Device work arrays of one evaluation.
+struct Work {
+    int nat;
+    const double *pos;   // [nat][3] caller positions
+    const int *Z;        // [nat]
+    int *eidx;           // element rank per atom
+    double4 *wpos;       // wrapped positions in the list frame
+    int *cellkey;        // cell id per atom
+    int *cell_atoms;     // atoms sorted by cell
+    int *cellkey_sorted;
+    int *cell_start, *cell_end;
+    int *count;          // [nat+1] neighbours per atom
+    int *offset;         // [nat+1] exclusive scan of count
+    int *ent_j;          // [E] neighbour atom
+    int *ent_shift;      // [E] packed image shift (3 x int8)
+    int *rev;            // [E] partner entry in the neighbour's list, -1 if none
+    int *perm;           // atoms sorted by element
+    int *elem_off;       // [kMaxEl+1] start of each element in perm
+    int *elem_count;     // [kMaxEl]
+    int *flags;          // [8]: 0 unknown element, 1 max neighbours, 2 shift overflow
+    double *Ei;          // [nat]
+    double *vir;         // [nat][9]
+    double *force;       // [nat][3]
+    double *result;      // [16]: energy, W[9]
+    double *bbox;        // [6]
+};
+
+__host__ __device__ inline int pack_shift(int x, int y, int z)
+{
+    return (x & 0xFF) | ((y & 0xFF) << 8) | ((z & 0xFF) << 16);
+}
+__host__ __device__ inline void unpack_shift(int p, int &x, int &y, int &z)
+{
+    x = (int)(signed char)(p & 0xFF);
+    y = (int)(signed char)((p >> 8) & 0xFF);
+    z = (int)(signed char)((p >> 16) & 0xFF);
+}
+
+// ---- launchers (kernels_nl.hip) ----
+void launch_map_elements(const DevModel &m, Work &w, hipStream_t s);
+void launch_bbox(Work &w, hipStream_t s);
+void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s);
+void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s);
+void launch_nl_count(const Geo &g, Work &w, hipStream_t s);
+void launch_nl_fill(const Geo &g, Work &w, hipStream_t s);
+void launch_nl_rev(Work &w, hipStream_t s);
+void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s);
+void launch_fill_int(int *p, int v, long n, hipStream_t s);
+
+// ---- launchers (kernels_sf.hip); R = double or float ----
+template <typename R>
+void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, R *G, hipStream_t s);
+template <typename R>
+void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, const R *dEdG, R *gent,
+                    hipStream_t s);
+size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, bool grad);
+// ---- kernels_mlp.hip: T atoms per work-group, maxw = widest layer/input ----
+template <typename R>
+void launch_mlp_t(const DevModel &m, Work &w, int T, int maxw, int nblocks, const R *G, R *dEdG,
+                  hipStream_t s);
+size_t mlp_lds_bytes(bool fp64, int T, int sumc, int maxw);
+// ---- kernels_misc.hip ----
+template <typename R>
+void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s);
+void launch_reduce(Work &w, hipStream_t s);
+
+}  // namespace hdnnp

@@ -1,0 +1,554 @@
+// Host orchestration of one HDNNP evaluation on the GPU.
+//
+// Reference flow: hdnnpEnergyAndForces (src/potentials/hdnnps.f90:333-371)
+// = neighbour lists (:373-388) -> symmetry functions (:390-485) -> NNs,
+// forces and dEdLat (:487-562). Here:
+//   map elements -> wrap + bin -> cell sort -> neighbour count / scan / fill
+//   -> partner entries -> SF values -> MLP (+dE/dG) -> SF gradients
+//   -> force gather -> FP64 energy / virial reductions.
+// This file is built with -ffp-contract=off: the cell-grid parameters below
+// must be rounded exactly as in the reference.
+#include "engine.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hdnnp {
+
+#pragma clang fp contract(off)
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            throw Error(5, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);     \
+    } while (0)
+
+void DevMem::ensure(size_t b)
+{
+    if (b <= bytes && p) return;
+    release();
+    const size_t want = std::max<size_t>(256, b + b / 4);
+    HIPCHK(hipMalloc(&p, want));
+    bytes = want;
+}
+
+void DevMem::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// det3D and inv3DM (src/math/linalg.f90:227-238, 264-289)
+double det3(const double *M)
+{
+#define A3(r, c) M[(r) + 3 * (c)]
+    double t = A3(0, 0) * A3(1, 1) * A3(2, 2);
+    t = t + A3(0, 1) * A3(1, 2) * A3(2, 0);
+    t = t + A3(0, 2) * A3(1, 0) * A3(2, 1);
+    t = t - A3(0, 0) * A3(1, 2) * A3(2, 1);
+    t = t - A3(0, 1) * A3(1, 0) * A3(2, 2);
+    t = t - A3(0, 2) * A3(1, 1) * A3(2, 0);
+    return t;
+#undef A3
+}
+
+void inv3(const double *A, double *B)
+{
+#define A3(r, c) A[(r) + 3 * (c)]
+    const double a1[3] = {A3(0, 0), A3(0, 1), A3(0, 2)};
+    const double a2[3] = {A3(1, 0), A3(1, 1), A3(1, 2)};
+    const double a3[3] = {A3(2, 0), A3(2, 1), A3(2, 2)};
+#undef A3
+    const double det = det3(A);
+    const double b1[3] = {a2[1] * a3[2] - a2[2] * a3[1], a1[2] * a3[1] - a1[1] * a3[2],
+                          a1[1] * a2[2] - a1[2] * a2[1]};
+    const double b2[3] = {a2[2] * a3[0] - a2[0] * a3[2], a1[0] * a3[2] - a1[2] * a3[0],
+                          a1[2] * a2[0] - a1[0] * a2[2]};
+    const double b3[3] = {a2[0] * a3[1] - a2[1] * a3[0], a1[1] * a3[0] - a1[0] * a3[1],
+                          a1[0] * a2[1] - a1[1] * a2[0]};
+    for (int c = 0; c < 3; ++c) {
+        B[0 + 3 * c] = b1[c] / det;
+        B[1 + 3 * c] = b2[c] / det;
+        B[2 + 3 * c] = b3[c] / det;
+    }
+}
+
+// getNcells (src/atoms/pointsets.f90:220-243)
+static void get_ncells(const double *L, double rc, int n[3])
+{
+    auto cross = [](const double *a, const double *b, double *c) {
+        c[0] = a[1] * b[2] - a[2] * b[1];
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double axb[3], axc[3], bxc[3];
+    cross(L + 0, L + 3, axb);
+    cross(L + 0, L + 6, axc);
+    cross(L + 3, L + 6, bxc);
+    for (double *v : {axb, axc, bxc}) {
+        const double nr = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; ++k) v[k] = v[k] / nr;
+    }
+    const double p0 = L[0] * bxc[0] + L[1] * bxc[1] + L[2] * bxc[2];
+    const double p1 = L[3] * axc[0] + L[4] * axc[1] + L[5] * axc[2];
+    const double p2 = L[6] * axb[0] + L[7] * axb[1] + L[8] * axb[2];
+    n[0] = (int)std::ceil(rc / std::fabs(p0));
+    n[1] = (int)std::ceil(rc / std::fabs(p1));
+    n[2] = (int)std::ceil(rc / std::fabs(p2));
+}
+
+__global__ void k_iota(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+
+Engine::Engine(HostModel m, int precision) : model_(std::move(m)), precision_(precision)
+{
+    if (precision_ != 0 && precision_ != 1) throw Error(1, "precision must be HDNNP_FP64 or HDNNP_MIXED");
+    if ((int)model_.el.size() > kMaxEl) throw Error(6, "too many elements for this build");
+    for (size_t e = 0; e < model_.el.size(); ++e) {
+        elem_valid_[e] = model_.el[e].valid;
+        elem_why_[e] = model_.el[e].why;
+    }
+}
+
+Engine::~Engine()
+{
+    for (auto &d : model_mem_) d.release();
+    for (DevMem *d : {&zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
+                      &iota_, &cell_start_, &cell_end_, &count_, &offset_, &perm_, &eidx_sorted_, &Ei_,
+                      &vir_, &misc_, &ent_j_, &ent_shift_, &rev_, &gent_, &G_, &dEdG_, &cub_tmp_})
+        d->release();
+    if (pinned_) (void)hipHostFree(pinned_);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename T>
+static const T *upload(std::vector<DevMem> &keep, const std::vector<T> &v)
+{
+    keep.emplace_back();
+    keep.back().ensure(sizeof(T) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) HIPCHK(hipMemcpy(keep.back().p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return keep.back().as<T>();
+}
+
+void Engine::init_device()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(5, "no HIP device available");
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipHostMalloc(&pinned_, 4096, hipHostMallocDefault));
+
+    const HostModel &m = model_;
+    DevModel d{};
+    d.nel = (int)m.el.size();
+    std::vector<int> zlut(128, -1);
+    std::vector<double> mean, den;
+    std::vector<int2> g2tab(kMaxEl * kMaxEl, make_int2(0, 0)), g4tab(kMaxEl * kMaxEl * kMaxEl, make_int2(0, 0));
+    std::vector<int> g2s, g4s, g4ipow, g4new;
+    std::vector<double> g2eta, g2rs, g4eta, g4lam, g4zeta, g4pre;
+    int off = 0;
+    for (int e = 0; e < d.nel; ++e) {
+        const ElementModel &em = m.el[e];
+        if (em.Z < 128) zlut[em.Z] = e;
+        d.nsf[e] = (int)em.sfs.size();
+        d.sf_off[e] = off;
+        off += d.nsf[e];
+        d.nsfmax = std::max(d.nsfmax, d.nsf[e]);
+        d.rc[e] = em.sfs.empty() ? 0.0 : em.sfs[0].rcut;
+        d.atom_energy[e] = em.atom_energy;
+        for (auto &s : em.sfs) {
+            mean.push_back(m.centersf ? s.sfmean : 0.0);
+            den.push_back(m.scalesf ? s.sfmax - s.sfmin : 1.0);
+        }
+        for (int f = 0; f < d.nel; ++f) {          // G2 of (e, f)
+            const int start = (int)g2s.size();
+            for (int s = 0; s < d.nsf[e]; ++s) {
+                const SymFn &sf = em.sfs[s];
+                if (sf.type == 2 && sf.e1 == m.el[f].Z) {
+                    g2s.push_back(s);
+                    g2eta.push_back(sf.eta);
+                    g2rs.push_back(sf.rs);
+                }
+            }
+            g2tab[e * kMaxEl + f] = make_int2(start, (int)g2s.size() - start);
+        }
+        for (int f = 0; f < d.nel; ++f)               // G4 of (e, {f, g})
+            for (int g = f; g < d.nel; ++g) {
+                const int start = (int)g4s.size();
+                double last = 0.0;
+                for (int s = 0; s < d.nsf[e]; ++s) {
+                    const SymFn &sf = em.sfs[s];
+                    if (sf.type != 3) continue;
+                    const int za = m.el[f].Z, zb = m.el[g].Z;
+                    if (!((sf.e1 == za && sf.e2 == zb) || (sf.e1 == zb && sf.e2 == za))) continue;
+                    const int iz = (int)sf.zeta;
+                    const bool integral = std::fabs(sf.zeta - iz) < 1.e-10;
+                    double pre;
+                    if (integral) {    // 2**(1-intzeta) by binary powering, exact
+                        pre = std::ldexp(1.0, 1 - iz);
+                    } else {
+                        pre = std::pow(2.0, 1.0 - sf.zeta);
+                    }
+                    const bool first = (int)g4s.size() == start;
+                    g4new.push_back(first || std::fabs(sf.eta - last) > 1.e-10 ? 1 : 0);
+                    if (g4new.back()) last = sf.eta;
+                    g4s.push_back(s);
+                    g4eta.push_back(g4new.back() ? sf.eta : last);
+                    g4lam.push_back(sf.lambda);
+                    g4zeta.push_back(sf.zeta);
+                    g4pre.push_back(pre);
+                    g4ipow.push_back(integral ? iz - 1 : kNonIntegralZeta);
+                }
+                g4tab[(e * kMaxEl + f) * kMaxEl + g] = make_int2(start, (int)g4s.size() - start);
+            }
+    }
+    d.zlut = upload(model_mem_, zlut);
+    d.sf_mean = upload(model_mem_, mean);
+    d.sf_den = upload(model_mem_, den);
+    d.g2_tab = upload(model_mem_, g2tab);
+    d.g4_tab = upload(model_mem_, g4tab);
+    d.g2_s = upload(model_mem_, g2s);
+    d.g2_eta = upload(model_mem_, g2eta);
+    d.g2_rs = upload(model_mem_, g2rs);
+    d.g4_s = upload(model_mem_, g4s);
+    d.g4_eta = upload(model_mem_, g4eta);
+    d.g4_lam = upload(model_mem_, g4lam);
+    d.g4_zeta = upload(model_mem_, g4zeta);
+    d.g4_pre = upload(model_mem_, g4pre);
+    d.g4_ipow = upload(model_mem_, g4ipow);
+    d.g4_neweta = upload(model_mem_, g4new);
+    d.nlayers = m.el.empty() ? 0 : (int)m.el[0].layers.size();
+    mlp_maxw_ = d.nsfmax;
+    mlp_sumc_ = 0;
+    for (int l = 0; l < d.nlayers; ++l) {
+        d.nodes[l] = m.el[0].layers[l].nout;
+        d.act[l] = m.el[0].layers[l].act;
+        mlp_maxw_ = std::max(mlp_maxw_, d.nodes[l]);
+        mlp_sumc_ += d.nodes[l];
+    }
+    for (int e = 0; e < d.nel; ++e) {
+        std::vector<double> w;
+        for (auto &L : m.el[e].layers) {          // W(i,j) at i + nout*j == [j][i]
+            w.insert(w.end(), L.W.begin(), L.W.end());
+            w.insert(w.end(), L.b.begin(), L.b.end());
+        }
+        std::vector<float> wf(w.begin(), w.end());
+        d.W64[e] = upload(model_mem_, w);
+        d.W32[e] = upload(model_mem_, wf);
+    }
+    dm_ = d;
+    misc_.ensure(1024);
+    dev_ready_ = true;
+}
+
+// misc buffer layout
+static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // ints
+static constexpr size_t kDblOffset = 256;                                    // bytes
+static constexpr int kDblResult = 0, kDblBbox = 16;                          // doubles
+
+template <typename R>
+void Engine::compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s)
+{
+    const int nat = w.nat;
+    G_.ensure(sizeof(R) * (size_t)nat * dm_.nsfmax);
+    dEdG_.ensure(sizeof(R) * (size_t)nat * dm_.nsfmax);
+    gent_.ensure(sizeof(R) * 3 * (size_t)std::max(total, 1));
+    R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
+
+    launch_sf_values<R>(dm_, g, w, nmax, G, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_[3], s));
+
+    const int *cnt = (const int *)pinned_ + 1;   // element counts (read back)
+    const bool fp64 = sizeof(R) == 8;
+    int T = 64;
+    while (T > 1 && mlp_lds_bytes(fp64, T, mlp_sumc_, mlp_maxw_) > 65536) T /= 2;
+    if (mlp_lds_bytes(fp64, T, mlp_sumc_, mlp_maxw_) > 65536) throw Error(6, "MLP too wide for this build");
+    int nblocks = 0;
+    for (int e = 0; e < dm_.nel; ++e) nblocks += (cnt[e] + T - 1) / T;
+    if (nblocks > 0) launch_mlp_t<R>(dm_, w, T, mlp_maxw_, nblocks, G, dEdG, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_[4], s));
+
+    launch_sf_grad<R>(dm_, g, w, nmax, dEdG, gent, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_[5], s));
+
+    launch_force_gather_t<R>(w, gent, total, s);
+    launch_reduce(w, s);
+    HIPCHK(hipGetLastError());
+}
+
+void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
+                 double *force_dev, hipStream_t s, double *energy, double *dEdLat)
+{
+    if (nat < 0) throw Error(1, "nat < 0");
+    if (periodic && !lat) throw Error(1, "periodic structure without lattice");
+    const double rc = model_.maxrc;
+    if (!(rc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
+    if (nat == 0) {
+        *energy = 0;
+        if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double));
+        last_nat_ = 0;
+        last_total_ = 0;
+        return;
+    }
+    HIPCHK(hipEventRecord(ev_[0], s));
+    // per-atom buffers
+    eidx_.ensure(4 * (size_t)nat);
+    wpos_.ensure(sizeof(double4) * (size_t)nat);
+    cellkey_.ensure(4 * (size_t)nat);
+    cellkey_sorted_.ensure(4 * (size_t)nat);
+    cell_atoms_.ensure(4 * (size_t)nat);
+    iota_.ensure(4 * (size_t)nat);
+    count_.ensure(4 * ((size_t)nat + 1));
+    offset_.ensure(4 * ((size_t)nat + 1));
+    perm_.ensure(4 * (size_t)nat);
+    eidx_sorted_.ensure(4 * (size_t)nat);
+    Ei_.ensure(8 * (size_t)nat);
+    vir_.ensure(72 * (size_t)nat);
+
+    int *mi = misc_.as<int>();
+    double *md = (double *)((char *)misc_.p + kDblOffset);
+    Work w{};
+    w.nat = nat;
+    w.pos = pos_dev;
+    w.Z = Z_dev;
+    w.eidx = eidx_.as<int>();
+    w.wpos = wpos_.as<double4>();
+    w.cellkey = cellkey_.as<int>();
+    w.cellkey_sorted = cellkey_sorted_.as<int>();
+    w.cell_atoms = cell_atoms_.as<int>();
+    w.count = count_.as<int>();
+    w.offset = offset_.as<int>();
+    w.perm = perm_.as<int>();
+    w.elem_count = mi + kIntElemCount;
+    w.elem_off = mi + kIntElemOff;
+    w.flags = mi + kIntFlags;
+    w.Ei = Ei_.as<double>();
+    w.vir = vir_.as<double>();
+    w.force = force_dev;
+    w.result = md + kDblResult;
+    w.bbox = md + kDblBbox;
+
+    HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
+    launch_map_elements(dm_, w, s);
+
+    Geo g{};
+    g.periodic = periodic ? 1 : 0;
+    g.nat = nat;
+    g.rc2 = rc * rc;
+    double *pin = (double *)pinned_;
+    if (periodic) {
+        std::memcpy(g.lat, lat, sizeof g.lat);
+    } else {                                     // neighborLists.f90:131-138
+        launch_bbox(w, s);
+        HIPCHK(hipMemcpyAsync(pin, w.bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::memset(g.lat, 0, sizeof g.lat);
+        for (int a = 0; a < 3; ++a) {
+            g.origin[a] = pin[a] - 1.e-4;
+            g.lat[a + 3 * a] = (pin[3 + a] - g.origin[a]) + 1.e-4;
+        }
+    }
+    inv3(g.lat, g.inv);
+    double grid[9];
+    const double target = rc * 0.6;               // neighborLists.f90:142-146
+    for (int a = 0; a < 3; ++a) {
+        const double *v = g.lat + 3 * a;
+        const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (!(len > 0) || !std::isfinite(len)) throw Error(1, "degenerate lattice");
+        const double ngd = std::floor(len / target) + 1;
+        if (ngd > 1e6) throw Error(6, "cell grid too large");
+        g.ng[a] = (int)ngd;
+        for (int k = 0; k < 3; ++k) grid[3 * a + k] = v[k] / g.ng[a];
+    }
+    get_ncells(grid, rc, g.nc);                   // neighborLists.f90:169
+    const long ncell = (long)g.ng[0] * g.ng[1] * g.ng[2];
+    if (ncell > (1L << 26)) throw Error(6, "cell grid too large");
+    const long S = (long)(2 * g.nc[0] + 1) * (2 * g.nc[1] + 1) * (2 * g.nc[2] + 1);
+    if (S > (1L << 22)) throw Error(6, "neighbour stencil too large");
+    int bits = 1;
+    while ((1L << bits) < ncell) ++bits;
+
+    launch_wrap_bin(g, w, s);
+    hipLaunchKernelGGL(k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, iota_.as<int>(), nat);
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                              w.cell_atoms, nat, 0, bits, s));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, w.count, w.offset, nat + 1, s));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
+                                              w.perm, nat, 0, 4, s));
+    cub_tmp_.ensure(std::max(b1, std::max(b2, b3)));
+    size_t bt = cub_tmp_.bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                              w.cell_atoms, nat, 0, bits, s));
+    cell_start_.ensure(4 * (size_t)ncell);
+    cell_end_.ensure(4 * (size_t)ncell);
+    w.cell_start = cell_start_.as<int>();
+    w.cell_end = cell_end_.as<int>();
+    launch_cell_range(g, w, (int)ncell, s);
+    HIPCHK(hipEventRecord(ev_[1], s));
+
+    HIPCHK(hipMemsetAsync(w.count + nat, 0, sizeof(int), s));
+    launch_nl_count(g, w, s);
+    bt = cub_tmp_.bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(cub_tmp_.p, bt, w.count, w.offset, nat + 1, s));
+    bt = cub_tmp_.bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
+                                              w.perm, nat, 0, 4, s));
+    launch_elem_bounds(w, nullptr, s);
+    HIPCHK(hipGetLastError());
+    // read back: total entries, element counts, flags
+    int *pi = (int *)pinned_;
+    HIPCHK(hipMemcpyAsync(pi, w.offset + nat, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pi + 1, w.elem_count, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pi + 17, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int total = pi[0];
+    const int *flags = pi + 17;
+    if (flags[0]) throw Error(4, "structure contains element Z=" + std::to_string(flags[0]) +
+                                     " that the potential does not describe");
+    for (int e = 0; e < dm_.nel; ++e)
+        if (pi[1 + e] > 0 && !elem_valid_[e]) throw Error(3, elem_why_[e]);
+    if (total < 0) throw Error(6, "neighbour list exceeds 2^31 entries");
+
+    ent_j_.ensure(4 * (size_t)std::max(total, 1));
+    ent_shift_.ensure(4 * (size_t)std::max(total, 1));
+    rev_.ensure(4 * (size_t)std::max(total, 1));
+    w.ent_j = ent_j_.as<int>();
+    w.ent_shift = ent_shift_.as<int>();
+    w.rev = rev_.as<int>();
+    launch_nl_fill(g, w, s);
+    launch_nl_rev(w, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_[2], s));
+
+    const int maxcnt = std::max(flags[1], 1);
+    const int nmax = (maxcnt + 63) / 64 * 64;
+    const bool fp64 = precision_ == 0;
+    if (sf_lds_bytes(fp64, nmax, dm_.nsfmax, true) > 65536)
+        throw Error(6, "an atom has " + std::to_string(maxcnt) + " neighbours; this build stages at most " +
+                           std::to_string(fp64 ? 768 : 1536) + " in LDS");
+    // the shift-overflow flag is raised by the fill kernel; checked below
+    if (fp64)
+        compute<double>(w, g, nmax, total, s);
+    else
+        compute<float>(w, g, nmax, total, s);
+    HIPCHK(hipEventRecord(ev_[6], s));
+    HIPCHK(hipMemcpyAsync(pin + 64, w.result, 10 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pi + 40, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (pi[40 + 2]) throw Error(6, "periodic image shift beyond +-127 cells");
+    last_nat_ = nat;
+    last_total_ = total;
+    for (int k = 0; k < 6; ++k) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
+        ms_[k] = ms;
+    }
+    const double *res = pin + 64;
+    *energy = res[0];
+    if (dEdLat) {
+        std::memset(dEdLat, 0, 9 * sizeof(double));
+        if (periodic) {    // dEdLat(a,x) = sum_b W(a,b) inv(lat)(x,b), stored at [3x + a]
+            double il[9];
+            inv3(lat, il);
+            for (int a = 0; a < 3; ++a)
+                for (int x = 0; x < 3; ++x) {
+                    double acc = 0;
+                    for (int b = 0; b < 3; ++b) acc += res[1 + 3 * a + b] * il[x + 3 * b];
+                    dEdLat[3 * x + a] = acc;
+                }
+        }
+    }
+}
+
+void Engine::evaluate_host(int nat, const int *Z, const double *pos, int periodic, const double *lat,
+                           double *energy, double *force, double *dEdLat)
+{
+    if (!dev_ready_) init_device();
+    if (nat > 0 && (!Z || !pos || !force)) throw Error(1, "null array");
+    zin_.ensure(4 * (size_t)std::max(nat, 1));
+    posin_.ensure(24 * (size_t)std::max(nat, 1));
+    fout_.ensure(24 * (size_t)std::max(nat, 1));
+    if (nat > 0) {
+        HIPCHK(hipMemcpyAsync(zin_.p, Z, 4 * (size_t)nat, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)nat, hipMemcpyHostToDevice, stream_));
+    }
+    run(nat, zin_.as<int>(), posin_.as<double>(), periodic, lat, fout_.as<double>(), stream_, energy, dEdLat);
+    if (nat > 0) {
+        HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)nat, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+}
+
+void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
+                             const double *lat, double *energy, double *force_dev, double *dEdLat,
+                             hipStream_t stream)
+{
+    if (!dev_ready_) init_device();
+    run(nat, Z_dev, pos_dev, periodic, lat, force_dev, stream ? stream : stream_, energy, dEdLat);
+}
+
+void Engine::get_neighbors(int64_t cap, int32_t *ent)
+{
+    if (cap < last_total_) throw Error(1, "neighbour buffer too small");
+    if (last_total_ == 0) return;
+    std::vector<int> off(last_nat_ + 1), j(last_total_), sh(last_total_);
+    HIPCHK(hipMemcpy(off.data(), offset_.p, 4 * (last_nat_ + 1), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(j.data(), ent_j_.p, 4 * last_total_, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sh.data(), ent_shift_.p, 4 * last_total_, hipMemcpyDeviceToHost));
+    for (int i = 0; i < last_nat_; ++i)
+        for (int q = off[i]; q < off[i + 1]; ++q) {
+            int x, y, z;
+            unpack_shift(sh[q], x, y, z);
+            ent[5 * (int64_t)q + 0] = i;
+            ent[5 * (int64_t)q + 1] = j[q];
+            ent[5 * (int64_t)q + 2] = x;
+            ent[5 * (int64_t)q + 3] = y;
+            ent[5 * (int64_t)q + 4] = z;
+        }
+}
+
+void Engine::get_symfunctions(int ld, double *G, double *dEdG, double *Ei)
+{
+    const int nat = last_nat_, ns = dm_.nsfmax;
+    if (nat == 0) return;
+    if ((G || dEdG) && ld < ns) throw Error(1, "ld smaller than the largest SF count");
+    const bool fp64 = precision_ == 0;
+    std::vector<double> tmp((size_t)nat * ns);
+    auto fetch = [&](const DevMem &src, double *dst) {
+        if (fp64) {
+            HIPCHK(hipMemcpy(tmp.data(), src.p, 8 * tmp.size(), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(tmp.size());
+            HIPCHK(hipMemcpy(t.data(), src.p, 4 * t.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < t.size(); ++k) tmp[k] = t[k];
+        }
+        for (int i = 0; i < nat; ++i)
+            for (int s = 0; s < ld; ++s) dst[(size_t)i * ld + s] = s < ns ? tmp[(size_t)i * ns + s] : 0.0;
+    };
+    if (G) fetch(G_, G);
+    if (dEdG) fetch(dEdG_, dEdG);
+    if (Ei) HIPCHK(hipMemcpy(Ei, Ei_.p, 8 * (size_t)nat, hipMemcpyDeviceToHost));
+}
+
+void Engine::timings(double *ms6) const
+{
+    for (int k = 0; k < 6; ++k) ms6[k] = ms_[k];
+}
+
+}  // namespace hdnnp

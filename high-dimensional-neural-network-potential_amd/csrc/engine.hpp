@@ -1,0 +1,81 @@
+// Evaluation engine: owns the device copy of one potential and the device
+// work buffers, and runs hdnnpEnergyAndForces on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "runner.hpp"
+
+namespace hdnnp {
+
+// grow-only device allocation
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b);
+    void release();
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+class Engine {
+  public:
+    Engine(HostModel m, int precision);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    const HostModel &model() const { return model_; }
+    int precision() const { return precision_; }
+
+    // host arrays in and out (hdnnp_evaluate)
+    void evaluate_host(int nat, const int *Z, const double *pos, int periodic, const double *lat,
+                       double *energy, double *force, double *dEdLat);
+    // device arrays Z, pos, force; lat/energy/dEdLat on the host
+    void evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
+                         const double *lat, double *energy, double *force_dev, double *dEdLat,
+                         hipStream_t stream);
+
+    int64_t neighbor_count() const { return last_total_; }
+    void get_neighbors(int64_t cap, int32_t *ent);
+    void get_symfunctions(int ld, double *G, double *dEdG, double *Ei);
+    void timings(double *ms6) const;
+
+  private:
+    void init_device();
+    void run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
+             double *force_dev, hipStream_t s, double *energy, double *dEdLat);
+    template <typename R>
+    void compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s);
+
+    HostModel model_;
+    int precision_;
+    bool dev_ready_ = false;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[7] = {};
+    double ms_[6] = {0, 0, 0, 0, 0, 0};
+    DevModel dm_{};
+    std::vector<DevMem> model_mem_;
+    bool elem_valid_[kMaxEl] = {};
+    std::string elem_why_[kMaxEl];
+    int mlp_maxw_ = 0, mlp_sumc_ = 0;
+
+    // work buffers
+    DevMem zin_, posin_, fout_;
+    DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
+    DevMem count_, offset_, perm_, eidx_sorted_, Ei_, vir_, misc_;
+    DevMem ent_j_, ent_shift_, rev_, gent_, G_, dEdG_, cub_tmp_;
+    void *pinned_ = nullptr;
+    int last_nat_ = 0;
+    int64_t last_total_ = 0;
+};
+
+// host copies of reference 3x3 routines (exact operation order)
+void inv3(const double A[9], double B[9]);
+double det3(const double A[9]);
+
+}  // namespace hdnnp

@@ -1,0 +1,89 @@
+// Force gather and the FP64 energy / virial reductions.
+//
+// Reference: the force scatter of hdnnpCalcNNs
+// (src/potentials/hdnnps.f90:533-538) and E = sum(atEns) (:557). Instead of
+// scattering with atomics, each atom gathers: its own entries' gradient rows
+// (the self slot, whose gradient is minus the sum of the others) and the
+// partner rows of the neighbours that see it, in a fixed order, so results are
+// reproducible bit for bit.
+#include "device.hpp"
+
+namespace hdnnp {
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_force_gather(Work w, const R *g)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= w.nat) return;
+    const int o = w.offset[m], n = w.count[m];
+    double fx = 0, fy = 0, fz = 0;
+    for (int q = o; q < o + n; ++q) {
+        fx += (double)g[3 * (size_t)q + 0];
+        fy += (double)g[3 * (size_t)q + 1];
+        fz += (double)g[3 * (size_t)q + 2];
+        const int r = w.rev[q];
+        if (r >= 0) {
+            fx -= (double)g[3 * (size_t)r + 0];
+            fy -= (double)g[3 * (size_t)r + 1];
+            fz -= (double)g[3 * (size_t)r + 2];
+        }
+    }
+    w.force[3 * m + 0] = fx;
+    w.force[3 * m + 1] = fy;
+    w.force[3 * m + 2] = fz;
+}
+
+// entries whose partner is missing (pair at the exact cutoff seen from one
+// side only): add their contribution to the neighbour directly
+template <typename R>
+__global__ __launch_bounds__(256) void k_orphans(Work w, const R *g, int nent)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nent || w.rev[q] >= 0) return;
+    const int j = w.ent_j[q];
+    atomicAdd(&w.force[3 * j + 0], -(double)g[3 * (size_t)q + 0]);
+    atomicAdd(&w.force[3 * j + 1], -(double)g[3 * (size_t)q + 1]);
+    atomicAdd(&w.force[3 * j + 2], -(double)g[3 * (size_t)q + 2]);
+}
+
+template <typename R>
+void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_force_gather<R>, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, gent);
+    if (nent > 0)
+        hipLaunchKernelGGL(k_orphans<R>, dim3((nent + 255) / 256), dim3(256), 0, s, w, gent, nent);
+}
+template void launch_force_gather_t<double>(Work &, const double *, int, hipStream_t);
+template void launch_force_gather_t<float>(Work &, const float *, int, hipStream_t);
+
+// result[0] = sum Ei, result[1..9] = sum vir (fixed order: thread t sums
+// atoms t, t+256, ..., then a tree over threads)
+__global__ __launch_bounds__(256) void k_reduce(Work w)
+{
+    __shared__ double red[10][256];
+    double a[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a[k] = 0;
+    for (int i = threadIdx.x; i < w.nat; i += 256) {
+        a[0] += w.Ei[i];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[1 + k] += w.vir[9 * (size_t)i + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) red[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < 10; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) w.result[threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_reduce(Work &w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, w);
+}
+
+}  // namespace hdnnp

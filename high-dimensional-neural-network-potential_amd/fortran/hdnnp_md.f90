@@ -1,0 +1,125 @@
+! Fortran driver of the MI355X HDNNP engine, written against the reference's
+! public interface (modules hdnnps / atomicStructure / constants).
+!
+!   hdnnp_md md   <paramdir/> <xyz> <momenta> <nsteps> <out> [mixed]
+!       velocity Verlet of hdnnpExample.f90:58-72 (dt = 0.5 fs), initial
+!       momenta read from a file (the reference draws unseeded random ones)
+!   hdnnp_md eval <paramdir/> <structure> <out> [mixed]
+!       one hdnnpEnergyAndForces call with dEdLat, plus dEdLat2Stress
+!
+! Structure file: "nat periodic" / three lattice-vector lines / "Z x y z" rows.
+program hdnnp_md
+    use precision
+    use constants
+    use atomicStructure
+    use hdnnps
+    implicit none
+    character(len=512) :: mode, pdir, a3, a4, a5, a6, a7
+    type(hdnnp) :: h
+    integer :: prec
+
+    call get_command_argument(1, mode)
+    call get_command_argument(2, pdir)
+    call get_command_argument(3, a3)
+    call get_command_argument(4, a4)
+    call get_command_argument(5, a5)
+    call get_command_argument(6, a6)
+    call get_command_argument(7, a7)
+    prec = HDNNP_FP64
+    select case (trim(mode))
+    case ('md')
+        if (trim(a7) == 'mixed') prec = HDNNP_MIXED
+        call parseRuNNerInput(h, trim(pdir), prec)
+        call runMD(h, trim(a3), trim(a4), trim(a5), trim(a6))
+    case ('eval')
+        if (trim(a5) == 'mixed') prec = HDNNP_MIXED
+        call parseRuNNerInput(h, trim(pdir), prec)
+        call runEval(h, trim(a3), trim(a4))
+    case default
+        print*, 'usage: hdnnp_md md|eval ...'
+        error stop 2
+    end select
+    call hdnnpDestroy(h)
+
+contains
+
+    subroutine runMD(h, xyz, pfile, nstr, fname)
+        type(hdnnp), intent(inout) :: h
+        character(len=*), intent(in) :: xyz, pfile, nstr, fname
+        type(atStruct) :: ats
+        real(dp), allocatable :: forces(:,:), m(:,:), p(:,:), lastForces(:,:)
+        real(dp), parameter :: stepsize = 0.5_dp * femtoseconds2atomic
+        real(dp) :: epot, ekin
+        integer :: i, u, nsteps
+
+        read(nstr, *) nsteps
+        call as_readXYZ(xyz, ats)
+        allocate(forces(3, ats%nat), lastForces(3, ats%nat), m(3, ats%nat), p(3, ats%nat))
+        m(1,:) = getAtomicMass(ats%el) * massUnit2ElectronMass
+        m(2,:) = m(1,:)
+        m(3,:) = m(1,:)
+        open(newunit=u, file=pfile, action='read', status='old')
+        do i = 1, ats%nat
+            read(u, *) p(:, i)
+        end do
+        close(u)
+
+        open(newunit=u, file=fname, action='write', status='replace')
+        call hdnnpEnergyAndForces(h, ats, epot, forces)
+        write(u, '(A,I6,ES25.16E3)') 'init ', -1, epot
+        do i = 0, nsteps - 1
+            ats%ats = ats%ats + p / m * stepSize + 0.5_dp * forces / m * stepSize**2
+            lastForces = forces
+            call hdnnpEnergyAndForces(h, ats, epot, forces)
+            p = p + stepSize / 2._dp * (forces + lastForces)
+            ekin = sum(p**2 / m) / 2._dp
+            write(u, '(A,I6,3ES25.16E3)') 'step ', i, epot, ekin, epot + ekin
+        end do
+        write(u, '(A)') 'positions'
+        do i = 1, ats%nat
+            write(u, '(3ES25.16E3)') ats%ats(:, i)
+        end do
+        close(u)
+    end subroutine runMD
+
+    subroutine runEval(h, sfile, fname)
+        type(hdnnp), intent(inout) :: h
+        character(len=*), intent(in) :: sfile, fname
+        type(atStruct) :: ats
+        real(dp) :: energy, dEdLat(3,3), stress(3,3)
+        real(dp), allocatable :: force(:,:)
+        integer :: u, i, iper
+
+        open(newunit=u, file=sfile, action='read', status='old')
+        read(u, *) ats%nat, iper
+        ats%periodic = (iper /= 0)
+        do i = 1, 3
+            read(u, *) ats%lat(:, i)
+        end do
+        allocate(ats%ats(3, ats%nat), ats%el(ats%nat), force(3, ats%nat))
+        do i = 1, ats%nat
+            read(u, *) ats%el(i), ats%ats(:, i)
+        end do
+        close(u)
+
+        call hdnnpEnergyAndForces(h, ats, energy, force, dEdLat)
+        stress = 0._dp
+        if (ats%periodic) call dEdLat2Stress(ats%lat, dEdLat, stress)
+        open(newunit=u, file=fname, action='write', status='replace')
+        write(u, '(A,ES25.16E3)') 'energy ', energy
+        write(u, '(A)') 'force'
+        do i = 1, ats%nat
+            write(u, '(3ES25.16E3)') force(:, i)
+        end do
+        write(u, '(A)') 'dEdLat'
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') dEdLat(i, :)
+        end do
+        write(u, '(A)') 'stress'
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') stress(i, :)
+        end do
+        close(u)
+    end subroutine runEval
+
+end program hdnnp_md

@@ -1,0 +1,124 @@
+/* hdnnp_capi.h -- C ABI of the MI355X HDNNP engine (libhdnnp_amd.so).
+ *
+ * The Fortran module `hdnnps` of this package binds these entry points with
+ * iso_c_binding; any C/C++/Python caller can use them directly.
+ *
+ * Conventions (those of the reference's atStruct, atomicStructure.f90:17-28):
+ *   pos[nat][3]   bohr, x1 y1 z1 x2 ...           (Fortran ats(3,nat))
+ *   lat[3][3]     row k = lattice vector k          (Fortran lat(:,k))
+ *   Z[nat]        atomic numbers, int32             (Fortran el(nat))
+ *   force[nat][3] Ha/bohr, same layout as pos
+ *   dEdLat[3][3]  dEdLat[k][a] = dE/d lat(a,k) ... i.e. the same 9 doubles
+ *                 as Fortran dEdLat(3,3) in memory order
+ *   energies in Hartree.
+ * Every entry point returns 0 on success and a non-zero status otherwise;
+ * hdnnp_last_error() then holds a message. Calls on one context must not
+ * overlap; separate contexts may be used concurrently.
+ */
+#ifndef HDNNP_CAPI_H
+#define HDNNP_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hdnnp_ctx hdnnp_ctx;
+
+/* arithmetic of an evaluation */
+enum {
+    HDNNP_FP64 = 0,   /* everything in FP64                                      */
+    HDNNP_MIXED = 1   /* wrap/list/energy/virial FP64, symmetry functions FP32,  */
+                      /* MLP FP32                                                */
+};
+
+/* status codes */
+enum {
+    HDNNP_OK = 0,
+    HDNNP_ERR_ARG = 1,      /* bad argument                                */
+    HDNNP_ERR_IO = 2,       /* parameter file missing or malformed         */
+    HDNNP_ERR_MODEL = 3,    /* model not usable (unsorted/mixed-rc SFs...) */
+    HDNNP_ERR_ELEMENT = 4,  /* structure holds an element the model lacks  */
+    HDNNP_ERR_DEVICE = 5,   /* HIP runtime error                           */
+    HDNNP_ERR_CAPACITY = 6  /* a size limit of this build was exceeded     */
+};
+
+/* Replaces parseRuNNerInput(handle, path) (src/potentials/hdnnps.f90:34-233):
+ * reads path/input.nn, path/scaling.data, path/weights.ZZZ.data (param_dir
+ * is the directory; a trailing '/' is optional). Parsing needs no GPU; device
+ * memory is set up on the first evaluation. */
+int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out);
+
+/* Replaces hdnnpEnergyAndForces(handle, ats, energy, force, dEdLat)
+ * (src/potentials/hdnnps.f90:333-371). Host arrays in, host arrays out.
+ * dEdLat may be NULL; for nonperiodic input it is set to zero (the reference
+ * leaves it undefined). */
+int hdnnp_evaluate(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *pos,
+                   int32_t periodic, const double *lat, double *energy, double *force,
+                   double *dEdLat);
+
+/* Same evaluation with Z, pos and force resident in device memory (HBM);
+ * lat, energy and dEdLat are host pointers. `stream` is a hipStream_t or
+ * NULL for the context's own stream; the call returns after the results are
+ * complete. No counterpart in the reference (which has no device). */
+int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev,
+                          const double *pos_dev, int32_t periodic, const double *lat,
+                          double *energy, double *force_dev, double *dEdLat, void *stream);
+
+/* Stage outputs of the last evaluation, for the reference's stage API:
+ * hdnnpCalcNeighborLists (hdnnps.f90:373-388) -> neighbour entries
+ * (i, j, Lx, Ly, Lz), 0-based atoms, L the integer image of j, self slot
+ * excluded, within the potential's largest cutoff. */
+int hdnnp_neighbor_count(hdnnp_ctx *ctx, int64_t *count);
+int hdnnp_get_neighbors(hdnnp_ctx *ctx, int64_t cap, int32_t *ent);
+
+/* hdnnpCalcSymfunctions (hdnnps.f90:390-485) and the dE/dG of hdnnpCalcNNs
+ * (hdnnps.f90:524-531) of the last evaluation: G[nat][ld] centred/scaled
+ * symmetry functions, dEdG[nat][ld] = dE_i/dG_s (w.r.t. the scaled G), Ei[nat]
+ * atomic energies (NN output + atom_energy). Any pointer may be NULL. */
+int hdnnp_get_symfunctions(hdnnp_ctx *ctx, int32_t ld, double *G, double *dEdG, double *Ei);
+
+typedef struct {
+    int32_t nelements;
+    int32_t Z[16];
+    int32_t nsf[16];
+    int32_t nlayers;        /* weight layers incl. output                */
+    int32_t nodes[8];       /* nOut per layer                           */
+    char act[8];            /* 'l' 't' 'p' 'r'                           */
+    int32_t scalesf, centersf;
+    double maxrc;
+    double atom_energy[16];
+    double emin, emax;      /* scaling.data energy range (parsed, unused) */
+} hdnnp_model_info;
+
+int hdnnp_get_model_info(hdnnp_ctx *ctx, hdnnp_model_info *info);
+
+/* Sorted symmetry-function table of element rank e (sortSymFunctions,
+ * hdnnps.f90:235-267; readScaling :269-300): rows of 11 doubles
+ * (type, Z1, Z2, rcut, eta, Rs, lambda, zeta, min, max, mean). */
+int hdnnp_get_sf_table(hdnnp_ctx *ctx, int32_t e, int32_t cap_rows, double *rows);
+
+/* Weights of element rank e in the reference's file order (readWeights,
+ * hdnnps.f90:318-327): per layer W(i,j) input-major, then the biases. */
+int hdnnp_get_weights(hdnnp_ctx *ctx, int32_t e, int64_t cap, double *w, int64_t *count);
+
+/* Replaces dEdLat2Stress(lat, dEdLat, stress) (atomicStructure.f90:679-686):
+ * stress = -1/det(lat) * dEdLat . lat^T in the reference's Fortran terms. */
+int hdnnp_dEdLat_to_stress(const double *lat, const double *dEdLat, double *stress);
+
+/* Message of the last failed call on ctx (or of the last failed create when
+ * ctx is NULL). Never NULL. */
+const char *hdnnp_last_error(const hdnnp_ctx *ctx);
+
+/* Timing of the last evaluation's stages in milliseconds (HIP events on the
+ * evaluation stream): [0] wrap+cells, [1] neighbour list, [2] symmetry
+ * functions, [3] MLP, [4] gradients, [5] force gather + reductions. */
+int hdnnp_last_timings(hdnnp_ctx *ctx, double *ms6);
+
+void hdnnp_destroy(hdnnp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
