@@ -376,6 +376,10 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
         for (int k = 0; k < 3; ++k) grid[3 * a + k] = v[k] / g.ng[a];
     }
     get_ncells(grid, rc, g.nc);                   // neighborLists.f90:169
+    // nonperiodic: offsets outside the grid are skipped anyway, so the stencil
+    // is clamped to the grid (same cells visited; flat inputs stay cheap)
+    if (!periodic)
+        for (int a = 0; a < 3; ++a) g.nc[a] = std::min(g.nc[a], g.ng[a] - 1);
     const long ncell = (long)g.ng[0] * g.ng[1] * g.ng[2];
     if (ncell > (1L << 26)) throw Error(6, "cell grid too large");
     const long S = (long)(2 * g.nc[0] + 1) * (2 * g.nc[1] + 1) * (2 * g.nc[2] + 1);

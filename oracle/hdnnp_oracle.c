@@ -227,6 +227,13 @@ static nlist *build_nl(int nat, const double *pos_in, int periodic, const double
         free(fill);
     }
     get_ncells(grid, rc, nc);                         /* :169 */
+    /* Without periodicity only offsets that land inside the grid matter, so
+     * the stencil is clamped to the grid: the same cells are visited, and a
+     * flat input (artificial box 2e-4 thick) no longer loops over ~1e10
+     * empty offsets. */
+    if (!periodic)
+        for (int a = 0; a < 3; ++a)
+            if (nc[a] > ng[a] - 1) nc[a] = ng[a] - 1;
     const double rc2 = rc * rc;
 
 #pragma omp parallel for schedule(dynamic, 16)

@@ -174,17 +174,29 @@ def read_potential(path: str) -> Potential:
             toks_lines = [ln.split() for ln in f.read().split("\n")]
     except OSError:
         raise RunnerError(path + "scaling.data not found")
-    toks_lines = [t for t in toks_lines if t]
     li = 0
+
+    def record(n):
+        """one list-directed READ of n values: it takes values from as many
+        lines as needed; the next READ starts on the following line"""
+        nonlocal li
+        out = []
+        while len(out) < n:
+            if li >= len(toks_lines):
+                raise RunnerError("unexpected end of scaling.data")
+            out.extend(toks_lines[li][:n - len(out)])
+            li += 1
+        return out
+
     for iel, z in enumerate(zs, start=1):
         for isf, sf in enumerate(sflists[z], start=1):
-            t = toks_lines[li]
-            li += 1
+            t = record(5)
             riel, risf = int(t[0]), int(t[1])
             sf.sfmin, sf.sfmax, sf.sfmean = _freal(t[2]), _freal(t[3]), _freal(t[4])
             if riel != iel or risf != isf:
                 raise RunnerError("wrong index reading scaling.data")
-    emin, emax = _freal(toks_lines[li][0]), _freal(toks_lines[li][1])
+    t = record(2)
+    emin, emax = _freal(t[0]), _freal(t[1])
     layers = {}
     for z in zs:                                              # readWeights :302-330
         fn = path + f"weights.{z:03d}.data"
