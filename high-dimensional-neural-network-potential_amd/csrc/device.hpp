@@ -117,6 +117,6 @@ size_t mlp_lds_bytes(bool fp64, int T, int sumc, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s);
-void launch_reduce(Work &w, hipStream_t s);
+void launch_reduce(Work &w, double *partial, hipStream_t s);   // partial: 2560 doubles
 
 }  // namespace hdnnp

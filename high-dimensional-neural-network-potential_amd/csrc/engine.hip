@@ -125,7 +125,7 @@ Engine::~Engine()
     for (auto &d : model_mem_) d.release();
     for (DevMem *d : {&zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
                       &iota_, &cell_start_, &cell_end_, &count_, &offset_, &perm_, &eidx_sorted_, &Ei_,
-                      &vir_, &misc_, &ent_j_, &ent_shift_, &rev_, &gent_, &G_, &dEdG_, &cub_tmp_})
+                      &vir_, &misc_, &ent_j_, &ent_shift_, &rev_, &gent_, &G_, &dEdG_, &cub_tmp_, &red_})
         d->release();
     if (pinned_) (void)hipHostFree(pinned_);
     for (auto &e : ev_)
@@ -287,7 +287,8 @@ void Engine::compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s)
     HIPCHK(hipEventRecord(ev_[5], s));
 
     launch_force_gather_t<R>(w, gent, total, s);
-    launch_reduce(w, s);
+    red_.ensure(8 * 2560);
+    launch_reduce(w, red_.as<double>(), s);
     HIPCHK(hipGetLastError());
 }
 
@@ -389,13 +390,14 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
 
     launch_wrap_bin(g, w, s);
     hipLaunchKernelGGL(k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, iota_.as<int>(), nat);
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, w.count, w.offset, nat + 1, s));
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
                                               w.perm, nat, 0, 4, s));
-    cub_tmp_.ensure(std::max(b1, std::max(b2, b3)));
+    HIPCHK(hipcub::DeviceReduce::Max(nullptr, b4, w.count, w.flags + 1, nat, s));
+    cub_tmp_.ensure(std::max(std::max(b1, b4), std::max(b2, b3)));
     size_t bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
@@ -411,9 +413,11 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(cub_tmp_.p, bt, w.count, w.offset, nat + 1, s));
     bt = cub_tmp_.bytes;
+    HIPCHK(hipcub::DeviceReduce::Max(cub_tmp_.p, bt, w.count, w.flags + 1, nat, s));
+    bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
                                               w.perm, nat, 0, 4, s));
-    launch_elem_bounds(w, nullptr, s);
+    launch_elem_bounds(w, eidx_sorted_.as<int>(), s);
     HIPCHK(hipGetLastError());
     // read back: total entries, element counts, flags
     int *pi = (int *)pinned_;

@@ -68,7 +68,7 @@ class Engine {
     DevMem zin_, posin_, fout_;
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
     DevMem count_, offset_, perm_, eidx_sorted_, Ei_, vir_, misc_;
-    DevMem ent_j_, ent_shift_, rev_, gent_, G_, dEdG_, cub_tmp_;
+    DevMem ent_j_, ent_shift_, rev_, gent_, G_, dEdG_, cub_tmp_, red_;
     void *pinned_ = nullptr;
     int last_nat_ = 0;
     int64_t last_total_ = 0;

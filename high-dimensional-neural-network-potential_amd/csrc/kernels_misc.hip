@@ -8,6 +8,8 @@
 // reproducible bit for bit.
 #include "device.hpp"
 
+#include <algorithm>
+
 namespace hdnnp {
 
 template <typename R>
@@ -56,19 +58,14 @@ void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s)
 template void launch_force_gather_t<double>(Work &, const double *, int, hipStream_t);
 template void launch_force_gather_t<float>(Work &, const float *, int, hipStream_t);
 
-// result[0] = sum Ei, result[1..9] = sum vir (fixed order: thread t sums
-// atoms t, t+256, ..., then a tree over threads)
-__global__ __launch_bounds__(256) void k_reduce(Work w)
+// Two-stage FP64 reduction, result[0] = sum Ei, result[1..9] = sum vir.
+// Fixed order for a given nat: block b sums a fixed contiguous chunk (thread
+// t takes every 256th atom of it, then a tree over threads); one block then
+// sums the block partials with the same tree.
+constexpr int kRedBlocks = 256;
+
+__device__ inline void tree10(double (*red)[256], double *a)
 {
-    __shared__ double red[10][256];
-    double a[10];
-#pragma unroll
-    for (int k = 0; k < 10; ++k) a[k] = 0;
-    for (int i = threadIdx.x; i < w.nat; i += 256) {
-        a[0] += w.Ei[i];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) a[1 + k] += w.vir[9 * (size_t)i + k];
-    }
 #pragma unroll
     for (int k = 0; k < 10; ++k) red[k][threadIdx.x] = a[k];
     __syncthreads();
@@ -78,12 +75,40 @@ __global__ __launch_bounds__(256) void k_reduce(Work w)
             for (int k = 0; k < 10; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partial(Work w, double *partial, int chunk)
+{
+    __shared__ double red[10][256];
+    double a[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a[k] = 0;
+    const int i0 = blockIdx.x * chunk, i1 = min(w.nat, i0 + chunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+        a[0] += w.Ei[i];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[1 + k] += w.vir[9 * (size_t)i + k];
+    }
+    tree10(red, a);
+    if (threadIdx.x < 10) partial[10 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(Work w, const double *partial, int nb)
+{
+    __shared__ double red[10][256];
+    double a[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a[k] = (int)threadIdx.x < nb ? partial[10 * threadIdx.x + k] : 0.0;
+    tree10(red, a);
     if (threadIdx.x < 10) w.result[threadIdx.x] = red[threadIdx.x][0];
 }
 
-void launch_reduce(Work &w, hipStream_t s)
+void launch_reduce(Work &w, double *partial, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, w);
+    const int nb = std::min(kRedBlocks, (w.nat + 255) / 256);
+    const int chunk = (w.nat + nb - 1) / nb;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, s, w, partial, chunk);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, w, (const double *)partial, nb);
 }
 
 }  // namespace hdnnp

@@ -54,22 +54,15 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
-// Element rank of each atom; per-element atom counts; unknown Z flagged.
+// Element rank of each atom; unknown Z flagged (rare, so the atomic is cheap).
 __global__ __launch_bounds__(256) void k_map_elements(DevModel m, Work w)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int e = -1;
-    if (i < w.nat) {
-        const int z = w.Z[i];
-        e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
-        if (e < 0) atomicMax(&w.flags[0], z > 0 ? z : 1);
-        w.eidx[i] = e < 0 ? 0 : e;
-    }
-    const int lane = threadIdx.x & 63;
-    for (int f = 0; f < m.nel; ++f) {
-        const unsigned long long b = __ballot(i < w.nat && e == f);
-        if (lane == 0 && b) atomicAdd(&w.elem_count[f], __popcll(b));
-    }
+    if (i >= w.nat) return;
+    const int z = w.Z[i];
+    const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
+    if (e < 0) atomicMax(&w.flags[0], z > 0 ? z : 1);
+    w.eidx[i] = e < 0 ? 0 : e;
 }
 
 void launch_map_elements(const DevModel &m, Work &w, hipStream_t s)
@@ -258,10 +251,7 @@ __global__ __launch_bounds__(256) void k_nl_count(Geo g, Work w)
         }
     }
     cnt = wave_sum_int(cnt);
-    if (lane == 0) {
-        w.count[i] = cnt;
-        atomicMax(&w.flags[1], cnt);
-    }
+    if (lane == 0) w.count[i] = cnt;
 }
 
 void launch_nl_count(const Geo &g, Work &w, hipStream_t s)
@@ -345,21 +335,29 @@ void launch_nl_rev(Work &w, hipStream_t s)
     hipLaunchKernelGGL(k_nl_rev, dim3((w.nat + 3) / 4), dim3(256), 0, s, w);
 }
 
-__global__ void k_elem_bounds(Work w, int nel)
+// Element segments of the element-sorted atom order: elem_off[e] = first
+// position whose element is >= e (binary search), elem_count = differences.
+__global__ void k_elem_bounds(Work w, const int *sorted)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int acc = 0;
-        for (int e = 0; e < nel; ++e) {
-            w.elem_off[e] = acc;
-            acc += w.elem_count[e];
+    __shared__ int off[kMaxEl + 1];
+    const int e = threadIdx.x;
+    if (e <= kMaxEl) {
+        int lo = 0, hi = w.nat;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sorted[mid] < e) lo = mid + 1;
+            else hi = mid;
         }
-        for (int e = nel; e <= kMaxEl; ++e) w.elem_off[e] = acc;
+        off[e] = lo;
+        w.elem_off[e] = lo;
     }
+    __syncthreads();
+    if (e < kMaxEl) w.elem_count[e] = off[e + 1] - off[e];
 }
 
-void launch_elem_bounds(Work &w, const int *, hipStream_t s)
+void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_elem_bounds, dim3(1), dim3(64), 0, s, w, kMaxEl);
+    hipLaunchKernelGGL(k_elem_bounds, dim3(1), dim3(64), 0, s, w, sorted_elem);
 }
 
 }  // namespace hdnnp
