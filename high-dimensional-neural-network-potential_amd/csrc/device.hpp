@@ -10,6 +10,16 @@ constexpr int kMaxEl = 16;       // elements per model (= kMaxElements)
 constexpr int kMaxLay = 8;       // weight layers
 constexpr int kNonIntegralZeta = -(1 << 30);   // g4_ipow marker
 
+// Per-element scalars, kept in device memory: kernels index them with the
+// (wave-uniform) element rank, which compiles to scalar loads. (Indexing an
+// array member of the by-value DevModel argument with a runtime index would
+// make the compiler copy the argument into private scratch memory.)
+struct ElemInfo {
+    int nsf, sf_off;
+    int g2_start, g2_count, g4_start, g4_count;
+    double rc, atom_energy;
+};
+
 // Model tables in device memory; the struct itself is a kernel argument.
 // G2 entries of centre element e with neighbour element f live at
 // g2_tab[e*kMaxEl+f] = (start, count); G4 entries of centre e with the
@@ -30,6 +40,15 @@ struct DevModel {
     const double *g4_eta, *g4_lam, *g4_zeta, *g4_pre;
     const int *g4_ipow;      // zeta-1 when zeta is integral, else kNonIntegralZeta
     const int *g4_neweta;
+    const int *g2_f;         // neighbour element rank of each G2 entry
+    // FP32 copies of the tables (the mixed path reads these, no conversions)
+    const float *g2f_eta, *g2f_rs, *g4f_eta, *g4f_lam, *g4f_zeta, *g4f_pre;
+    // the entries of centre element e are contiguous: [start, start+count)
+    int g2_start[kMaxEl], g2_count[kMaxEl];
+    int g4_start[kMaxEl], g4_count[kMaxEl];
+    int g4_max;              // largest g4_count
+    int powbits;             // bits of the largest |g4_ipow| of integral zeta
+    const ElemInfo *einfo;   // [nel]
     const double *sf_mean;   // [sf_off[e] + s]; 0 when not centring
     const double *sf_den;    // max - min, 1 when not scaling
     int nlayers;
@@ -108,12 +127,11 @@ void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, R *G, 
 template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, const R *dEdG, R *gent,
                     hipStream_t s);
-size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, bool grad);
-// ---- kernels_mlp.hip: T atoms per work-group, maxw = widest layer/input ----
+size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad);
+// ---- kernels_mlp.hip: 16 atoms per wavefront, maxw = widest layer/input ----
 template <typename R>
-void launch_mlp_t(const DevModel &m, Work &w, int T, int maxw, int nblocks, const R *G, R *dEdG,
-                  hipStream_t s);
-size_t mlp_lds_bytes(bool fp64, int T, int sumc, int maxw);
+void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, hipStream_t s);
+size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s);

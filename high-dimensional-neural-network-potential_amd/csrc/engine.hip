@@ -214,6 +214,48 @@ void Engine::init_device()
                 g4tab[(e * kMaxEl + f) * kMaxEl + g] = make_int2(start, (int)g4s.size() - start);
             }
     }
+    {   // per-element ranges, neighbour element of each G2 entry, power bits
+        std::vector<int> g2f(g2s.size(), 0);
+        for (int e = 0; e < d.nel; ++e) {
+            d.g2_start[e] = g2tab[e * kMaxEl].x;
+            d.g4_start[e] = g4tab[(e * kMaxEl) * kMaxEl].x;
+            int n2 = 0, n4 = 0;
+            for (int f = 0; f < d.nel; ++f) {
+                const int2 t = g2tab[e * kMaxEl + f];
+                for (int k = 0; k < t.y; ++k) g2f[t.x + k] = f;
+                n2 += t.y;
+                for (int g = f; g < d.nel; ++g) n4 += g4tab[(e * kMaxEl + f) * kMaxEl + g].y;
+            }
+            d.g2_count[e] = n2;
+            d.g4_count[e] = n4;
+            d.g4_max = std::max(d.g4_max, n4);
+        }
+        d.g2_f = upload(model_mem_, g2f);
+        int maxp = 0;
+        for (int ip : g4ipow)
+            if (ip != kNonIntegralZeta) maxp = std::max(maxp, std::abs(ip));
+        d.powbits = 0;
+        while ((1 << d.powbits) <= maxp) ++d.powbits;
+        auto tof = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+        d.g2f_eta = upload(model_mem_, tof(g2eta));
+        d.g2f_rs = upload(model_mem_, tof(g2rs));
+        d.g4f_eta = upload(model_mem_, tof(g4eta));
+        d.g4f_lam = upload(model_mem_, tof(g4lam));
+        d.g4f_zeta = upload(model_mem_, tof(g4zeta));
+        d.g4f_pre = upload(model_mem_, tof(g4pre));
+        std::vector<ElemInfo> ei(std::max(d.nel, 1));
+        for (int e = 0; e < d.nel; ++e) {
+            ei[e].nsf = d.nsf[e];
+            ei[e].sf_off = d.sf_off[e];
+            ei[e].g2_start = d.g2_start[e];
+            ei[e].g2_count = d.g2_count[e];
+            ei[e].g4_start = d.g4_start[e];
+            ei[e].g4_count = d.g4_count[e];
+            ei[e].rc = d.rc[e];
+            ei[e].atom_energy = d.atom_energy[e];
+        }
+        d.einfo = upload(model_mem_, ei);
+    }
     d.zlut = upload(model_mem_, zlut);
     d.sf_mean = upload(model_mem_, mean);
     d.sf_den = upload(model_mem_, den);
@@ -272,13 +314,10 @@ void Engine::compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s)
     HIPCHK(hipEventRecord(ev_[3], s));
 
     const int *cnt = (const int *)pinned_ + 1;   // element counts (read back)
-    const bool fp64 = sizeof(R) == 8;
-    int T = 64;
-    while (T > 1 && mlp_lds_bytes(fp64, T, mlp_sumc_, mlp_maxw_) > 65536) T /= 2;
-    if (mlp_lds_bytes(fp64, T, mlp_sumc_, mlp_maxw_) > 65536) throw Error(6, "MLP too wide for this build");
+    if (mlp_lds_bytes(sizeof(R) == 8, dm_, mlp_maxw_) > 65536) throw Error(6, "MLP too wide for this build");
     int nblocks = 0;
-    for (int e = 0; e < dm_.nel; ++e) nblocks += (cnt[e] + T - 1) / T;
-    if (nblocks > 0) launch_mlp_t<R>(dm_, w, T, mlp_maxw_, nblocks, G, dEdG, s);
+    for (int e = 0; e < dm_.nel; ++e) nblocks += (cnt[e] + 15) / 16;
+    if (nblocks > 0) launch_mlp_t<R>(dm_, w, mlp_maxw_, nblocks, G, dEdG, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[4], s));
 
@@ -447,9 +486,12 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     const int maxcnt = std::max(flags[1], 1);
     const int nmax = (maxcnt + 63) / 64 * 64;
     const bool fp64 = precision_ == 0;
-    if (sf_lds_bytes(fp64, nmax, dm_.nsfmax, true) > 65536)
+    if (nmax > 1024 || sf_lds_bytes(fp64, nmax, dm_.nsfmax, dm_.g4_max, true) > 65536) {
+        int lim = 64;
+        while (lim + 64 <= 1024 && sf_lds_bytes(fp64, lim + 64, dm_.nsfmax, dm_.g4_max, true) <= 65536) lim += 64;
         throw Error(6, "an atom has " + std::to_string(maxcnt) + " neighbours; this build stages at most " +
-                           std::to_string(fp64 ? 768 : 1536) + " in LDS");
+                           std::to_string(lim) + " per atom in LDS");
+    }
     // the shift-overflow flag is raised by the fill kernel; checked below
     if (fp64)
         compute<double>(w, g, nmax, total, s);

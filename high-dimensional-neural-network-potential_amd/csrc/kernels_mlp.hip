@@ -1,19 +1,43 @@
-// Per-element MLP: forward pass and backpropagation of dE/dG.
+// Per-element MLP on the matrix cores: forward pass and backpropagation of
+// dE/dG for tiles of 16 atoms of one element.
 //
 // Reference: nn_forward / layer_forward and nn_backward / layer_backward
 // (src/neuralnetwork/neuralNetworks.f90:94-158, 220-313) with activations
 // activate / deactivate (:33-84), called per atom by hdnnpCalcNNs
 // (src/potentials/hdnnps.f90:523-531) with gradOut = 1.
 //
-// One work-group runs a tile of T atoms of one element (atoms grouped by
-// element through `perm`). Layer activations of the tile live in LDS: the
-// pre-activations c_l of every layer (needed by the backward pass) and two
-// ping-pong buffers for the running activation / gradient.
+// One wavefront per tile. A layer is C[16 x nout] = X[16 x nin] . W[nin x nout]
+// computed with v_mfma_{f32,f64}_16x16x4 (FP32 / FP64 accumulation), 16 output
+// columns per accumulator; the backward pass is G[16 x nin] = D[16 x nout] .
+// W^T. Operand layout of the 16x16x4 instructions (wave64): lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; it holds D[4*(l/16)+r][l%16] (f32) or
+// D[(l/16)+4r][l%16] (f64), r = 0..3 (checked on gfx950 with an in-bounds
+// probe of both instructions).
+// The tile's activations and pre-activations stay in LDS (row stride odd, so
+// the 16 rows of one column fall in distinct banks).
 #include "device.hpp"
 
 namespace hdnnp {
 
-__device__ inline double act_f(char a, double x)
+template <typename R> struct Mma;
+template <> struct Mma<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V op(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
+};
+template <> struct Mma<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V op(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kq, int r) { return kq + 4 * r; }
+};
+
+__device__ __forceinline__ double act_f(char a, double x)
 {
     switch (a) {
     case 't': return tanh(x);
@@ -22,7 +46,7 @@ __device__ inline double act_f(char a, double x)
     default: return x;
     }
 }
-__device__ inline double act_d(char a, double x)
+__device__ __forceinline__ double act_d(char a, double x)
 {
     switch (a) {
     case 't': { const double t = tanh(x); return 1.0 - t * t; }
@@ -31,7 +55,7 @@ __device__ inline double act_d(char a, double x)
     default: return 1.0;
     }
 }
-__device__ inline float act_f(char a, float x)
+__device__ __forceinline__ float act_f(char a, float x)
 {
     switch (a) {
     case 't': return tanhf(x);
@@ -40,7 +64,7 @@ __device__ inline float act_f(char a, float x)
     default: return x;
     }
 }
-__device__ inline float act_d(char a, float x)
+__device__ __forceinline__ float act_d(char a, float x)
 {
     switch (a) {
     case 't': { const float t = tanhf(x); return 1.0f - t * t; }
@@ -50,15 +74,28 @@ __device__ inline float act_d(char a, float x)
     }
 }
 
-template <typename R> __device__ inline const R *weights(const DevModel &m, int e);
-template <> __device__ inline const double *weights<double>(const DevModel &m, int e) { return m.W64[e]; }
-template <> __device__ inline const float *weights<float>(const DevModel &m, int e) { return m.W32[e]; }
+template <typename R> __device__ __forceinline__ const R *weights(const DevModel &m, int e);
+template <> __device__ __forceinline__ const double *weights<double>(const DevModel &m, int e) { return m.W64[e]; }
+template <> __device__ __forceinline__ const float *weights<float>(const DevModel &m, int e) { return m.W32[e]; }
+
+// Layer geometry passed by value: nodes, activations and LDS offsets of the
+// pre-activations (only indexed with loop counters the compiler unrolls).
+struct MlpShape {
+    int nlayers;
+    int nodes[kMaxLay];
+    char act[kMaxLay];
+    int ldc[kMaxLay];       // row stride of the pre-activations of layer l
+    int coff[kMaxLay];      // their offset (elements) in LDS
+    int ld;                 // row stride of the two activation buffers
+    int total;              // LDS elements
+};
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_mlp(DevModel m, Work w, int T, int maxw, const R *G, R *dEdG)
+__global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, const R *G, R *dEdG)
 {
     extern __shared__ __align__(16) char smem[];
-    // which (element, tile) this work-group runs
+    typedef typename Mma<R>::V V;
+    constexpr int T = 16;
     int e = -1, t0 = 0, tn = 0;
     {
         int acc = 0;
@@ -76,103 +113,136 @@ __global__ __launch_bounds__(256) void k_mlp(DevModel m, Work w, int T, int maxw
         }
     }
     if (e < 0) return;
-    const int nsf = m.nsf[e];
-    const int ld = m.nsfmax;
-    int sumc = 0;
-    for (int l = 0; l < m.nlayers; ++l) sumc += m.nodes[l];
-    R *cbuf = (R *)smem;                 // [sum nodes][T]: c_l stored at layer offset
-    R *P0 = cbuf + (size_t)sumc * T;     // [T][maxw]
-    R *P1 = P0 + (size_t)T * maxw;
+    const int lane = threadIdx.x;
+    const int ri = lane & 15, kq = lane >> 4;
+    const int nsf = m.einfo[e].nsf;
+    const int ldg = m.nsfmax;
+    const int ld = sh.ld;
+    R *P0 = (R *)smem;
+    R *P1 = P0 + T * ld;
+    R *Cb = P1 + T * ld;
     const R *W = weights<R>(m, e);
 
-    for (int t = threadIdx.x; t < T * nsf; t += blockDim.x) {
-        const int a = t / nsf, s = t - a * nsf;
-        P0[a * maxw + s] = a < tn ? G[(size_t)w.perm[t0 + a] * ld + s] : R(0);
+    for (int t = lane; t < T * ld; t += 64) {
+        const int a = t / ld, s = t - a * ld;
+        P0[t] = (a < tn && s < nsf) ? G[(size_t)w.perm[t0 + a] * ldg + s] : R(0);
     }
     __syncthreads();
 
     // forward, layer_forward (neuralNetworks.f90:94-114): c = W x + b, y = act(c)
     const R *Wl[kMaxLay];
-    int nin = nsf, coff = 0;
     R *in = P0, *out = P1;
+    int nin = nsf;
     const R *wp = W;
-    for (int l = 0; l < m.nlayers; ++l) {
-        const int nout = m.nodes[l];
+#pragma unroll
+    for (int l = 0; l < kMaxLay; ++l) {
+        if (l >= sh.nlayers) break;
+        const int nout = sh.nodes[l];
         Wl[l] = wp;
         const R *bias = wp + (size_t)nin * nout;
-        R *c = cbuf + (size_t)coff * T;
-        const char act = m.act[l];
-        for (int t = threadIdx.x; t < T * nout; t += blockDim.x) {
-            const int a = t / nout, o = t - a * nout;
-            R acc = 0;
-            for (int j = 0; j < nin; ++j) acc += wp[j * nout + o] * in[a * maxw + j];
-            const R cv = acc + bias[o];
-            c[a * nout + o] = cv;
-            out[a * maxw + o] = act_f(act, cv);
+        R *c = Cb + sh.coff[l];
+        const int ldc = sh.ldc[l];
+        const char act = sh.act[l];
+        for (int j0 = 0; j0 < nout; j0 += 16) {
+            V acc = {0, 0, 0, 0};
+            const int j = j0 + ri;
+            for (int k0 = 0; k0 < nin; k0 += 4) {
+                const int k = k0 + kq;
+                const R a = k < nin ? in[ri * ld + k] : R(0);
+                const R b = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                acc = Mma<R>::op(a, b, acc);
+            }
+            if (j < nout) {
+                const R bj = bias[j];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = Mma<R>::row(kq, r);
+                    const R cv = acc[r] + bj;
+                    c[row * ldc + j] = cv;
+                    out[row * ld + j] = act_f(act, cv);
+                }
+            }
         }
         __syncthreads();
         wp = bias + nout;
-        coff += nout;
         nin = nout;
         R *tmp = in;
         in = out;
         out = tmp;
     }
     // atomic energy (hdnnps.f90:526); the output layer has one node
-    for (int a = threadIdx.x; a < tn; a += blockDim.x)
-        w.Ei[w.perm[t0 + a]] = (double)in[a * maxw] + m.atom_energy[e];
-    __syncthreads();
+    if (lane < tn) w.Ei[w.perm[t0 + lane]] = (double)in[lane * ld] + m.einfo[e].atom_energy;
 
     // backward, layer_backward (:116-158) with gradOut = 1
     R *g = out;
-    for (int a = threadIdx.x; a < T; a += blockDim.x) g[a * maxw] = 1;
+    if (lane < T) g[lane * ld] = 1;
     __syncthreads();
-    for (int l = m.nlayers - 1; l >= 0; --l) {
-        const int nout = m.nodes[l];
-        const int ni = l == 0 ? nsf : m.nodes[l - 1];
-        coff -= nout;
-        const R *c = cbuf + (size_t)coff * T;
-        const char act = m.act[l];
-        for (int t = threadIdx.x; t < T * nout; t += blockDim.x) {
+#pragma unroll
+    for (int l = kMaxLay - 1; l >= 0; --l) {
+        if (l >= sh.nlayers) continue;
+        const int nout = sh.nodes[l];
+        const int ni = l == 0 ? nsf : sh.nodes[l > 0 ? l - 1 : 0];
+        const R *c = Cb + sh.coff[l];
+        const int ldc = sh.ldc[l];
+        const char act = sh.act[l];
+        for (int t = lane; t < T * nout; t += 64) {
             const int a = t / nout, o = t - a * nout;
-            g[a * maxw + o] *= act_d(act, c[a * nout + o]);
+            g[a * ld + o] *= act_d(act, c[a * ldc + o]);
         }
         __syncthreads();
         R *gn = (g == P0) ? P1 : P0;
         const R *wl = Wl[l];
-        for (int t = threadIdx.x; t < T * ni; t += blockDim.x) {
-            const int a = t / ni, j = t - a * ni;
-            R acc = 0;
-            for (int o = 0; o < nout; ++o) acc += g[a * maxw + o] * wl[j * nout + o];
-            gn[a * maxw + j] = acc;
+        for (int j0 = 0; j0 < ni; j0 += 16) {
+            V acc = {0, 0, 0, 0};
+            const int j = j0 + ri;
+            for (int k0 = 0; k0 < nout; k0 += 4) {
+                const int k = k0 + kq;
+                const R a = k < nout ? g[ri * ld + k] : R(0);
+                const R b = (k < nout && j < ni) ? wl[(size_t)j * nout + k] : R(0);
+                acc = Mma<R>::op(a, b, acc);
+            }
+            if (j < ni) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) gn[Mma<R>::row(kq, r) * ld + j] = acc[r];
+            }
         }
         __syncthreads();
         g = gn;
     }
-    for (int t = threadIdx.x; t < T * ld; t += blockDim.x) {
-        const int a = t / ld, s = t - a * ld;
-        if (a < tn) dEdG[(size_t)w.perm[t0 + a] * ld + s] = s < nsf ? g[a * maxw + s] : R(0);
+    for (int t = lane; t < T * ldg; t += 64) {
+        const int a = t / ldg, s = t - a * ldg;
+        if (a < tn) dEdG[(size_t)w.perm[t0 + a] * ldg + s] = s < nsf ? g[a * ld + s] : R(0);
     }
 }
 
-size_t mlp_lds_bytes(bool fp64, int T, int sumc, int maxw)
+static int odd(int x) { return x | 1; }
+
+size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw)
 {
-    return (fp64 ? 8 : 4) * ((size_t)sumc * T + 2 * (size_t)T * maxw);
+    size_t tot = 2 * 16 * (size_t)odd(maxw);
+    for (int l = 0; l < m.nlayers; ++l) tot += 16 * (size_t)odd(m.nodes[l]);
+    return tot * (fp64 ? 8 : 4);
 }
 
 template <typename R>
-void launch_mlp_t(const DevModel &m, Work &w, int T, int maxw, int nblocks, const R *G, R *dEdG,
-                  hipStream_t s)
+void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, hipStream_t s)
 {
-    int sumc = 0;
-    for (int l = 0; l < m.nlayers; ++l) sumc += m.nodes[l];
-    const size_t sh = mlp_lds_bytes(sizeof(R) == 8, T, sumc, maxw);
-    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(256), sh, s, m, w, T, maxw, G, dEdG);
+    MlpShape sh{};
+    sh.nlayers = m.nlayers;
+    sh.ld = odd(maxw);
+    int off = 0;
+    for (int l = 0; l < m.nlayers; ++l) {
+        sh.nodes[l] = m.nodes[l];
+        sh.act[l] = m.act[l];
+        sh.ldc[l] = odd(m.nodes[l]);
+        sh.coff[l] = off;
+        off += 16 * sh.ldc[l];
+    }
+    sh.total = 2 * 16 * sh.ld + off;
+    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total, s, m, w, sh, G, dEdG);
 }
 
-template void launch_mlp_t<double>(const DevModel &, Work &, int, int, int, const double *, double *,
-                                   hipStream_t);
-template void launch_mlp_t<float>(const DevModel &, Work &, int, int, int, const float *, float *,
-                                  hipStream_t);
+template void launch_mlp_t<double>(const DevModel &, Work &, int, int, const double *, double *, hipStream_t);
+template void launch_mlp_t<float>(const DevModel &, Work &, int, int, const float *, float *, hipStream_t);
 
 }  // namespace hdnnp
