@@ -57,7 +57,8 @@ struct DevModel {
     const double *W64[kMaxEl];   // per layer: W(o,j) at [j*nout + o], then bias[nout]
     const float *W32[kMaxEl];
     double atom_energy[kMaxEl];
-    const int *zlut;             // Z (0..127) -> element rank, -1 if absent
+    const int *zlut;             // Z (0..127) -> element rank; -1 absent, -2-rank unusable
+    int fallback_el;             // usable element given to flagged atoms (results discarded)
 };
 
 // Cell grid of one evaluation (host-computed, reference formulas).
@@ -83,21 +84,40 @@ struct Work {
     int *cell_atoms;     // atoms sorted by cell
     int *cellkey_sorted;
     int *cell_start, *cell_end;
-    int *count;          // [nat+1] neighbours per atom
-    int *offset;         // [nat+1] exclusive scan of count
-    int *ent_j;          // [E] neighbour atom
-    int *ent_shift;      // [E] packed image shift (3 x int8)
-    int *rev;            // [E] partner entry in the neighbour's list, -1 if none
+    // Neighbour entries of atom i occupy slots [i*cap, i*cap + count[i]) of
+    // the entry arrays (cap a power of two, fixed for one evaluation), in the
+    // reference's list order: stencil cell, then ascending atom index.
+    int cap, cap_log2;
+    int *count;          // [nat] neighbours per atom (at most cap)
+    int *ent_j;          // [nat*cap] neighbour atom
+    int *ent_shift;      // [nat*cap] packed image shift (3 x int8)
+    int *ent_s;          // [nat*cap] stencil-cell index of the entry
+    int *rev;            // [nat*cap] partner entry in the neighbour's list, -1 if none
+    int nstencil;        // stencil cells per grid cell, S
+    unsigned short *pos_start;   // [nat*S] first list slot of each stencil cell, or null
     int *perm;           // atoms sorted by element
     int *elem_off;       // [kMaxEl+1] start of each element in perm
     int *elem_count;     // [kMaxEl]
-    int *flags;          // [8]: 0 unknown element, 1 max neighbours, 2 shift overflow
+    int *flags;          // [8]: 0 unknown Z, 1 neighbours needed when above cap,
+                         //      2 image-shift overflow, 3 Z of an unusable element
     double *Ei;          // [nat]
     double *vir;         // [nat][9]
     double *force;       // [nat][3]
     double *result;      // [16]: energy, W[9]
     double *bbox;        // [6]
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2. xcd_block maps b to a logical block so
+// that every XCD works through one contiguous range of logical blocks, and
+// blocks that share data (neighbouring atoms) share an L2. A bijection on
+// [0, nb) for any nb.
+__device__ inline int xcd_block(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
 
 __host__ __device__ inline int pack_shift(int x, int y, int z)
 {
@@ -115,8 +135,7 @@ void launch_map_elements(const DevModel &m, Work &w, hipStream_t s);
 void launch_bbox(Work &w, hipStream_t s);
 void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s);
 void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s);
-void launch_nl_count(const Geo &g, Work &w, hipStream_t s);
-void launch_nl_fill(const Geo &g, Work &w, hipStream_t s);
+void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_rev(Work &w, hipStream_t s);
 void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s);
 void launch_fill_int(int *p, int v, long n, hipStream_t s);
@@ -134,7 +153,7 @@ void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G,
 size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>
-void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s);
+void launch_force_gather_t(Work &w, const R *gent, hipStream_t s);
 void launch_reduce(Work &w, double *partial, hipStream_t s);   // partial: 2560 doubles
 
 }  // namespace hdnnp

@@ -124,8 +124,8 @@ Engine::~Engine()
 {
     for (auto &d : model_mem_) d.release();
     for (DevMem *d : {&zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
-                      &iota_, &cell_start_, &cell_end_, &count_, &offset_, &perm_, &eidx_sorted_, &Ei_,
-                      &vir_, &misc_, &ent_j_, &ent_shift_, &rev_, &gent_, &G_, &dEdG_, &cub_tmp_, &red_})
+                      &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
+                      &vir_, &misc_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &red_})
         d->release();
     if (pinned_) (void)hipHostFree(pinned_);
     for (auto &e : ev_)
@@ -153,6 +153,7 @@ void Engine::init_device()
     const HostModel &m = model_;
     DevModel d{};
     d.nel = (int)m.el.size();
+    d.fallback_el = -1;
     std::vector<int> zlut(128, -1);
     std::vector<double> mean, den;
     std::vector<int2> g2tab(kMaxEl * kMaxEl, make_int2(0, 0)), g4tab(kMaxEl * kMaxEl * kMaxEl, make_int2(0, 0));
@@ -161,7 +162,8 @@ void Engine::init_device()
     int off = 0;
     for (int e = 0; e < d.nel; ++e) {
         const ElementModel &em = m.el[e];
-        if (em.Z < 128) zlut[em.Z] = e;
+        if (em.Z < 128) zlut[em.Z] = em.valid ? e : -2 - e;
+        if (em.valid && d.fallback_el < 0) d.fallback_el = e;
         d.nsf[e] = (int)em.sfs.size();
         d.sf_off[e] = off;
         off += d.nsf[e];
@@ -298,46 +300,50 @@ void Engine::init_device()
 // misc buffer layout
 static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // ints
 static constexpr size_t kDblOffset = 256;                                    // bytes
+// largest per-(atom, stencil cell) slot table; beyond it partners are found by bisection
+static constexpr size_t kPosStartBytes = size_t(1) << 30;
 static constexpr int kDblResult = 0, kDblBbox = 16;                          // doubles
 
 template <typename R>
-void Engine::compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s)
+void Engine::compute(Work &w, const Geo &g, hipStream_t s)
 {
     const int nat = w.nat;
     G_.ensure(sizeof(R) * (size_t)nat * dm_.nsfmax);
     dEdG_.ensure(sizeof(R) * (size_t)nat * dm_.nsfmax);
-    gent_.ensure(sizeof(R) * 3 * (size_t)std::max(total, 1));
+    gent_.ensure(sizeof(R) * 3 * (size_t)nat * w.cap);
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
-    launch_sf_values<R>(dm_, g, w, nmax, G, s);
+    launch_sf_values<R>(dm_, g, w, w.cap, G, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[3], s));
 
-    const int *cnt = (const int *)pinned_ + 1;   // element counts (read back)
+    // one block per 16-atom tile of one element; the element counts stay on
+    // the device, so the grid covers the most tiles any split can need and
+    // surplus blocks exit at once
     if (mlp_lds_bytes(sizeof(R) == 8, dm_, mlp_maxw_) > 65536) throw Error(6, "MLP too wide for this build");
-    int nblocks = 0;
-    for (int e = 0; e < dm_.nel; ++e) nblocks += (cnt[e] + 15) / 16;
-    if (nblocks > 0) launch_mlp_t<R>(dm_, w, mlp_maxw_, nblocks, G, dEdG, s);
+    launch_mlp_t<R>(dm_, w, mlp_maxw_, (nat + 15) / 16 + dm_.nel, G, dEdG, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[4], s));
 
-    launch_sf_grad<R>(dm_, g, w, nmax, dEdG, gent, s);
+    launch_sf_grad<R>(dm_, g, w, w.cap, dEdG, gent, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[5], s));
 
-    launch_force_gather_t<R>(w, gent, total, s);
+    launch_force_gather_t<R>(w, gent, s);
     red_.ensure(8 * 2560);
     launch_reduce(w, red_.as<double>(), s);
     HIPCHK(hipGetLastError());
 }
 
 void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
-                 double *force_dev, hipStream_t s, double *energy, double *dEdLat)
+                 double *force_dev, hipStream_t s, double *energy, double *dEdLat, int retry)
 {
     if (nat < 0) throw Error(1, "nat < 0");
     if (periodic && !lat) throw Error(1, "periodic structure without lattice");
     const double rc = model_.maxrc;
     if (!(rc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
+    if (dm_.fallback_el < 0) throw Error(3, elem_why_[0].empty() ? "no element of the potential can be evaluated"
+                                                                 : elem_why_[0]);
     if (nat == 0) {
         *energy = 0;
         if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double));
@@ -353,8 +359,7 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     cellkey_sorted_.ensure(4 * (size_t)nat);
     cell_atoms_.ensure(4 * (size_t)nat);
     iota_.ensure(4 * (size_t)nat);
-    count_.ensure(4 * ((size_t)nat + 1));
-    offset_.ensure(4 * ((size_t)nat + 1));
+    count_.ensure(4 * (size_t)nat);
     perm_.ensure(4 * (size_t)nat);
     eidx_sorted_.ensure(4 * (size_t)nat);
     Ei_.ensure(8 * (size_t)nat);
@@ -372,7 +377,6 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     w.cellkey_sorted = cellkey_sorted_.as<int>();
     w.cell_atoms = cell_atoms_.as<int>();
     w.count = count_.as<int>();
-    w.offset = offset_.as<int>();
     w.perm = perm_.as<int>();
     w.elem_count = mi + kIntElemCount;
     w.elem_off = mi + kIntElemOff;
@@ -429,14 +433,12 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
 
     launch_wrap_bin(g, w, s);
     hipLaunchKernelGGL(k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, iota_.as<int>(), nat);
-    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    size_t b1 = 0, b3 = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, w.count, w.offset, nat + 1, s));
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
                                               w.perm, nat, 0, 4, s));
-    HIPCHK(hipcub::DeviceReduce::Max(nullptr, b4, w.count, w.flags + 1, nat, s));
-    cub_tmp_.ensure(std::max(std::max(b1, b4), std::max(b2, b3)));
+    cub_tmp_.ensure(std::max(b1, b3));
     size_t bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
@@ -445,65 +447,85 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
     launch_cell_range(g, w, (int)ncell, s);
-    HIPCHK(hipEventRecord(ev_[1], s));
-
-    HIPCHK(hipMemsetAsync(w.count + nat, 0, sizeof(int), s));
-    launch_nl_count(g, w, s);
-    bt = cub_tmp_.bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(cub_tmp_.p, bt, w.count, w.offset, nat + 1, s));
-    bt = cub_tmp_.bytes;
-    HIPCHK(hipcub::DeviceReduce::Max(cub_tmp_.p, bt, w.count, w.flags + 1, nat, s));
+    // atoms sorted by element (MLP tiles); needs no host round trip
     bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
                                               w.perm, nat, 0, 4, s));
     launch_elem_bounds(w, eidx_sorted_.as<int>(), s);
     HIPCHK(hipGetLastError());
-    // read back: total entries, element counts, flags
-    int *pi = (int *)pinned_;
-    HIPCHK(hipMemcpyAsync(pi, w.offset + nat, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(pi + 1, w.elem_count, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(pi + 17, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const int total = pi[0];
-    const int *flags = pi + 17;
-    if (flags[0]) throw Error(4, "structure contains element Z=" + std::to_string(flags[0]) +
-                                     " that the potential does not describe");
-    for (int e = 0; e < dm_.nel; ++e)
-        if (pi[1 + e] > 0 && !elem_valid_[e]) throw Error(3, elem_why_[e]);
-    if (total < 0) throw Error(6, "neighbour list exceeds 2^31 entries");
+    HIPCHK(hipEventRecord(ev_[1], s));
 
-    ent_j_.ensure(4 * (size_t)std::max(total, 1));
-    ent_shift_.ensure(4 * (size_t)std::max(total, 1));
-    rev_.ensure(4 * (size_t)std::max(total, 1));
+    // Entry capacity per atom: the most neighbours seen by this context, or an
+    // estimate from the mean density, rounded up to a staging size of the
+    // symmetry-function kernels. An atom with more neighbours raises flags[1]
+    // and the evaluation is repeated with room for it.
+    const bool fp64 = precision_ == 0;
+    int lds_lim = 0;
+    for (int c = 64; c <= 1024; c *= 2)
+        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536) lds_lim = c;
+    if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
+    {
+        const double vol = std::fabs(det3(g.lat));
+        double est = 4.0 / 3.0 * M_PI * rc * rc * rc * nat / std::max(vol, 1e-300);
+        if (!periodic) est = std::min(est, (double)(nat - 1));
+        const double want = std::max((double)cap_hint_, std::min(1.2 * est + 8.0, 1e6));
+        int cap = 64;
+        while (cap < want && cap < lds_lim) cap *= 2;
+        w.cap = cap;
+        w.cap_log2 = 0;
+        while ((1 << w.cap_log2) < cap) ++w.cap_log2;
+    }
+    if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
+    const size_t slots = (size_t)nat * w.cap;
+    ent_j_.ensure(4 * slots);
+    ent_shift_.ensure(4 * slots);
+    ent_s_.ensure(4 * slots);
+    rev_.ensure(4 * slots);
     w.ent_j = ent_j_.as<int>();
     w.ent_shift = ent_shift_.as<int>();
+    w.ent_s = ent_s_.as<int>();
     w.rev = rev_.as<int>();
-    launch_nl_fill(g, w, s);
+    w.nstencil = (int)S;
+    w.pos_start = nullptr;
+    if ((size_t)nat * S * 2 <= kPosStartBytes) {
+        pos_start_.ensure((size_t)nat * S * 2);
+        w.pos_start = pos_start_.as<unsigned short>();
+    }
+    launch_nl_build(g, w, (int)ncell, s);
     launch_nl_rev(w, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[2], s));
 
-    const int maxcnt = std::max(flags[1], 1);
-    const int nmax = (maxcnt + 63) / 64 * 64;
-    const bool fp64 = precision_ == 0;
-    if (nmax > 1024 || sf_lds_bytes(fp64, nmax, dm_.nsfmax, dm_.g4_max, true) > 65536) {
-        int lim = 64;
-        while (lim + 64 <= 1024 && sf_lds_bytes(fp64, lim + 64, dm_.nsfmax, dm_.g4_max, true) <= 65536) lim += 64;
-        throw Error(6, "an atom has " + std::to_string(maxcnt) + " neighbours; this build stages at most " +
-                           std::to_string(lim) + " per atom in LDS");
-    }
-    // the shift-overflow flag is raised by the fill kernel; checked below
     if (fp64)
-        compute<double>(w, g, nmax, total, s);
+        compute<double>(w, g, s);
     else
-        compute<float>(w, g, nmax, total, s);
+        compute<float>(w, g, s);
     HIPCHK(hipEventRecord(ev_[6], s));
+    int *pi = (int *)pinned_;
     HIPCHK(hipMemcpyAsync(pin + 64, w.result, 10 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(pi + 40, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (pi[40 + 2]) throw Error(6, "periodic image shift beyond +-127 cells");
+    const int *flags = pi + 40;
+    if (flags[0]) throw Error(4, "structure contains element Z=" + std::to_string(flags[0]) +
+                                     " that the potential does not describe");
+    if (flags[3]) {
+        for (int e = 0; e < dm_.nel; ++e)
+            if (model_.el[e].Z == flags[3]) throw Error(3, elem_why_[e]);
+        throw Error(3, "element Z=" + std::to_string(flags[3]) + " cannot be evaluated");
+    }
+    if (flags[1] > 0) {
+        if (flags[1] > lds_lim)
+            throw Error(6, "an atom has " + std::to_string(flags[1]) + " neighbours; this build stages at most " +
+                               std::to_string(lds_lim) + " per atom in LDS");
+        if (retry == 0) throw Error(6, "neighbour capacity did not converge");
+        cap_hint_ = std::max(cap_hint_, flags[1]);
+        run(nat, Z_dev, pos_dev, periodic, lat, force_dev, s, energy, dEdLat, retry - 1);
+        return;
+    }
+    if (flags[2]) throw Error(6, "periodic image shift beyond +-127 cells");
     last_nat_ = nat;
-    last_total_ = total;
+    last_cap_ = w.cap;
+    last_total_ = -1;
     for (int k = 0; k < 6; ++k) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
@@ -538,7 +560,7 @@ void Engine::evaluate_host(int nat, const int *Z, const double *pos, int periodi
         HIPCHK(hipMemcpyAsync(zin_.p, Z, 4 * (size_t)nat, hipMemcpyHostToDevice, stream_));
         HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)nat, hipMemcpyHostToDevice, stream_));
     }
-    run(nat, zin_.as<int>(), posin_.as<double>(), periodic, lat, fout_.as<double>(), stream_, energy, dEdLat);
+    run(nat, zin_.as<int>(), posin_.as<double>(), periodic, lat, fout_.as<double>(), stream_, energy, dEdLat, 1);
     if (nat > 0) {
         HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)nat, hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
@@ -550,26 +572,41 @@ void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, i
                              hipStream_t stream)
 {
     if (!dev_ready_) init_device();
-    run(nat, Z_dev, pos_dev, periodic, lat, force_dev, stream ? stream : stream_, energy, dEdLat);
+    run(nat, Z_dev, pos_dev, periodic, lat, force_dev, stream ? stream : stream_, energy, dEdLat, 1);
+}
+
+int64_t Engine::neighbor_count()
+{
+    if (last_total_ < 0) {
+        std::vector<int> cnt(last_nat_);
+        if (last_nat_) HIPCHK(hipMemcpy(cnt.data(), count_.p, 4 * (size_t)last_nat_, hipMemcpyDeviceToHost));
+        last_total_ = 0;
+        for (int c : cnt) last_total_ += c;
+    }
+    return last_total_;
 }
 
 void Engine::get_neighbors(int64_t cap, int32_t *ent)
 {
-    if (cap < last_total_) throw Error(1, "neighbour buffer too small");
-    if (last_total_ == 0) return;
-    std::vector<int> off(last_nat_ + 1), j(last_total_), sh(last_total_);
-    HIPCHK(hipMemcpy(off.data(), offset_.p, 4 * (last_nat_ + 1), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(j.data(), ent_j_.p, 4 * last_total_, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(sh.data(), ent_shift_.p, 4 * last_total_, hipMemcpyDeviceToHost));
+    const int64_t total = neighbor_count();
+    if (cap < total) throw Error(1, "neighbour buffer too small");
+    if (total == 0) return;
+    const size_t slots = (size_t)last_nat_ * last_cap_;
+    std::vector<int> cnt(last_nat_), j(slots), sh(slots);
+    HIPCHK(hipMemcpy(cnt.data(), count_.p, 4 * (size_t)last_nat_, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(j.data(), ent_j_.p, 4 * slots, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sh.data(), ent_shift_.p, 4 * slots, hipMemcpyDeviceToHost));
+    int64_t q = 0;
     for (int i = 0; i < last_nat_; ++i)
-        for (int q = off[i]; q < off[i + 1]; ++q) {
+        for (int n = 0; n < cnt[i]; ++n, ++q) {
+            const size_t t = (size_t)i * last_cap_ + n;
             int x, y, z;
-            unpack_shift(sh[q], x, y, z);
-            ent[5 * (int64_t)q + 0] = i;
-            ent[5 * (int64_t)q + 1] = j[q];
-            ent[5 * (int64_t)q + 2] = x;
-            ent[5 * (int64_t)q + 3] = y;
-            ent[5 * (int64_t)q + 4] = z;
+            unpack_shift(sh[t], x, y, z);
+            ent[5 * q + 0] = i;
+            ent[5 * q + 1] = j[t];
+            ent[5 * q + 2] = x;
+            ent[5 * q + 3] = y;
+            ent[5 * q + 4] = z;
         }
 }
 

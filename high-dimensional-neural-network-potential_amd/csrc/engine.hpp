@@ -40,17 +40,18 @@ class Engine {
                          const double *lat, double *energy, double *force_dev, double *dEdLat,
                          hipStream_t stream);
 
-    int64_t neighbor_count() const { return last_total_; }
+    int64_t neighbor_count();
     void get_neighbors(int64_t cap, int32_t *ent);
     void get_symfunctions(int ld, double *G, double *dEdG, double *Ei);
     void timings(double *ms6) const;
 
   private:
     void init_device();
+    // retry: how often an evaluation may be repeated with a larger neighbour capacity
     void run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
-             double *force_dev, hipStream_t s, double *energy, double *dEdLat);
+             double *force_dev, hipStream_t s, double *energy, double *dEdLat, int retry);
     template <typename R>
-    void compute(Work &w, const Geo &g, int nmax, int total, hipStream_t s);
+    void compute(Work &w, const Geo &g, hipStream_t s);
 
     HostModel model_;
     int precision_;
@@ -67,11 +68,12 @@ class Engine {
     // work buffers
     DevMem zin_, posin_, fout_;
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
-    DevMem count_, offset_, perm_, eidx_sorted_, Ei_, vir_, misc_;
-    DevMem ent_j_, ent_shift_, rev_, gent_, G_, dEdG_, cub_tmp_, red_;
+    DevMem count_, perm_, eidx_sorted_, Ei_, vir_, misc_;
+    DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_, red_;
     void *pinned_ = nullptr;
-    int last_nat_ = 0;
-    int64_t last_total_ = 0;
+    int last_nat_ = 0, last_cap_ = 0;
+    int64_t last_total_ = 0;   // -1: not yet summed
+    int cap_hint_ = 0;         // most neighbours of one atom seen so far
 };
 
 // host copies of reference 3x3 routines (exact operation order)

@@ -12,14 +12,18 @@
 
 namespace hdnnp {
 
+// One wave per atom m, lanes over its entries: F_m = sum_q (g[q] - g[rev[q]]),
+// where g[q] = dE_m/dR_j of entry q and g[rev[q]] = dE_j/dR_m of its partner.
+// The lane sums are combined by a fixed butterfly, so the result does not
+// depend on scheduling.
 template <typename R>
-__global__ __launch_bounds__(256) void k_force_gather(Work w, const R *g)
+__global__ __launch_bounds__(64) void k_force_gather(Work w, const R *g)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= w.nat) return;
-    const int o = w.offset[m], n = w.count[m];
+    const int m = xcd_block(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int o = m * w.cap, n = w.count[m];
     double fx = 0, fy = 0, fz = 0;
-    for (int q = o; q < o + n; ++q) {
+    for (int q = o + lane; q < o + n; q += 64) {
         fx += (double)g[3 * (size_t)q + 0];
         fy += (double)g[3 * (size_t)q + 1];
         fz += (double)g[3 * (size_t)q + 2];
@@ -30,18 +34,29 @@ __global__ __launch_bounds__(256) void k_force_gather(Work w, const R *g)
             fz -= (double)g[3 * (size_t)r + 2];
         }
     }
-    w.force[3 * m + 0] = fx;
-    w.force[3 * m + 1] = fy;
-    w.force[3 * m + 2] = fz;
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) {
+        fx += __shfl_xor(fx, k);
+        fy += __shfl_xor(fy, k);
+        fz += __shfl_xor(fz, k);
+    }
+    if (lane == 0) {
+        w.force[3 * m + 0] = fx;
+        w.force[3 * m + 1] = fy;
+        w.force[3 * m + 2] = fz;
+    }
 }
 
 // entries whose partner is missing (pair at the exact cutoff seen from one
-// side only): add their contribution to the neighbour directly
+// side only): add their contribution to the neighbour directly. One thread
+// per entry slot; slot q belongs to atom q >> cap_log2.
 template <typename R>
-__global__ __launch_bounds__(256) void k_orphans(Work w, const R *g, int nent)
+__global__ __launch_bounds__(256) void k_orphans(Work w, const R *g)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nent || w.rev[q] >= 0) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)w.nat * w.cap) return;
+    const int i = (int)(q >> w.cap_log2);
+    if ((int)(q - ((long)i << w.cap_log2)) >= w.count[i] || w.rev[q] >= 0) return;
     const int j = w.ent_j[q];
     atomicAdd(&w.force[3 * j + 0], -(double)g[3 * (size_t)q + 0]);
     atomicAdd(&w.force[3 * j + 1], -(double)g[3 * (size_t)q + 1]);
@@ -49,14 +64,14 @@ __global__ __launch_bounds__(256) void k_orphans(Work w, const R *g, int nent)
 }
 
 template <typename R>
-void launch_force_gather_t(Work &w, const R *gent, int nent, hipStream_t s)
+void launch_force_gather_t(Work &w, const R *gent, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_force_gather<R>, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, gent);
-    if (nent > 0)
-        hipLaunchKernelGGL(k_orphans<R>, dim3((nent + 255) / 256), dim3(256), 0, s, w, gent, nent);
+    hipLaunchKernelGGL(k_force_gather<R>, dim3(w.nat), dim3(64), 0, s, w, gent);
+    const long slots = (long)w.nat * w.cap;
+    hipLaunchKernelGGL(k_orphans<R>, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, w, gent);
 }
-template void launch_force_gather_t<double>(Work &, const double *, int, hipStream_t);
-template void launch_force_gather_t<float>(Work &, const float *, int, hipStream_t);
+template void launch_force_gather_t<double>(Work &, const double *, hipStream_t);
+template void launch_force_gather_t<float>(Work &, const float *, hipStream_t);
 
 // Two-stage FP64 reduction, result[0] = sum Ei, result[1..9] = sum vir.
 // Fixed order for a given nat: block b sums a fixed contiguous chunk (thread

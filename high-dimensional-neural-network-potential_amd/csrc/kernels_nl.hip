@@ -54,15 +54,19 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
-// Element rank of each atom; unknown Z flagged (rare, so the atomic is cheap).
+// Element rank of each atom. Unknown Z and elements the model cannot
+// evaluate are flagged (rare, so the atomics are cheap); such atoms get a
+// usable element so that every kernel stays in bounds, and the host discards
+// the results.
 __global__ __launch_bounds__(256) void k_map_elements(DevModel m, Work w)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w.nat) return;
     const int z = w.Z[i];
     const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
-    if (e < 0) atomicMax(&w.flags[0], z > 0 ? z : 1);
-    w.eidx[i] = e < 0 ? 0 : e;
+    if (e == -1) atomicMax(&w.flags[0], z > 0 ? z : 1);
+    if (e <= -2) atomicMax(&w.flags[3], z);
+    w.eidx[i] = e < 0 ? m.fallback_el : e;
 }
 
 void launch_map_elements(const DevModel &m, Work &w, hipStream_t s)
@@ -205,134 +209,177 @@ __device__ inline bool stencil_cell(const Geo &g, int key, int s, StencilCell &c
     return true;
 }
 
-// the strict distance test of neighborLists.f90:213-214
-__device__ inline bool is_neighbor(const Geo &g, const double4 &pi, const double4 &pj,
-                                   const StencilCell &c)
+__device__ inline int wave_incl_scan_int(int v, int lane)
 {
-    const double d0 = pi.x - (pj.x + c.dl0);
-    const double d1 = pi.y - (pj.y + c.dl1);
-    const double d2 = pi.z - (pj.z + c.dl2);
-    return (d0 * d0 + d1 * d1) + d2 * d2 < g.rc2;
-}
-
-__device__ inline int wave_sum_int(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
     return v;
 }
 
-__device__ inline int wave_excl_scan_int(int v, int lane)
+// Neighbour lists of the atoms of one grid cell per wave (one-wave blocks).
+// All atoms of a cell share the stencil, so the candidates are enumerated
+// once and tested against each of them.
+//
+// Per chunk of 64 stencil cells each lane resolves one cell (wrapped id,
+// image shift and the lattice offset dl, summed as the reference does); the
+// chunk's candidate atoms are numbered in stencil order, atoms of a cell in
+// ascending index, and read 64 at a time: lane t finds its cell by a binary
+// search over the chunk's prefix counts. For each atom of the cell a ballot
+// compacts the hits, so its entries are written in exactly the reference's
+// list order. pos_start (when present) receives, per atom and stencil cell,
+// the slot of that cell's first entry; the partner search uses it.
+// Entries beyond `cap` are counted but not stored; flags[1] then receives
+// the number needed so that the host can repeat the evaluation.
+__global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
 {
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x - v;
-}
-
-// Count pass: one wave per atom, lanes over the stencil cells.
-__global__ __launch_bounds__(256) void k_nl_count(Geo g, Work w)
-{
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= w.nat) return;
-    const double4 pi = w.wpos[i];
-    const int key = w.cellkey[i];
-    const int S = (2 * g.nc[0] + 1) * (2 * g.nc[1] + 1) * (2 * g.nc[2] + 1);
-    int cnt = 0;
-    for (int s = lane; s < S; s += 64) {
-        StencilCell c;
-        if (!stencil_cell(g, key, s, c)) continue;
-        for (int q = w.cell_start[c.cell]; q < w.cell_end[c.cell]; ++q) {
-            const int j = w.cell_atoms[q];
-            if (j == i && c.centre) continue;
-            cnt += is_neighbor(g, pi, w.wpos[j], c) ? 1 : 0;
+    __shared__ int pref[64], q0[64], info[64], ia[64];
+    __shared__ double dl[3][64];
+    __shared__ double4 pa[64];
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    const int a0 = w.cell_start[c], A = w.cell_end[c] - a0;
+    if (A <= 0) return;
+    const int S = w.nstencil;
+    for (int g0 = 0; g0 < A; g0 += 64) {
+        const int Ag = min(64, A - g0);
+        __syncthreads();                      // previous group done with ia / pa
+        if (lane < Ag) {
+            const int t = w.cell_atoms[a0 + g0 + lane];
+            ia[lane] = t;
+            pa[lane] = w.wpos[t];
         }
-    }
-    cnt = wave_sum_int(cnt);
-    if (lane == 0) w.count[i] = cnt;
-}
-
-void launch_nl_count(const Geo &g, Work &w, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_nl_count, dim3((w.nat + 3) / 4), dim3(256), 0, s, g, w);
-}
-
-// Fill pass: the same enumeration; entries are written in stencil order and,
-// within a cell, in atom order -- the reference's list order (self excluded).
-__global__ __launch_bounds__(256) void k_nl_fill(Geo g, Work w)
-{
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= w.nat) return;
-    const double4 pi = w.wpos[i];
-    const int key = w.cellkey[i];
-    const int S = (2 * g.nc[0] + 1) * (2 * g.nc[1] + 1) * (2 * g.nc[2] + 1);
-    int base = w.offset[i];
-    for (int s0 = 0; s0 < S; s0 += 64) {
-        const int s = s0 + lane;
-        StencilCell c;
-        const bool ok = s < S && stencil_cell(g, key, s, c);
-        int cnt = 0;
-        if (ok)
-            for (int q = w.cell_start[c.cell]; q < w.cell_end[c.cell]; ++q) {
-                const int j = w.cell_atoms[q];
-                if (j == i && c.centre) continue;
-                cnt += is_neighbor(g, pi, w.wpos[j], c) ? 1 : 0;
+        int n = 0;                            // lane a: entries of atom a so far
+        for (int s0 = 0; s0 < S; s0 += 64) {
+            StencilCell sc{};
+            int m = 0, start = 0;
+            const bool in_s = s0 + lane < S;
+            if (in_s && stencil_cell(g, c, s0 + lane, sc)) {
+                start = w.cell_start[sc.cell];
+                m = w.cell_end[sc.cell] - start;
             }
-        int pos = base + wave_excl_scan_int(cnt, lane);
-        if (ok && cnt) {
-            if (c.sx < -127 || c.sx > 127 || c.sy < -127 || c.sy > 127 || c.sz < -127 || c.sz > 127)
-                atomicMax(&w.flags[2], 1);
-            const int sh = pack_shift(c.sx, c.sy, c.sz);
-            for (int q = w.cell_start[c.cell]; q < w.cell_end[c.cell]; ++q) {
-                const int j = w.cell_atoms[q];
-                if (j == i && c.centre) continue;
-                if (is_neighbor(g, pi, w.wpos[j], c)) {
-                    w.ent_j[pos] = j;
-                    w.ent_shift[pos] = sh;
-                    ++pos;
+            const int incl = wave_incl_scan_int(m, lane);
+            const int tot = __shfl(incl, 63);
+            const int my_pref = incl - m;
+            __syncthreads();                  // previous chunk done with the tables
+            pref[lane] = my_pref;
+            q0[lane] = start;
+            const bool big = sc.sx < -127 || sc.sx > 127 || sc.sy < -127 || sc.sy > 127 || sc.sz < -127 ||
+                             sc.sz > 127;
+            info[lane] = m ? (pack_shift(sc.sx, sc.sy, sc.sz) | (sc.centre ? 1 << 24 : 0) | (big ? 1 << 25 : 0)) : 0;
+            dl[0][lane] = sc.dl0;
+            dl[1][lane] = sc.dl1;
+            dl[2][lane] = sc.dl2;
+            __syncthreads();
+            for (int t0 = 0; t0 < tot; t0 += 64) {
+                const int t = t0 + lane;
+                int j = -1, o = 0, inf = 0;
+                double e0 = 0, e1 = 0, e2 = 0;
+                if (t < tot) {
+#pragma unroll
+                    for (int b = 32; b > 0; b >>= 1)
+                        if (pref[o + b] <= t) o += b;
+                    j = w.cell_atoms[q0[o] + (t - pref[o])];
+                    inf = info[o];
+                    const double4 pj = w.wpos[j];
+                    e0 = pj.x + dl[0][o];             // image of j: pj + dl, as neighborLists.f90:212
+                    e1 = pj.y + dl[1][o];
+                    e2 = pj.z + dl[2][o];
+                }
+                // this stencil lane's first candidate falls in this round
+                const bool first_here = in_s && my_pref >= t0 && my_pref < t0 + 64;
+                for (int a = 0; a < Ag; ++a) {
+                    const int ai = ia[a];
+                    bool hit = false;
+                    if (t < tot && !(j == ai && (inf >> 24 & 1))) {
+                        const double4 p = pa[a];
+                        const double d0 = p.x - e0, d1 = p.y - e1, d2 = p.z - e2;
+                        hit = (d0 * d0 + d1 * d1) + d2 * d2 < g.rc2;    // neighborLists.f90:213-214
+                    }
+                    const unsigned long long mask = __ballot(hit);
+                    const int na = __shfl(n, a);
+                    if (hit) {
+                        const int pos = na + (int)__builtin_amdgcn_mbcnt_hi(
+                                                 (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+                        if (inf >> 25 & 1) atomicMax(&w.flags[2], 1);
+                        if (pos < w.cap) {
+                            const int q = ai * w.cap + pos;
+                            w.ent_j[q] = j;
+                            w.ent_shift[q] = inf & 0xFFFFFF;
+                            w.ent_s[q] = s0 + o;
+                        }
+                    }
+                    if (w.pos_start && first_here) {
+                        const int k = my_pref - t0;
+                        const int ps = na + __popcll(mask & ((1ull << k) - 1));
+                        w.pos_start[(size_t)ai * S + s0 + lane] = (unsigned short)min(ps, 65535);
+                    }
+                    if (lane == a) n += __popcll(mask);
                 }
             }
+            // stencil cells of this chunk that start at its end (empty tail cells)
+            if (w.pos_start)
+                for (int a = 0; a < Ag; ++a) {
+                    const int na = __shfl(n, a);
+                    if (in_s && my_pref >= tot) w.pos_start[(size_t)ia[a] * S + s0 + lane] = (unsigned short)min(na, 65535);
+                }
         }
-        base += wave_sum_int(cnt);
+        if (lane < Ag) {
+            w.count[ia[lane]] = min(n, w.cap);
+            if (n > w.cap) atomicMax(&w.flags[1], n);
+        }
     }
 }
 
-void launch_nl_fill(const Geo &g, Work &w, hipStream_t s)
+void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nl_fill, dim3((w.nat + 3) / 4), dim3(256), 0, s, g, w);
+    hipLaunchKernelGGL(k_nl_build, dim3(ncell), dim3(64), 0, s, g, w);
 }
 
-// Partner entries: entry (i -> j, L) pairs with (j -> i, -L) in j's list.
-// The strict test is evaluated separately in each direction, so a pair at
-// the exact cutoff can lack its partner; such entries get rev = -1.
+// Partner entries: entry (i -> j, image L) found in stencil cell s of i pairs
+// with (j -> i, -L), which j finds in the mirrored stencil cell S-1-s. With
+// pos_start the partner lies among the few entries of that cell in j's list;
+// without it, lists are sorted by (stencil cell, atom) and are searched
+// by bisection. The strict distance test is evaluated separately in each
+// direction, so a pair at the exact cutoff can lack its partner: rev = -1.
+// One thread per entry slot; slot q belongs to atom q >> cap_log2.
 __global__ __launch_bounds__(256) void k_nl_rev(Work w)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= w.nat) return;
-    const int o = w.offset[i], n = w.count[i];
-    for (int q = lane; q < n; q += 64) {
-        const int j = w.ent_j[o + q];
-        int x, y, z;
-        unpack_shift(w.ent_shift[o + q], x, y, z);
-        const int want = pack_shift(-x, -y, -z);
-        const int oj = w.offset[j], nj = w.count[j];
-        int r = -1;
-        for (int t = 0; t < nj; ++t)
-            if (w.ent_j[oj + t] == i && w.ent_shift[oj + t] == want) {
-                r = oj + t;
+    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= (long)w.nat * w.cap) return;
+    const int i = (int)(q >> w.cap_log2);
+    if ((int)(q - ((long)i << w.cap_log2)) >= w.count[i]) return;
+    const int S = w.nstencil;
+    const int j = w.ent_j[q];
+    const int sp = S - 1 - w.ent_s[q];
+    const int bj = j * w.cap, nj = w.count[j];
+    int r = -1;
+    if (w.pos_start) {
+        for (int t = w.pos_start[(size_t)j * S + sp]; t < nj; ++t) {
+            if (w.ent_s[bj + t] != sp) break;
+            if (w.ent_j[bj + t] == i) {
+                r = bj + t;
                 break;
             }
-        w.rev[o + q] = r;
+        }
+    } else {
+        int lo = 0, hi = nj;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int st = w.ent_s[bj + mid];
+            if (st < sp || (st == sp && w.ent_j[bj + mid] < i)) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < nj && w.ent_s[bj + lo] == sp && w.ent_j[bj + lo] == i) r = bj + lo;
     }
+    w.rev[q] = r;
 }
 
 void launch_nl_rev(Work &w, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nl_rev, dim3((w.nat + 3) / 4), dim3(256), 0, s, w);
+    const long slots = (long)w.nat * w.cap;
+    hipLaunchKernelGGL(k_nl_rev, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, w);
 }
 
 // Element segments of the element-sorted atom order: elem_off[e] = first

@@ -251,7 +251,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
                       int *seg, R *gent)
 {
     const int lane = lane_id();
-    const int off = w.offset[i], n = w.count[i];
+    const int off = i * w.cap, n = w.count[i];
     const double4 pi = w.wpos[i];
     const double rce = m.einfo[e].rc;
     const R rrc = R(1) / (R)rce;
