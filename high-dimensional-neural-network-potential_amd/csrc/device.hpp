@@ -39,10 +39,12 @@ struct DevModel {
     const int *g4_s;
     const double *g4_eta, *g4_lam, *g4_zeta, *g4_pre;
     const int *g4_ipow;      // zeta-1 when zeta is integral, else kNonIntegralZeta
+    const int *g4_sq;        // k when zeta = 2^k (k >= 0), else -1
     const int *g4_neweta;
     const int *g2_f;         // neighbour element rank of each G2 entry
     // FP32 copies of the tables (the mixed path reads these, no conversions)
     const float *g2f_eta, *g2f_rs, *g4f_eta, *g4f_lam, *g4f_zeta, *g4f_pre;
+    const float *g2f_ea, *g4f_ea;   // -eta * log2(e): exp(-eta x) = exp2(ea x)
     // the entries of centre element e are contiguous: [start, start+count)
     int g2_start[kMaxEl], g2_count[kMaxEl];
     int g4_start[kMaxEl], g4_count[kMaxEl];

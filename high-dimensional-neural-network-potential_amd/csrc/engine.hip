@@ -157,7 +157,7 @@ void Engine::init_device()
     std::vector<int> zlut(128, -1);
     std::vector<double> mean, den;
     std::vector<int2> g2tab(kMaxEl * kMaxEl, make_int2(0, 0)), g4tab(kMaxEl * kMaxEl * kMaxEl, make_int2(0, 0));
-    std::vector<int> g2s, g4s, g4ipow, g4new;
+    std::vector<int> g2s, g4s, g4ipow, g4sq, g4new;
     std::vector<double> g2eta, g2rs, g4eta, g4lam, g4zeta, g4pre;
     int off = 0;
     for (int e = 0; e < d.nel; ++e) {
@@ -212,6 +212,10 @@ void Engine::init_device()
                     g4zeta.push_back(sf.zeta);
                     g4pre.push_back(pre);
                     g4ipow.push_back(integral ? iz - 1 : kNonIntegralZeta);
+                    int sq = -1;
+                    for (int k = 0; k < 31 && integral; ++k)
+                        if (iz == (1 << k)) sq = k;
+                    g4sq.push_back(sq);
                 }
                 g4tab[(e * kMaxEl + f) * kMaxEl + g] = make_int2(start, (int)g4s.size() - start);
             }
@@ -245,6 +249,13 @@ void Engine::init_device()
         d.g4f_lam = upload(model_mem_, tof(g4lam));
         d.g4f_zeta = upload(model_mem_, tof(g4zeta));
         d.g4f_pre = upload(model_mem_, tof(g4pre));
+        auto toea = [](const std::vector<double> &v) {
+            std::vector<float> o(v.size());
+            for (size_t k = 0; k < v.size(); ++k) o[k] = (float)(-v[k] * 1.4426950408889634);
+            return o;
+        };
+        d.g2f_ea = upload(model_mem_, toea(g2eta));
+        d.g4f_ea = upload(model_mem_, toea(g4eta));
         std::vector<ElemInfo> ei(std::max(d.nel, 1));
         for (int e = 0; e < d.nel; ++e) {
             ei[e].nsf = d.nsf[e];
@@ -272,6 +283,7 @@ void Engine::init_device()
     d.g4_zeta = upload(model_mem_, g4zeta);
     d.g4_pre = upload(model_mem_, g4pre);
     d.g4_ipow = upload(model_mem_, g4ipow);
+    d.g4_sq = upload(model_mem_, g4sq);
     d.g4_neweta = upload(model_mem_, g4new);
     d.nlayers = m.el.empty() ? 0 : (int)m.el[0].layers.size();
     mlp_maxw_ = d.nsfmax;
@@ -462,7 +474,9 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     const bool fp64 = precision_ == 0;
     int lds_lim = 0;
     for (int c = 64; c <= 1024; c *= 2)
-        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536) lds_lim = c;
+        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536 &&
+            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, false) <= 65536)
+            lds_lim = c;
     if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
     {
         const double vol = std::fabs(det3(g.lat));

@@ -8,19 +8,29 @@
 // dEdLat scatter of hdnnpCalcNNs (hdnnps.f90:533-553).
 //
 // One 64-lane wavefront (= one work-group) per atom. The atom's neighbours
-// within its cutoff are staged in LDS grouped by element. Triplets (i, j, k)
-// whose third side r_jk reaches the cutoff contribute nothing (fc(r_jk) = 0
-// and fc'(r_jk) = 0 there), so each pass first compacts the triplets with
-// r_jk < rc into an LDS list and evaluates only those, with all 64 lanes busy.
-//   values:    per element pair (f, g) of the neighbours, the block's SF
-//              parameters are wave-uniform (scalar registers); each lane
-//              accumulates its triplets in registers; one DPP reduction per SF.
-//              G2: one lane per SF, looping over that SF's neighbour element.
-//   gradients: all triplets in circulant order (j, j+d mod n), so the lanes of
-//              one round touch distinct j and distinct k; each lane reads its
-//              block's SF parameters from an LDS table and adds g_j, g_k to the
-//              per-neighbour LDS accumulators without contention.
-// The FP32 path uses the hardware exp/rcp/sqrt approximations; the FP64 path
+// within its cutoff are staged in LDS, grouped by element, as two 4-vectors
+// each: (dx, dy, dz, 1/r) and (r, fc, fc'/r, r^2). Triplets whose third side
+// reaches the cutoff contribute nothing (fc(r_jk) = 0), so both passes first
+// compact the candidate triplets with r_jk < rc into an LDS list.
+//   Pass A (values) needs one sum per function, so it runs per element-pair
+//   block (f, h) of the neighbours: the block's candidates are enumerated
+//   with one division each (circulant rows for f = h, diagonals for f != h),
+//   and the block's SF parameters are wave-uniform, in scalar registers
+//   (loaded through the constant address space, so uniform indices compile to
+//   scalar loads); each lane accumulates per function, one wave reduction per
+//   function and block.
+//   Pass B (gradients) needs no per-function sums, so it runs over the
+//   triplets of all blocks at once, in circulant order, with every lane busy;
+//   each lane reads its block's parameters, with dE/dG folded in, from LDS
+//   tables, forms per triplet
+//     S1 = sum w P C E, S2 = sum w P C E 2 eta, S3 = sum w P b^(zeta-1) lambda zeta E
+//   and adds g_j = S1 dF/dR_j + S2 F (d_ij - d_jk) + S3 F dcos/dR_j (same for
+//   k) to FP64 per-neighbour LDS accumulators.
+// zeta = 2^k (the usual choice) takes k squarings; other zeta go through the
+// integer power or pow. Tiles of several atoms of one element per wavefront
+// are supported (AtomsPerWave) but measured slower here: LDS per atom limits
+// the waves in flight.
+// The FP32 path uses the hardware exp2/rcp/sqrt approximations; the FP64 path
 // keeps the libm-accurate functions.
 #include <rocprim/intrinsics/warp_shuffle.hpp>
 
@@ -33,28 +43,79 @@ constexpr int kPairChunk = 512;    // triplet candidates compacted per LDS chunk
 // ---------------------------------------------------------------- math
 template <typename R> struct M;
 template <> struct M<double> {
-    static __device__ double exp(double x) { return ::exp(x); }
     static __device__ double sqrt(double x) { return ::sqrt(x); }
     static __device__ double rcp(double x) { return 1.0 / x; }
     static __device__ double tanh(double x) { return ::tanh(x); }
     static __device__ double pow(double x, double y) { return ::pow(x, y); }
+    // exp(-eta x) from the table value a = -eta
+    static __device__ double expsc(double a, double x) { return ::exp(a * x); }
 };
 template <> struct M<float> {
-    static __device__ float exp(float x) { return __expf(x); }
     static __device__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-    // tanh on [0, 1] through one exponential
+    // tanh on [0, 1] through one exponential: e = exp(-2x) = exp2(-2 log2(e) x)
     static __device__ float tanh(float x)
     {
-        const float e = __expf(-2.0f * x);
+        const float e = __builtin_amdgcn_exp2f(x * -2.8853900817779268f);
         return (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
     }
     static __device__ float pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+    // exp(-eta x) from the table value a = -eta log2(e)
+    static __device__ float expsc(float a, float x) { return __builtin_amdgcn_exp2f(a * x); }
 };
 
-template <typename R> __device__ __forceinline__ R tab(const double *d, const float *f, int x);
-template <> __device__ __forceinline__ double tab<double>(const double *d, const float *, int x) { return d[x]; }
-template <> __device__ __forceinline__ float tab<float>(const double *, const float *f, int x) { return f[x]; }
+// The model tables are not written while a kernel runs: reading them through
+// the constant address space lets the compiler use scalar loads wherever the
+// index is wave-uniform.
+#define CAS __attribute__((address_space(4)))
+template <typename T> __device__ __forceinline__ T cld(const T *p, int x) { return ((const CAS T *)p)[x]; }
+// (start, count) of an int2 table entry
+__device__ __forceinline__ void cld2(const int2 *p, int x, int &a, int &b)
+{
+    const CAS int *q = (const CAS int *)p;
+    a = q[2 * x];
+    b = q[2 * x + 1];
+}
+__device__ __forceinline__ ElemInfo cld_einfo(const ElemInfo *p, int e)
+{
+    const CAS ElemInfo *q = (const CAS ElemInfo *)p + e;
+    ElemInfo r;
+    r.nsf = q->nsf;
+    r.sf_off = q->sf_off;
+    r.g2_start = q->g2_start;
+    r.g2_count = q->g2_count;
+    r.g4_start = q->g4_start;
+    r.g4_count = q->g4_count;
+    r.rc = q->rc;
+    r.atom_energy = q->atom_energy;
+    return r;
+}
+
+template <typename R> __device__ __forceinline__ R tabc(const double *d, const float *f, int x);
+template <> __device__ __forceinline__ double tabc<double>(const double *d, const float *, int x) { return cld(d, x); }
+template <> __device__ __forceinline__ float tabc<float>(const double *, const float *f, int x) { return cld(f, x); }
+
+// exponent scale of exp(-eta x) in the form M<R>::expsc expects
+template <typename R> __device__ __forceinline__ R g2_ea(const DevModel &m, int x);
+template <> __device__ __forceinline__ double g2_ea<double>(const DevModel &m, int x) { return -cld(m.g2_eta, x); }
+template <> __device__ __forceinline__ float g2_ea<float>(const DevModel &m, int x) { return cld(m.g2f_ea, x); }
+template <typename R> __device__ __forceinline__ R g4_ea(const DevModel &m, int x);
+template <> __device__ __forceinline__ double g4_ea<double>(const DevModel &m, int x) { return -cld(m.g4_eta, x); }
+template <> __device__ __forceinline__ float g4_ea<float>(const DevModel &m, int x) { return cld(m.g4f_ea, x); }
+
+// wave-uniform copies (scalar registers)
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double uni(double x)
+{
+    const long long v = __builtin_bit_cast(long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | lo);
+}
 
 // tanh^3 cutoff, symmetryFunctions.f90:310-327; dfc = fc'(r)/r
 template <typename R>
@@ -71,34 +132,21 @@ __device__ __forceinline__ void cutoff_fn(R r, R rc, R rrc, R &fc, R &dfc)
     }
 }
 
-// b^(zeta-1) with a wave-uniform exponent: integer powers by repeated
-// squaring (the integer branch of symmetryFunctions.f90:265-273)
+// the same without the range test, for r_jk of a compacted triplet: r_jk^2 < rc^2
+// was tested already, and at r = rc both expressions vanish
 template <typename R>
-__device__ __forceinline__ R zpow_uniform(R b, int ip, R zm1)
+__device__ __forceinline__ void cutoff_in(R r, R rrc, R &fc, R &dfc)
 {
-    if (ip == kNonIntegralZeta) return b > R(0) ? M<R>::pow(b, zm1) : (zm1 > R(0) ? R(0) : R(1));
-    int n = ip < 0 ? -ip : ip;
-    R r = 1, p = b;
-    while (n) {
-        if (n & 1) r *= p;
-        n >>= 1;
-        if (n) p *= p;
-    }
-    return ip < 0 ? R(1) / r : r;
+    const R t = M<R>::tanh(R(1) - r * rrc);
+    const R tt = t * t;
+    fc = tt * t;
+    dfc = -(R(3) * rrc * M<R>::rcp(r)) * (tt - tt * tt);
 }
-
-// the same with a per-lane exponent: branch-free over `bits` binary digits
 template <typename R>
-__device__ __forceinline__ R zpow_lane(R b, int ip, R zm1, int bits)
+__device__ __forceinline__ R cutoff_in(R r, R rrc)
 {
-    if (ip == kNonIntegralZeta) return b > R(0) ? M<R>::pow(b, zm1) : (zm1 > R(0) ? R(0) : R(1));
-    const int n = ip < 0 ? -ip : ip;
-    R r = 1, p = b;
-    for (int k = 0; k < bits; ++k) {
-        r = ((n >> k) & 1) ? r * p : r;
-        p = p * p;
-    }
-    return ip < 0 ? R(1) / r : r;
+    const R t = M<R>::tanh(R(1) - r * rrc);
+    return t * t * t;
 }
 
 // full-wave sum with DPP (the result is read from lane 63)
@@ -115,13 +163,6 @@ __device__ __forceinline__ R wave_total(R v)
     return rocprim::warp_shuffle(v, 63, 64);
 }
 
-template <typename R>
-__device__ __forceinline__ R wave_sum(R v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 __device__ __forceinline__ int mbcnt(unsigned long long m)
@@ -130,38 +171,49 @@ __device__ __forceinline__ int mbcnt(unsigned long long m)
 }
 
 // ---------------------------------------------------------------- LDS layout
-// 32-bit pointers into LDS: they stay in scalar registers and keep the LDS
-// address space, so accesses compile to ds_* instructions
+// 32-bit pointers into LDS: they keep the LDS address space, so accesses
+// compile to ds_* instructions
 #define LDS __attribute__((address_space(3)))
 
+template <typename R> struct Vec4 { typedef R T __attribute__((ext_vector_type(4))); };
+typedef int I2 __attribute__((ext_vector_type(2)));
+
+// Atoms per wavefront for a staging capacity. Several atoms of one element
+// can share a wavefront (64/GA lanes each; the SF parameters stay
+// wave-uniform), which fills the lanes of the small element-pair blocks, but
+// the LDS per atom then limits the waves in flight: one atom per wavefront
+// measured fastest on MI355X for 50-100 neighbours.
+template <int NMAX> struct AtomsPerWave {
+    static constexpr int value = 1;
+};
+
+// One atom's LDS region.
 template <typename R>
 struct Lds {
-    LDS R *dx, *dy, *dz, *r, *ir, *fc, *dfc;   // staged neighbours
-    LDS R *acc;                                 // values: G sums; gradients: dE/dG of raw G
-    LDS double *gx, *gy, *gz;                   // gradients: per-neighbour accumulators (FP64:
-                                                // ds_add_f64 is much faster than ds_add_f32 here)
-    LDS int *ent, *el;                          // list entry and element rank of a staged neighbour
-    LDS int *pairs;                             // compacted triplets
-    LDS R *t_eta, *t_lam, *t_zeta, *t_pre;      // gradients: G4 table of the centre element
-    LDS int *t_s, *t_ip, *t_new;
-    LDS int *btab;                              // gradients: start, count of block f*kMaxEl+g at [2b], [2b+1]
+    typedef typename Vec4<R>::T V;
+    LDS V *a;                     // (dx, dy, dz, 1/r) of each staged neighbour
+    LDS V *b;                     // (r, fc, fc'/r, r^2)
+    LDS int *el, *ent;            // element rank and list entry
+    LDS int *pairs;               // compacted triplets (j | k << 16)
+    LDS int *seg;                 // [kMaxEl+1] element segments of the staged neighbours
+    LDS double *gx, *gy, *gz;     // gradients: per-neighbour accumulators (FP64: ds_add_f64
+                                  // is much faster than ds_add_f32 here)
+    LDS R *acc;                   // values: G sums; gradients: dE/dG of the raw G
+    // gradients: G4 functions of the centre element (local index u), block table
+    LDS V *sp0;                   // (exp scale, lambda, q1 = w P, q2 = w P 2 eta)
+    LDS R *sp1;                   // q3 = w P lambda zeta
+    LDS int *smeta;               // zeta = 2^k: k, else 0xFF; | new-eta flag << 8
+    LDS I2 *btab;                 // block f*kMaxEl+h (f <= h): (first u, count)
 };
 
-template <typename R>
-__device__ __forceinline__ void lds_add(LDS R *p, R v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// LDS layout. Everything sized by the staging capacity nmax comes first, so
-// that with nmax a compile-time constant all these arrays sit at constant
-// offsets; the per-SF accumulators and the SF table follow.
+// Everything sized by the staging capacity nmax comes first, so that with
+// nmax a compile-time constant these arrays sit at constant offsets.
 struct LdsOffsets {
-    size_t R, D, I, P, B, A, T, TI, total;
+    size_t a, b, el, ent, pairs, seg, g, acc, sp0, sp1, smeta, btab, total;
 };
 
 template <typename R>
-__device__ __host__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, bool grad)
+__host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, bool grad)
 {
     LdsOffsets o{};
     size_t off = 0;
@@ -170,22 +222,29 @@ __device__ __host__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
         off += (bytes + 15) / 16 * 16;
         return at;
     };
-    o.R = take(sizeof(R) * (size_t)nmax * 7);
-    o.D = grad ? take(sizeof(double) * (size_t)nmax * 3) : 0;
-    o.I = take(sizeof(int) * 2 * (size_t)nmax);
-    o.P = take(sizeof(int) * kPairChunk);
-    o.B = grad ? take(sizeof(int) * 2 * kMaxEl * kMaxEl) : 0;
-    o.A = take(sizeof(R) * (size_t)nsfmax);
-    o.T = grad ? take(sizeof(R) * 4 * (size_t)g4max) : 0;
-    o.TI = grad ? take(sizeof(int) * 3 * (size_t)g4max) : 0;
+    o.a = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
+    o.b = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
+    o.el = take(sizeof(int) * (size_t)nmax);
+    o.ent = take(sizeof(int) * (size_t)nmax);
+    o.pairs = take(sizeof(int) * kPairChunk);
+    o.seg = take(sizeof(int) * (kMaxEl + 1));
+    o.g = grad ? take(sizeof(double) * 3 * (size_t)nmax) : 0;
+    o.acc = take(sizeof(R) * (size_t)nsfmax);
+    if (grad) {
+        o.sp0 = take(sizeof(typename Vec4<R>::T) * (size_t)g4max);
+        o.sp1 = take(sizeof(R) * (size_t)g4max);
+        o.smeta = take(sizeof(int) * (size_t)g4max);
+        o.btab = take(sizeof(I2) * kMaxEl * kMaxEl);
+    }
     o.total = off;
     return o;
 }
 
+// bytes of dynamic LDS of a wavefront (GA atom regions)
 template <typename R>
-__host__ __device__ inline size_t lds_bytes(int nmax, int nsfmax, int g4max, bool grad)
+__host__ __device__ inline size_t lds_bytes(int nmax, int ga, int nsfmax, int g4max, bool grad)
 {
-    return lds_offsets<R>(nmax, nsfmax, g4max, grad).total;
+    return ga * lds_offsets<R>(nmax, nsfmax, g4max, grad).total;
 }
 
 template <typename R>
@@ -193,42 +252,88 @@ __device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, bool
 {
     const LdsOffsets o = lds_offsets<R>(nmax, nsfmax, g4max, grad);
     Lds<R> L{};
-    LDS R *p = (LDS R *)(base + o.R);
-    L.dx = p;
-    L.dy = p + nmax;
-    L.dz = p + 2 * nmax;
-    L.r = p + 3 * nmax;
-    L.ir = p + 4 * nmax;
-    L.fc = p + 5 * nmax;
-    L.dfc = p + 6 * nmax;
+    L.a = (LDS typename Lds<R>::V *)(base + o.a);
+    L.b = (LDS typename Lds<R>::V *)(base + o.b);
+    L.el = (LDS int *)(base + o.el);
+    L.ent = (LDS int *)(base + o.ent);
+    L.pairs = (LDS int *)(base + o.pairs);
+    L.seg = (LDS int *)(base + o.seg);
     if (grad) {
-        LDS double *pd = (LDS double *)(base + o.D);
+        LDS double *pd = (LDS double *)(base + o.g);
         L.gx = pd;
         L.gy = pd + nmax;
         L.gz = pd + 2 * nmax;
     }
-    L.ent = (LDS int *)(base + o.I);
-    L.el = L.ent + nmax;
-    L.pairs = (LDS int *)(base + o.P);
-    L.acc = (LDS R *)(base + o.A);
+    L.acc = (LDS R *)(base + o.acc);
     if (grad) {
-        L.btab = (LDS int *)(base + o.B);
-        LDS R *t = (LDS R *)(base + o.T);
-        L.t_eta = t;
-        L.t_lam = t + g4max;
-        L.t_zeta = t + 2 * g4max;
-        L.t_pre = t + 3 * g4max;
-        LDS int *ti = (LDS int *)(base + o.TI);
-        L.t_s = ti;
-        L.t_ip = ti + g4max;
-        L.t_new = ti + 2 * g4max;
+        L.sp0 = (LDS typename Lds<R>::V *)(base + o.sp0);
+        L.sp1 = (LDS R *)(base + o.sp1);
+        L.smeta = (LDS int *)(base + o.smeta);
+        L.btab = (LDS I2 *)(base + o.btab);
     }
     return L;
 }
 
+// ---------------------------------------------------------------- lane groups
+// GL lanes per atom: group = lane / GL. Ballot bits of the own group, sums
+// and maxima over the group / the wavefront.
+template <int GL>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long mk)
+{
+    if (GL == 64) return mk;
+    return (mk >> (lane_id() / GL * GL)) & ((1ull << (GL & 63)) - 1);
+}
+
+template <int GL, typename R>
+__device__ __forceinline__ R group_sum(R v)
+{
+    if (GL == 64) return wave_total(v);
+#pragma unroll
+    for (int o = GL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// a group-uniform value: wave-uniform (scalar register) when one group fills the wavefront
+template <int GL>
+__device__ __forceinline__ int gu(int v) { return GL == 64 ? uni(v) : v; }
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return uni(v);
+}
+
+// GA-atom tile `blk` of the element-sorted atom order: element e, first
+// position t0 in perm, tn atoms (e = -1 past the last tile). One atom per
+// wavefront keeps the index order (t0 = atom, no perm lookup: tn = -1).
+__device__ __forceinline__ void elem_tile(const Work &w, int nel, int blk, int ga, int &e, int &t0, int &tn)
+{
+    if (ga == 1) {
+        e = blk < w.nat ? uni(w.eidx[blk]) : -1;
+        t0 = blk;
+        tn = -1;
+        return;
+    }
+    e = -1;
+    t0 = tn = 0;
+    int acc = 0;
+    for (int f = 0; f < nel; ++f) {
+        const int cnt = uni(w.elem_count[f]);
+        const int nt = (cnt + ga - 1) / ga;
+        if (blk < acc + nt) {
+            e = f;
+            t0 = uni(w.elem_off[f]) + (blk - acc) * ga;
+            tn = min(ga, cnt - (blk - acc) * ga);
+            return;
+        }
+        acc += nt;
+    }
+}
+
 // Geometry of one list entry: d = R_i - (R_j + L.h) (FP64), r = |d|.
 __device__ __forceinline__ void entry_geometry(const Geo &g, const Work &w, const double4 &pi, int e,
-                                      double &d0, double &d1, double &d2, double &r, int &j)
+                                               double &d0, double &d1, double &d2, double &r, int &j)
 {
     j = w.ent_j[e];
     int sx, sy, sz;
@@ -243,35 +348,53 @@ __device__ __forceinline__ void entry_geometry(const Geo &g, const Work &w, cons
     r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
 }
 
-// Stage the neighbours of atom i with r < rc(e) into LDS, grouped by element
-// (seg[f]..seg[f+1]), list order within an element. Entries beyond the cutoff
-// contribute nothing (fc = dfc = 0 there); their gradient rows are zeroed.
-template <typename R>
-__device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Work &w, int i, int e, const Lds<R> L,
-                      int *seg, R *gent)
+// Stage the neighbours of atom i (i < 0: idle group) with r < rc(e) into its
+// LDS region, grouped by element (seg[f]..seg[f+1]), list order within an
+// element. Entries beyond the cutoff contribute nothing (fc = dfc = 0
+// there); their gradient rows are zeroed. nwave: most entries of an atom of
+// the wavefront.
+template <typename R, int GL, int NMAX, bool KEEP>
+__device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Work &w, int i, int nwave, double rce,
+                                      const Lds<R> &L, R *gent)
 {
-    const int lane = lane_id();
-    const int off = i * w.cap, n = w.count[i];
-    const double4 pi = w.wpos[i];
-    const double rce = m.einfo[e].rc;
+    // KEEP: up to 4 chunks of GL entries the counting pass keeps its geometry
+    // in registers for the placing pass (pays in pass B; in pass A the extra
+    // registers cost more occupancy than the recomputation)
+    constexpr int NC = (NMAX + GL - 1) / GL;
+    constexpr bool kKeep = KEEP && NC <= 4;
+    constexpr int NK = kKeep ? NC : 1;
+    const int gl = lane_id() % GL;
+    const int n = i >= 0 ? w.count[i] : 0;
+    const int off = i >= 0 ? i * w.cap : 0;
+    const double4 pi = i >= 0 ? w.wpos[i] : make_double4(0, 0, 0, 0);
     const R rrc = R(1) / (R)rce;
+    double k0[NK], k1[NK], k2[NK], kr[NK];
+    int kf[NK];
     int cnt[kMaxEl];
 #pragma unroll
     for (int f = 0; f < kMaxEl; ++f) cnt[f] = 0;
-    for (int b = 0; b < n; b += 64) {
-        const int q = b + lane;
-        bool keep = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int c0 = c * GL;
+        if (c0 >= nwave) break;
+        const int q = c0 + gl;
         int f = -1;
+        double d0 = 0, d1 = 0, d2 = 0, r = 1e300;
         if (q < n) {
-            double d0, d1, d2, r;
             int j;
             entry_geometry(g, w, pi, off + q, d0, d1, d2, r, j);
-            keep = r < rce;
-            f = w.eidx[j];
+            f = r < rce ? w.eidx[j] : -1;
+        }
+        if (kKeep) {
+            k0[kKeep ? c : 0] = d0;
+            k1[kKeep ? c : 0] = d1;
+            k2[kKeep ? c : 0] = d2;
+            kr[kKeep ? c : 0] = r;
+            kf[kKeep ? c : 0] = f;
         }
 #pragma unroll
         for (int ff = 0; ff < kMaxEl; ++ff)
-            if (ff < m.nel) cnt[ff] += __popcll(__ballot(keep && f == ff));
+            if (ff < m.nel) cnt[ff] += __popcll(group_bits<GL>(__ballot(f == ff)));
     }
     int start[kMaxEl], run[kMaxEl];
     int acc = 0;
@@ -279,41 +402,58 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
     for (int f = 0; f < kMaxEl; ++f) {
         start[f] = acc;
         run[f] = 0;
-        if (lane == 0) seg[f] = acc;
+        if (gl == 0) L.seg[f] = acc;
         acc += (f < m.nel) ? cnt[f] : 0;
     }
-    if (lane == 0) seg[kMaxEl] = acc;
-    for (int b = 0; b < n; b += 64) {
-        const int q = b + lane;
-        bool keep = false;
-        int f = -1;
-        double d0 = 0, d1 = 0, d2 = 0, r = 1;
-        if (q < n) {
-            int j;
-            entry_geometry(g, w, pi, off + q, d0, d1, d2, r, j);
-            keep = r < rce;
-            f = w.eidx[j];
+    if (gl == 0) L.seg[kMaxEl] = acc;
+    const unsigned long long below = (1ull << gl) - 1;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int c0 = c * GL;
+        if (c0 >= nwave) break;
+        const int q = c0 + gl;
+        int f;
+        double d0, d1, d2, r;
+        if (kKeep) {
+            d0 = k0[kKeep ? c : 0];
+            d1 = k1[kKeep ? c : 0];
+            d2 = k2[kKeep ? c : 0];
+            r = kr[kKeep ? c : 0];
+            f = kf[kKeep ? c : 0];
+        } else {
+            f = -1;
+            d0 = d1 = d2 = 0;
+            r = 1e300;
+            if (q < n) {
+                int j;
+                entry_geometry(g, w, pi, off + q, d0, d1, d2, r, j);
+                f = r < rce ? w.eidx[j] : -1;
+            }
         }
         int pos = -1;
 #pragma unroll
         for (int ff = 0; ff < kMaxEl; ++ff) {
             if (ff < m.nel) {
-                const unsigned long long mk = __ballot(keep && f == ff);
-                if (keep && f == ff) pos = start[ff] + run[ff] + mbcnt(mk);
+                const unsigned long long mk = group_bits<GL>(__ballot(f == ff));
+                if (f == ff) pos = start[ff] + run[ff] + __popcll(mk & below);
                 run[ff] += __popcll(mk);
             }
         }
-        if (keep) {
+        if (f >= 0) {
             const R rr = (R)r;
             R fc, dfc;
             cutoff_fn<R>(rr, (R)rce, rrc, fc, dfc);
-            L.dx[pos] = (R)d0;
-            L.dy[pos] = (R)d1;
-            L.dz[pos] = (R)d2;
-            L.r[pos] = rr;
-            L.ir[pos] = (R)(1.0 / r);
-            L.fc[pos] = fc;
-            L.dfc[pos] = dfc;
+            typename Lds<R>::V va, vb;
+            va.x = (R)d0;
+            va.y = (R)d1;
+            va.z = (R)d2;
+            va.w = (R)(1.0 / r);
+            vb.x = rr;
+            vb.y = fc;
+            vb.z = dfc;
+            vb.w = rr * rr;
+            L.a[pos] = va;
+            L.b[pos] = vb;
             L.ent[pos] = off + q;
             L.el[pos] = f;
         } else if (q < n && gent) {
@@ -324,30 +464,288 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
     }
 }
 
-// candidate p of an element-pair block -> staged indices (j, k), no division
-__device__ __forceinline__ void block_pair(int p, bool same, int nf, int ng, float inv_ng, int sf, int sg, int &j, int &k)
+// ---------------------------------------------------------------- G4
+// b^n, n >= 0 wave-uniform: square-and-multiply with scalar control (the
+// integer branch of the reference's power, symmetryFunctions.f90:265-273)
+template <typename R>
+__device__ __forceinline__ R upow(R b, int n)
 {
-    if (same) {
-        // pairs (a, b), 0 <= a < b < nf, row-major; T(a) = a*(2nf-a-1)/2
-        const float c = (float)(2 * nf - 1);
-        int a = (int)((c - sqrtf(c * c - 8.0f * (float)p)) * 0.5f);
-        a = max(0, min(a, nf - 2));
-        if (a > 0 && a * (2 * nf - a - 1) / 2 > p) --a;
-        if (a + 1 <= nf - 2 && (a + 1) * (2 * nf - a - 2) / 2 <= p) ++a;
-        j = sf + a;
-        k = sf + p - a * (2 * nf - a - 1) / 2 + a + 1;
-    } else {
-        int a = (int)(((float)p + 0.5f) * inv_ng);
-        if (a * ng > p) --a;
-        if ((a + 1) * ng <= p) ++a;
-        j = sf + a;
-        k = sg + (p - a * ng);
+    R r = R(1);
+    while (true) {
+        if (n & 1) r *= b;
+        n >>= 1;
+        if (!n) break;
+        b *= b;
+    }
+    return r;
+}
+
+// b^(zeta-1): ip = zeta-1 for integral zeta, else kNonIntegralZeta (wave-uniform)
+template <typename R>
+__device__ __forceinline__ R angpow(R b, int ip, R zm1)
+{
+    if (ip >= 0) return upow(b, ip);
+    if (ip == kNonIntegralZeta) return b > R(0) ? M<R>::pow(b, zm1) : (zm1 > R(0) ? R(0) : R(1));
+    return R(1) / upow(b, -ip);
+}
+
+// zeta = 2^sq (sq wave-uniform): bz = b^zeta by sq squarings and
+// bm = b^(zeta-1) = prod_{i<sq} b^(2^i), no selects
+template <typename R>
+__device__ __forceinline__ void pow2z(R b, int sq, R &bz, R &bm)
+{
+    R r = R(1);
+    for (int k = 0; k < sq; ++k) {
+        r *= b;
+        b *= b;
+    }
+    bz = b;
+    bm = r;
+}
+template <typename R>
+__device__ __forceinline__ R pow2z(R b, int sq)
+{
+    for (int k = 0; k < sq; ++k) b *= b;
+    return b;
+}
+
+// The candidate triplets [cb, ce) of one element-pair block of each atom,
+// compacted to those with r_jk < rc (the others give fc(r_jk) = 0). Same
+// element (segment sa, na): circulant rows, p = q*na + a is
+// (a, a + q + 1 mod na) (for even na the last row holds (a, a + na/2),
+// a < na/2). Two elements (na <= nb): diagonals, p = q*na + a is
+// (a, a + q mod nb). One division by na, through its reciprocal. cew: the
+// largest ce of the wavefront. Returns the atom's count.
+template <typename R, int GL>
+__device__ __forceinline__ int compact(const Lds<R> &L, int cb, int ce, int cew, bool same, int sa, int na,
+                                       float inv_na, int sb, int nb, R rc2)
+{
+    const int gl = lane_id() % GL;
+    const unsigned long long below = (1ull << gl) - 1;
+    int nv = 0;
+    for (int p0 = cb; p0 < cew; p0 += GL) {
+        const int p = p0 + gl;
+        bool ok = false;
+        int pk = 0;
+        if (p < ce) {
+            int q = (int)(((float)p + 0.5f) * inv_na);
+            if (q * na > p) --q;
+            if ((q + 1) * na <= p) ++q;
+            const int a = p - q * na;
+            int j, k;
+            if (same) {
+                int b = a + q + 1;
+                if (b >= na) b -= na;
+                j = sa + a;
+                k = sa + b;
+            } else {
+                int b = a + q;
+                if (b >= nb) b -= nb;
+                j = sa + a;
+                k = sb + b;
+            }
+            const typename Lds<R>::V vj = L.a[j], vk = L.a[k];
+            const R ex = vk.x - vj.x, ey = vk.y - vj.y, ez = vk.z - vj.z;
+            ok = ex * ex + ey * ey + ez * ez < rc2;
+            pk = j | (k << 16);
+        }
+        const unsigned long long mk = group_bits<GL>(__ballot(ok));
+        if (ok) L.pairs[nv + __popcll(mk & below)] = pk;
+        nv += __popcll(mk);
+    }
+    return nv;
+}
+
+constexpr int kSfChunk = 4;        // G4 functions per pass over a compacted list
+
+// Parameters of up to kSfChunk G4 functions, wave-uniform.
+template <typename R>
+struct G4Chunk {
+    R ea[kSfChunk], lam[kSfChunk], zm1[kSfChunk], pre[kSfChunk];
+    int ip[kSfChunk], nw[kSfChunk], s[kSfChunk], sq[kSfChunk];
+};
+
+template <typename R, int NC>
+__device__ __forceinline__ void load_chunk(const DevModel &m, int x0, G4Chunk<R> &c)
+{
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        const int x = x0 + t;
+        c.ea[t] = uni(g4_ea<R>(m, x));
+        c.lam[t] = uni(tabc<R>(m.g4_lam, m.g4f_lam, x));
+        c.zm1[t] = uni(tabc<R>(m.g4_zeta, m.g4f_zeta, x)) - R(1);
+        c.pre[t] = uni(tabc<R>(m.g4_pre, m.g4f_pre, x));
+        c.ip[t] = uni(cld(m.g4_ipow, x));
+        c.sq[t] = uni(cld(m.g4_sq, x));
+        c.nw[t] = t == 0 ? 1 : uni(cld(m.g4_neweta, x));
+        c.s[t] = uni(cld(m.g4_s, x));
     }
 }
 
-// candidate p of the circulant enumeration of the unordered pairs of n
-// neighbours: rows d = 1..(n-1)/2 hold (j, j+d mod n) for every j; for even
-// n the row d = n/2 holds (j, j+n/2) for j < n/2.
+// Pass A over the compacted lists for NC functions of a block.
+template <typename R, int GL, int NC, bool FAST>
+__device__ __forceinline__ void g4_values(const DevModel &m, const Lds<R> &L, int nv, int nvw, int x0, R rrc)
+{
+    G4Chunk<R> c;
+    load_chunk<R, NC>(m, x0, c);
+    const int gl = lane_id() % GL;
+    R acc[NC];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) acc[t] = 0;
+    for (int q0 = 0; q0 < nvw; q0 += GL) {
+        const int q = q0 + gl;
+        if (q >= nv) continue;
+        const int pk = L.pairs[q];
+        const int j = pk & 0xFFFF, k = pk >> 16;
+        const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+        const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk = d_ik - d_ij
+        const R rjk2 = ex * ex + ey * ey + ez * ez;
+        const R fcjk = cutoff_in<R>(M<R>::sqrt(rjk2), rrc);
+        const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * aj.w * ak.w;
+        const R F = bj.y * bk.y * fcjk;
+        const R r2 = bj.w + bk.w + rjk2;
+        R EF = 0;
+#pragma unroll
+        for (int t = 0; t < NC; ++t) {
+            if (c.nw[t]) EF = M<R>::expsc(c.ea[t], r2) * F;
+            const R b = R(1) + c.lam[t] * cs;
+            acc[t] += (FAST ? pow2z<R>(b, c.sq[t]) : b * angpow<R>(b, c.ip[t], c.zm1[t])) * EF;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        const R v = group_sum<GL>(acc[t]);
+        if (gl == 0) L.acc[c.s[t]] += c.pre[t] * v;
+    }
+}
+
+// Pass B over the compacted lists for NC functions of a block: S1..S3 of the
+// triplet, then g_j = cjj d_ij + cjk d_ik and g_k = ckk d_ik + cjk d_ij,
+// added to the FP64 per-neighbour accumulators.
+template <typename R, int GL, int NC, bool FAST>
+__device__ __forceinline__ void g4_grad(const DevModel &m, const Lds<R> &L, int nv, int nvw, int x0, R rrc)
+{
+    G4Chunk<R> c;
+    load_chunk<R, NC>(m, x0, c);
+    R q1[NC], q2[NC], q3[NC];     // w P, w P 2 eta, w P lambda zeta (w = dE/dG of the raw G)
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        q1[t] = L.acc[c.s[t]] * c.pre[t];
+        q2[t] = q1[t] * R(2) * uni(tabc<R>(m.g4_eta, m.g4f_eta, x0 + t));
+        q3[t] = q1[t] * c.lam[t] * (c.zm1[t] + R(1));
+    }
+    const int gl = lane_id() % GL;
+    for (int q0 = 0; q0 < nvw; q0 += GL) {
+        const int q = q0 + gl;
+        if (q >= nv) continue;
+        const int pk = L.pairs[q];
+        const int j = pk & 0xFFFF, k = pk >> 16;
+        const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+        const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
+        const R rjk2 = ex * ex + ey * ey + ez * ez;
+        R fcjk, dfcjk;
+        cutoff_in<R>(M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+        const R irj = aj.w, irk = ak.w;
+        const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
+        const R inv = irj * irk;
+        const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * inv;
+        const R F = fcij * fcik * fcjk;
+        const R r2 = bj.w + bk.w + rjk2;
+        R S1 = 0, S2 = 0, S3 = 0, E = 0;
+#pragma unroll
+        for (int t = 0; t < NC; ++t) {
+            if (c.nw[t]) E = M<R>::expsc(c.ea[t], r2);
+            const R b = R(1) + c.lam[t] * cs;
+            R bz, bm;
+            if (FAST) {
+                pow2z<R>(b, c.sq[t], bz, bm);
+            } else {
+                bm = angpow<R>(b, c.ip[t], c.zm1[t]);
+                bz = bm * b;
+            }
+            const R EB = E * bm;
+            const R C = E * bz;
+            S1 += C * q1[t];
+            S2 += C * q2[t];
+            S3 += EB * q3[t];
+        }
+        // dF/dR_j = a d_jk - bj d_ij, dF/dR_k = -(a d_jk + bk d_ik)
+        const R a = fcik * dfcjk * fcij;
+        const R bjj = fcik * dfcij * fcjk, bkk = fcij * dfcik * fcjk;
+        const R S2F = S2 * F, S3F = S3 * F;
+        const R beta = S1 * a - S2F;
+        const R cjk = beta - S3F * inv;
+        const R cjj = S2F - S1 * bjj + S3F * cs * irj * irj - beta;
+        const R ckk = S2F - S1 * bkk + S3F * cs * irk * irk - beta;
+        __hip_atomic_fetch_add(&L.gx[j], (double)(cjj * aj.x + cjk * ak.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gy[j], (double)(cjj * aj.y + cjk * ak.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gz[j], (double)(cjj * aj.z + cjk * ak.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gx[k], (double)(ckk * ak.x + cjk * aj.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gy[k], (double)(ckk * ak.y + cjk * aj.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gz[k], (double)(ckk * ak.z + cjk * aj.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// All G4 blocks of the centre element e, for every atom of the wavefront:
+// per block, the candidates in chunks of kPairChunk are compacted, then the
+// block's functions are evaluated kSfChunk at a time. GRAD selects pass B.
+template <typename R, int GL, bool GRAD>
+__device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, int e, R rce)
+{
+    const R rce2 = rce * rce, rrc = R(1) / rce;
+    for (int f = 0; f < m.nel; ++f) {
+        for (int h = f; h < m.nel; ++h) {
+            int tbx, tby;
+            cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, tbx, tby);
+            tbx = uni(tbx);
+            tby = uni(tby);
+            if (tby == 0) continue;
+            int sa = gu<GL>(L.seg[f]), na = gu<GL>(L.seg[f + 1]) - sa;
+            int sb = gu<GL>(L.seg[h]), nb = gu<GL>(L.seg[h + 1]) - sb;
+            const bool same = f == h;
+            if (!same && na > nb) {
+                int t = sa; sa = sb; sb = t;
+                t = na; na = nb; nb = t;
+            }
+            const int np = same ? na * (na - 1) / 2 : na * nb;
+            const int npw = GL == 64 ? np : wave_max(np);
+            if (npw == 0) continue;
+            const float inv_na = 1.0f / (float)max(na, 1);
+            bool fast = true;    // every zeta a power of two
+            for (int t = 0; t < tby; ++t) fast = fast && uni(cld(m.g4_sq, tbx + t)) >= 0;
+            for (int cb = 0; cb < npw; cb += kPairChunk) {
+                const int nv = compact<R, GL>(L, cb, min(np, cb + kPairChunk), min(npw, cb + kPairChunk), same, sa,
+                                              na, inv_na, sb, nb, rce2);
+                const int nvw = GL == 64 ? uni(nv) : wave_max(nv);
+                __syncthreads();
+                for (int c0 = 0; c0 < tby; c0 += kSfChunk) {
+                    const int x0 = tbx + c0;
+#define HDNNP_G4(NC)                                                                       \
+    case NC:                                                                               \
+        if (GRAD) {                                                                        \
+            if (fast) g4_grad<R, GL, NC, true>(m, L, nv, nvw, x0, rrc);                    \
+            else g4_grad<R, GL, NC, false>(m, L, nv, nvw, x0, rrc);                        \
+        } else {                                                                           \
+            if (fast) g4_values<R, GL, NC, true>(m, L, nv, nvw, x0, rrc);                  \
+            else g4_values<R, GL, NC, false>(m, L, nv, nvw, x0, rrc);                      \
+        }                                                                                  \
+        break;
+                    switch (min(kSfChunk, tby - c0)) { HDNNP_G4(1) HDNNP_G4(2) HDNNP_G4(3) HDNNP_G4(4) }
+#undef HDNNP_G4
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pass B, all pairs
+// Pass B does not need per-function sums, so it runs over the triplets of all
+// blocks at once with every lane busy: the unordered pairs of the n staged
+// neighbours in circulant order (row d = 1..(n-1)/2 holds (j, j+d mod n) for
+// every j; for even n the row d = n/2 holds (j, j+n/2), j < n/2), so the
+// lanes of one round touch distinct j and distinct k. Each lane reads the G4
+// parameters of its triplet's block from the LDS tables.
 __device__ __forceinline__ void circulant_pair(int p, int n, float inv_n, int &j, int &k)
 {
     const int rows = (n - 1) / 2;
@@ -367,280 +765,264 @@ __device__ __forceinline__ void circulant_pair(int p, int n, float inv_n, int &j
     if (k >= n) k -= n;
 }
 
+// G4 tables of the centre element for the all-pairs pass, with the weights
+// w = dE/dG of the raw G (in acc) folded in.
+template <typename R>
+__device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, int e, int g4s, int g4n)
+{
+    const int lane = lane_id();
+    for (int u = lane; u < g4n; u += 64) {
+        const int x = g4s + u;
+        const R lam = tabc<R>(m.g4_lam, m.g4f_lam, x), zeta = tabc<R>(m.g4_zeta, m.g4f_zeta, x);
+        const R q1 = L.acc[cld(m.g4_s, x)] * tabc<R>(m.g4_pre, m.g4f_pre, x);
+        typename Lds<R>::V p;
+        p.x = g4_ea<R>(m, x);
+        p.y = lam;
+        p.z = q1;
+        p.w = q1 * R(2) * tabc<R>(m.g4_eta, m.g4f_eta, x);
+        L.sp0[u] = p;
+        L.sp1[u] = q1 * lam * zeta;
+        const int sq = cld(m.g4_sq, x);
+        L.smeta[u] = (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0);
+    }
+    for (int t = lane; t < m.nel * kMaxEl; t += 64) {
+        const int f = t / kMaxEl, h = t - f * kMaxEl;
+        int sx = 0, cnt = 0;
+        if (h < m.nel && f <= h) {
+            cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, sx, cnt);
+            sx -= g4s;
+        }
+        L.btab[t] = I2{sx, cnt};
+    }
+}
+
+// candidates [cb, ce) of the circulant enumeration, compacted to the
+// triplets whose block has G4 functions and r_jk < rc
+template <typename R>
+__device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int n, float inv_n, R rc2)
+{
+    const int lane = lane_id();
+    int nv = 0;
+    for (int p0 = cb; p0 < ce; p0 += 64) {
+        const int p = p0 + lane;
+        bool ok = false;
+        int pk = 0;
+        if (p < ce) {
+            int j, k;
+            circulant_pair(p, n, inv_n, j, k);
+            const int ej = L.el[j], ek = L.el[k];
+            const int blk = min(ej, ek) * kMaxEl + max(ej, ek);
+            const typename Lds<R>::V vj = L.a[j], vk = L.a[k];
+            const R ex = vk.x - vj.x, ey = vk.y - vj.y, ez = vk.z - vj.z;
+            ok = L.btab[blk].y > 0 && ex * ex + ey * ey + ez * ez < rc2;
+            pk = j | (k << 10) | (blk << 20);
+        }
+        const unsigned long long mk = __ballot(ok);
+        if (ok) L.pairs[nv + mbcnt(mk)] = pk;
+        nv += __popcll(mk);
+    }
+    return nv;
+}
+
+template <typename R>
+__device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
+{
+    const int lane = lane_id();
+    for (int q = lane; q < nv; q += 64) {
+        const int pk = L.pairs[q];
+        const int j = pk & 0x3FF, k = (pk >> 10) & 0x3FF;
+        const I2 bt = L.btab[pk >> 20];
+        const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+        const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
+        const R rjk2 = ex * ex + ey * ey + ez * ez;
+        R fcjk, dfcjk;
+        cutoff_in<R>(M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+        const R irj = aj.w, irk = ak.w;
+        const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
+        const R inv = irj * irk;
+        const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * inv;
+        const R F = fcij * fcik * fcjk;
+        const R r2 = bj.w + bk.w + rjk2;
+        R S1 = 0, S2 = 0, S3 = 0, E = 0;
+        for (int t = 0; t < bt.y; ++t) {
+            const int u = bt.x + t;
+            const typename Lds<R>::V p = L.sp0[u];
+            const int mt = L.smeta[u];
+            if (t == 0 || (mt & (1 << 8))) E = M<R>::expsc(p.x, r2);
+            R b = R(1) + p.y * cs;
+            R bz, bm;
+            const int sq = mt & 0xFF;
+            if (sq != 0xFF) {
+                bm = R(1);
+                for (int kk = 0; kk < sq; ++kk) {
+                    bm *= b;
+                    b *= b;
+                }
+                bz = b;
+            } else {
+                const int x = g4s + u;
+                bm = angpow<R>(b, cld(m.g4_ipow, x), tabc<R>(m.g4_zeta, m.g4f_zeta, x) - R(1));
+                bz = bm * b;
+            }
+            const R C = E * bz;
+            S1 += C * p.z;
+            S2 += C * p.w;
+            S3 += E * bm * L.sp1[u];
+        }
+        // dF/dR_j = a d_jk - bj d_ij, dF/dR_k = -(a d_jk + bk d_ik)
+        const R a = fcik * dfcjk * fcij;
+        const R bjj = fcik * dfcij * fcjk, bkk = fcij * dfcik * fcjk;
+        const R S2F = S2 * F, S3F = S3 * F;
+        const R beta = S1 * a - S2F;
+        const R cjk = beta - S3F * inv;
+        const R cjj = S2F - S1 * bjj + S3F * cs * irj * irj - beta;
+        const R ckk = S2F - S1 * bkk + S3F * cs * irk * irk - beta;
+        __hip_atomic_fetch_add(&L.gx[j], (double)(cjj * aj.x + cjk * ak.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gy[j], (double)(cjj * aj.y + cjk * ak.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gz[j], (double)(cjj * aj.z + cjk * ak.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gx[k], (double)(ckk * ak.x + cjk * aj.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gy[k], (double)(ckk * ak.y + cjk * aj.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&L.gz[k], (double)(ckk * ak.z + cjk * aj.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
 // ---------------------------------------------------------------- pass A
 template <typename R, int NMAX>
 __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *G)
 {
+    constexpr int GA = AtomsPerWave<NMAX>::value, GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
-    __shared__ int seg[kMaxEl + 1];
-    const int i = blockIdx.x;
-    const int lane = lane_id();
-    const int e = w.eidx[i];
-    const int nsf = m.einfo[e].nsf;
-    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, 0, false, (LDS char *)smem);
-    for (int s = lane; s < nsf; s += 64) L.acc[s] = 0;
-    stage<R>(m, g, w, i, e, L, seg, (R *)nullptr);
+    int e, t0, tn;
+    elem_tile(w, m.nel, blockIdx.x, GA, e, t0, tn);
+    if (e < 0) return;
+    const int lane = lane_id(), grp = lane / GL, gl = lane % GL;
+    const int i = tn < 0 ? t0 : (grp < tn ? w.perm[t0 + grp] : -1);
+    const ElemInfo ei = cld_einfo(m.einfo, e);
+    const int nsf = ei.nsf;
+    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, 0, false,
+                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, 0, false).total);
+    for (int s = gl; s < nsf; s += GL) L.acc[s] = 0;
+    stage<R, GL, NMAX, false>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, (R *)nullptr);
     __syncthreads();
-    const R rce = (R)m.einfo[e].rc;
-    const R rce2 = rce * rce, rrc = R(1) / rce;
+    const R rce = (R)ei.rc;
 
     // G2, symmetryFunctions.f90:220-230: one lane per SF of the centre element
-    for (int t = lane; t < m.einfo[e].g2_count; t += 64) {
-        const int x = m.einfo[e].g2_start + t;
-        const int f = m.g2_f[x];
-        const R eta = tab<R>(m.g2_eta, m.g2f_eta, x), rs = tab<R>(m.g2_rs, m.g2f_rs, x);
+    for (int t = gl; t < ei.g2_count; t += GL) {
+        const int x = ei.g2_start + t;
+        const int f = cld(m.g2_f, x);
+        const R ea = g2_ea<R>(m, x), rs = tabc<R>(m.g2_rs, m.g2f_rs, x);
         R sum = 0;
-        for (int q = seg[f]; q < seg[f + 1]; ++q) {
-            const R dr = L.r[q] - rs;
-            sum += M<R>::exp(-eta * dr * dr) * L.fc[q];
+        for (int q = L.seg[f]; q < L.seg[f + 1]; ++q) {
+            const typename Lds<R>::V b = L.b[q];
+            const R dr = b.x - rs;
+            sum += M<R>::expsc(ea, dr * dr) * b.y;
         }
-        L.acc[m.g2_s[x]] += sum;
+        L.acc[cld(m.g2_s, x)] += sum;
     }
-    // G4, symmetryFunctions.f90:231-305, one element-pair block at a time
-    for (int f = 0; f < m.nel; ++f) {
-        for (int h = f; h < m.nel; ++h) {
-            const int2 tb = m.g4_tab[(e * kMaxEl + f) * kMaxEl + h];
-            if (tb.y == 0) continue;
-            const int nf = seg[f + 1] - seg[f], nh = seg[h + 1] - seg[h];
-            const bool same = f == h;
-            const int np = same ? nf * (nf - 1) / 2 : nf * nh;
-            if (np == 0) continue;
-            const float inv_nh = 1.0f / (float)max(nh, 1);
-            for (int c0 = 0; c0 < tb.y; c0 += 8) {
-                const int nc = min(8, tb.y - c0);
-                R acc[8];
-#pragma unroll
-                for (int t = 0; t < 8; ++t) acc[t] = 0;
-                for (int cb = 0; cb < np; cb += kPairChunk) {
-                    const int ce = min(np, cb + kPairChunk);
-                    int nv = 0;
-                    for (int p0 = cb; p0 < ce; p0 += 64) {      // compact r_jk < rc
-                        const int p = p0 + lane;
-                        bool ok = false;
-                        int j = 0, k = 0;
-                        if (p < ce) {
-                            block_pair(p, same, nf, nh, inv_nh, seg[f], seg[h], j, k);
-                            const R ex = L.dx[k] - L.dx[j], ey = L.dy[k] - L.dy[j], ez = L.dz[k] - L.dz[j];
-                            ok = ex * ex + ey * ey + ez * ez < rce2;
-                        }
-                        const unsigned long long mk = __ballot(ok);
-                        if (ok) L.pairs[nv + mbcnt(mk)] = j | (k << 16);
-                        nv += __popcll(mk);
-                    }
-                    __syncthreads();
-                    for (int q = lane; q < nv; q += 64) {
-                        const int pk = L.pairs[q];
-                        const int j = pk & 0xFFFF, k = pk >> 16;
-                        const R xj = L.dx[j], yj = L.dy[j], zj = L.dz[j];
-                        const R xk = L.dx[k], yk = L.dy[k], zk = L.dz[k];
-                        const R ex = xk - xj, ey = yk - yj, ez = zk - zj;   // d_jk = d_ik - d_ij
-                        const R rjk2 = ex * ex + ey * ey + ez * ez;
-                        const R rjk = M<R>::sqrt(rjk2);
-                        R fcjk, dfcjk;
-                        cutoff_fn<R>(rjk, rce, rrc, fcjk, dfcjk);
-                        const R rij = L.r[j], rik = L.r[k];
-                        const R cs = (xj * xk + yj * yk + zj * zk) * L.ir[j] * L.ir[k];
-                        const R F = L.fc[j] * L.fc[k] * fcjk;
-                        const R r2 = rij * rij + rik * rik + rjk2;
-                        R E = 0;
-#pragma unroll
-                        for (int t = 0; t < 8; ++t)
-                            if (t < nc) {
-                                const int x = tb.x + c0 + t;
-                                if (t == 0 || m.g4_neweta[x]) E = M<R>::exp(-tab<R>(m.g4_eta, m.g4f_eta, x) * r2);
-                                const R b = R(1) + tab<R>(m.g4_lam, m.g4f_lam, x) * cs;
-                                const R C = b * zpow_uniform<R>(b, m.g4_ipow[x], tab<R>(m.g4_zeta, m.g4f_zeta, x) - R(1));
-                                acc[t] += tab<R>(m.g4_pre, m.g4f_pre, x) * C * E * F;
-                            }
-                    }
-                    __syncthreads();
-                }
-#pragma unroll
-                for (int t = 0; t < 8; ++t)
-                    if (t < nc) {
-                        const R v = wave_total(acc[t]);
-                        if (lane == 0) L.acc[m.g4_s[tb.x + c0 + t]] += v;
-                    }
-            }
-        }
-    }
+    // G4, symmetryFunctions.f90:231-305
+    g4_blocks<R, GL, false>(m, L, e, rce);
     __syncthreads();
     // centre and scale, hdnnps.f90:471-481
-    const int so = m.einfo[e].sf_off;
-    for (int s = lane; s < m.nsfmax; s += 64)
+    if (i < 0) return;
+    const int so = ei.sf_off;
+    for (int s = gl; s < m.nsfmax; s += GL)
         G[(size_t)i * m.nsfmax + s] =
             s < nsf ? (R)(((double)L.acc[s] - m.sf_mean[so + s]) / m.sf_den[so + s]) : R(0);
 }
 
 // ---------------------------------------------------------------- pass B
 template <typename R, int NMAX>
-__global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const R *dEdG,
-                                                R *gent)
+__global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const R *dEdG, R *gent)
 {
+    constexpr int GA = AtomsPerWave<NMAX>::value, GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
-    __shared__ int seg[kMaxEl + 1];
-    const int i = blockIdx.x;
-    const int lane = lane_id();
-    const int e = w.eidx[i];
-    const int nsf = m.einfo[e].nsf;
-    const int so = m.einfo[e].sf_off;
-    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, m.g4_max, true, (LDS char *)smem);
+    int e, t0, tn;
+    elem_tile(w, m.nel, blockIdx.x, GA, e, t0, tn);
+    if (e < 0) return;
+    const int lane = lane_id(), grp = lane / GL, gl = lane % GL;
+    const int i = tn < 0 ? t0 : (grp < tn ? w.perm[t0 + grp] : -1);
+    const ElemInfo ei = cld_einfo(m.einfo, e);
+    const int nsf = ei.nsf;
+    const int so = ei.sf_off;
+    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, m.g4_max, true,
+                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, m.g4_max, true).total);
     // dE/dG of the unscaled G: the scaled G is (G - mean)/den
-    for (int s = lane; s < nsf; s += 64)
-        L.acc[s] = (R)((double)dEdG[(size_t)i * m.nsfmax + s] / m.sf_den[so + s]);
-    // G4 table of the centre element; block (f,g) -> (start, count) relative
-    const int g4s = m.einfo[e].g4_start, g4n = m.einfo[e].g4_count;
-    for (int t = lane; t < g4n; t += 64) {
-        const int x = g4s + t;
-        L.t_eta[t] = tab<R>(m.g4_eta, m.g4f_eta, x);
-        L.t_lam[t] = tab<R>(m.g4_lam, m.g4f_lam, x);
-        L.t_zeta[t] = tab<R>(m.g4_zeta, m.g4f_zeta, x);
-        L.t_pre[t] = tab<R>(m.g4_pre, m.g4f_pre, x);
-        L.t_s[t] = m.g4_s[x];
-        L.t_ip[t] = m.g4_ipow[x];
-        L.t_new[t] = m.g4_neweta[x];
-    }
-    for (int t = lane; t < m.nel * kMaxEl; t += 64) {
-        const int f = t / kMaxEl, h = t - f * kMaxEl;
-        int2 tb = make_int2(0, 0);
-        if (h < m.nel) {
-            tb = m.g4_tab[(e * kMaxEl + min(f, h)) * kMaxEl + max(f, h)];
-            tb.x -= g4s;
-        }
-        L.btab[2 * t] = tb.x;
-        L.btab[2 * t + 1] = tb.y;
-    }
-    stage<R>(m, g, w, i, e, L, seg, gent);
+    for (int s = gl; s < nsf; s += GL)
+        L.acc[s] = i >= 0 ? (R)((double)dEdG[(size_t)i * m.nsfmax + s] / m.sf_den[so + s]) : R(0);
+    stage<R, GL, NMAX, true>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, gent);
     __syncthreads();
-    const int n = seg[kMaxEl];
-    for (int q = lane; q < n; q += 64) L.gx[q] = L.gy[q] = L.gz[q] = 0;
+    const int n = gu<GL>(L.seg[kMaxEl]);
+    for (int q = gl; q < n; q += GL) L.gx[q] = L.gy[q] = L.gz[q] = 0;
     __syncthreads();
-    const R rce = (R)m.einfo[e].rc;
-    const R rce2 = rce * rce, rrc = R(1) / rce;
+    const R rce = (R)ei.rc;
 
     // G2: g_j = d_ij * (-dfc * sum w E + fc/r * sum w 2 eta (r - Rs) E), one lane per neighbour
-    for (int q = lane; q < n; q += 64) {
-        const int2 tb = m.g2_tab[e * kMaxEl + L.el[q]];
-        const R rr = L.r[q];
+    for (int q = gl; q < n; q += GL) {
+        int tbx, tby;
+        cld2(m.g2_tab, e * kMaxEl + L.el[q], tbx, tby);
+        const typename Lds<R>::V a = L.a[q], b = L.b[q];
         R A = 0, B = 0;
-        for (int t = 0; t < tb.y; ++t) {
-            const int x = tb.x + t;
-            const R eta = tab<R>(m.g2_eta, m.g2f_eta, x);
-            const R dr = rr - tab<R>(m.g2_rs, m.g2f_rs, x);
-            const R ws = L.acc[m.g2_s[x]] * M<R>::exp(-eta * dr * dr);
+        for (int t = 0; t < tby; ++t) {
+            const int x = tbx + t;
+            const R dr = b.x - tabc<R>(m.g2_rs, m.g2f_rs, x);
+            const R ws = L.acc[cld(m.g2_s, x)] * M<R>::expsc(g2_ea<R>(m, x), dr * dr);
             A += ws;
-            B += ws * R(2) * eta * dr;
+            B += ws * R(2) * tabc<R>(m.g2_eta, m.g2f_eta, x) * dr;
         }
-        const R c = -L.dfc[q] * A + L.fc[q] * B * L.ir[q];
-        L.gx[q] += (double)(c * L.dx[q]);
-        L.gy[q] += (double)(c * L.dy[q]);
-        L.gz[q] += (double)(c * L.dz[q]);
+        const R c = -b.z * A + b.y * B * a.w;
+        L.gx[q] += (double)(c * a.x);
+        L.gy[q] += (double)(c * a.y);
+        L.gz[q] += (double)(c * a.z);
     }
     __syncthreads();
-    // G4: per triplet, S1 = sum w P C E, S2 = sum w P C E 2 eta,
-    // S3 = sum w P b^(zeta-1) lambda zeta E, then
-    // g_j = S1 dF/dR_j + S2 F (d_ij - d_jk) + S3 F dcos/dR_j, same for k.
-    const int np = n * (n - 1) / 2;
-    const float inv_n = 1.0f / (float)max(n, 1);
-    for (int cb = 0; cb < np; cb += kPairChunk) {
-        const int ce = min(np, cb + kPairChunk);
-        int nv = 0;
-        for (int p0 = cb; p0 < ce; p0 += 64) {                  // compact r_jk < rc
-            const int p = p0 + lane;
-            bool ok = false;
-            int pk = 0;
-            if (p < ce) {
-                int j, k;
-                circulant_pair(p, n, inv_n, j, k);
-                const int ej = L.el[j], ek = L.el[k];
-                const int blk = min(ej, ek) * kMaxEl + max(ej, ek);
-                const R ex = L.dx[k] - L.dx[j], ey = L.dy[k] - L.dy[j], ez = L.dz[k] - L.dz[j];
-                ok = L.btab[2 * blk + 1] > 0 && ex * ex + ey * ey + ez * ez < rce2;
-                pk = j | (k << 10) | (blk << 20);
-            }
-            const unsigned long long mk = __ballot(ok);
-            if (ok) L.pairs[nv + mbcnt(mk)] = pk;
-            nv += __popcll(mk);
-        }
+    // G4
+    if (GA == 1) {
+        grad_tables<R>(m, L, e, ei.g4_start, ei.g4_count);
         __syncthreads();
-        for (int q = lane; q < nv; q += 64) {
-            const int pk = L.pairs[q];
-            const int j = pk & 0x3FF, k = (pk >> 10) & 0x3FF;
-            const int tbx = L.btab[2 * (pk >> 20)], tby = L.btab[2 * (pk >> 20) + 1];
-            const R xj = L.dx[j], yj = L.dy[j], zj = L.dz[j];
-            const R xk = L.dx[k], yk = L.dy[k], zk = L.dz[k];
-            const R ex = xk - xj, ey = yk - yj, ez = zk - zj;
-            const R rjk2 = ex * ex + ey * ey + ez * ez;
-            const R rjk = M<R>::sqrt(rjk2);
-            R fcjk, dfcjk;
-            cutoff_fn<R>(rjk, rce, rrc, fcjk, dfcjk);
-            const R rij = L.r[j], rik = L.r[k], irj = L.ir[j], irk = L.ir[k];
-            const R fcij = L.fc[j], fcik = L.fc[k], dfcij = L.dfc[j], dfcik = L.dfc[k];
-            const R inv = irj * irk;
-            const R cs = (xj * xk + yj * yk + zj * zk) * inv;
-            const R F = fcij * fcik * fcjk;
-            const R r2 = rij * rij + rik * rik + rjk2;
-            R S1 = 0, S2 = 0, S3 = 0, E = 0;
-            for (int t = tbx; t < tbx + tby; ++t) {
-                const R eta = L.t_eta[t];
-                if (t == tbx || L.t_new[t]) E = M<R>::exp(-eta * r2);
-                const R lam = L.t_lam[t];
-                const R zeta = L.t_zeta[t];
-                const R b = R(1) + lam * cs;
-                const R Bp = zpow_lane<R>(b, L.t_ip[t], zeta - R(1), m.powbits);
-                const R wpe = L.acc[L.t_s[t]] * L.t_pre[t] * E;
-                const R wpc = wpe * Bp * b;
-                S1 += wpc;
-                S2 += wpc * R(2) * eta;
-                S3 += wpe * Bp * lam * zeta;
-            }
-            const R aj = fcik * dfcjk * fcij, bj = fcik * dfcij * fcjk;   // dF/dR_j = aj d_jk - bj d_ij
-            const R ak = fcij * dfcjk * fcik, bk = fcij * dfcik * fcjk;   // dF/dR_k = -(ak d_jk + bk d_ik)
-            const R cj = cs * irj * irj, ck = cs * irk * irk;
-            const R S2F = S2 * F, S3F = S3 * F;
-            const R gjx = S1 * (aj * ex - bj * xj) + S2F * (xj - ex) + S3F * (-xk * inv + xj * cj);
-            const R gjy = S1 * (aj * ey - bj * yj) + S2F * (yj - ey) + S3F * (-yk * inv + yj * cj);
-            const R gjz = S1 * (aj * ez - bj * zj) + S2F * (zj - ez) + S3F * (-zk * inv + zj * cj);
-            const R gkx = -S1 * (ak * ex + bk * xk) + S2F * (xk + ex) + S3F * (-xj * inv + xk * ck);
-            const R gky = -S1 * (ak * ey + bk * yk) + S2F * (yk + ey) + S3F * (-yj * inv + yk * ck);
-            const R gkz = -S1 * (ak * ez + bk * zk) + S2F * (zk + ez) + S3F * (-zj * inv + zk * ck);
-            lds_add<double>(&L.gx[j], (double)gjx);
-            lds_add<double>(&L.gy[j], (double)gjy);
-            lds_add<double>(&L.gz[j], (double)gjz);
-            lds_add<double>(&L.gx[k], (double)gkx);
-            lds_add<double>(&L.gy[k], (double)gky);
-            lds_add<double>(&L.gz[k], (double)gkz);
+        const R rce2 = rce * rce, rrc = R(1) / rce;
+        const int np = n * (n - 1) / 2;
+        const float inv_n = 1.0f / (float)max(n, 1);
+        for (int cb = 0; cb < np; cb += kPairChunk) {
+            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk), n, inv_n, rce2));
+            __syncthreads();
+            g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
+            __syncthreads();
         }
-        __syncthreads();
+    } else {
+        g4_blocks<R, GL, true>(m, L, e, rce);
     }
     // per-entry gradient rows and the atom's virial -sum_n g_n (x) d_in
     double v[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) v[t] = 0;
-    for (int q = lane; q < n; q += 64) {
+    for (int q = gl; q < n; q += GL) {
         const double gx = L.gx[q], gy = L.gy[q], gz = L.gz[q];
         const size_t en = (size_t)L.ent[q];
         gent[3 * en + 0] = (R)gx;
         gent[3 * en + 1] = (R)gy;
         gent[3 * en + 2] = (R)gz;
-        const double dxq = L.dx[q], dyq = L.dy[q], dzq = L.dz[q];
+        const typename Lds<R>::V a = L.a[q];
+        const double dxq = a.x, dyq = a.y, dzq = a.z;
         v[0] -= gx * dxq; v[1] -= gx * dyq; v[2] -= gx * dzq;
         v[3] -= gy * dxq; v[4] -= gy * dyq; v[5] -= gy * dzq;
         v[6] -= gz * dxq; v[7] -= gz * dyq; v[8] -= gz * dzq;
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-        const double s = wave_total(v[t]);
-        if (lane == 0) w.vir[9 * (size_t)i + t] = s;
+        const double s = group_sum<GL>(v[t]);
+        if (gl == 0 && i >= 0) w.vir[9 * (size_t)i + t] = s;
     }
 }
+
+// wavefronts of a launch: tiles of GA atoms of one element
+static int sf_blocks(const DevModel &m, const Work &w, int ga) { return ga == 1 ? w.nat : (w.nat + ga - 1) / ga + m.nel; }
 
 template <typename R>
 void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, R *G, hipStream_t s)
 {
-#define HDNNP_SFV(N)                                                                          \
-    hipLaunchKernelGGL((k_sf_values<R, N>), dim3(w.nat), dim3(64),                         \
-                       lds_bytes<R>(N, m.nsfmax, 0, false), s, m, g, w, G)
+#define HDNNP_SFV(N)                                                                               \
+    hipLaunchKernelGGL((k_sf_values<R, N>), dim3(sf_blocks(m, w, AtomsPerWave<N>::value)), dim3(64), \
+                       lds_bytes<R>(N, AtomsPerWave<N>::value, m.nsfmax, 0, false), s, m, g, w, G)
     if (nmax <= 64) HDNNP_SFV(64);
     else if (nmax <= 128) HDNNP_SFV(128);
     else if (nmax <= 256) HDNNP_SFV(256);
@@ -653,9 +1035,9 @@ template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, const R *dEdG, R *gent,
                     hipStream_t s)
 {
-#define HDNNP_SFG(N)                                                                          \
-    hipLaunchKernelGGL((k_sf_grad<R, N>), dim3(w.nat), dim3(64),                           \
-                       lds_bytes<R>(N, m.nsfmax, m.g4_max, true), s, m, g, w, dEdG, gent)
+#define HDNNP_SFG(N)                                                                               \
+    hipLaunchKernelGGL((k_sf_grad<R, N>), dim3(sf_blocks(m, w, AtomsPerWave<N>::value)), dim3(64),   \
+                       lds_bytes<R>(N, AtomsPerWave<N>::value, m.nsfmax, m.g4_max, true), s, m, g, w, dEdG, gent)
     if (nmax <= 64) HDNNP_SFG(64);
     else if (nmax <= 128) HDNNP_SFG(128);
     else if (nmax <= 256) HDNNP_SFG(256);
@@ -679,10 +1061,16 @@ int sf_bucket(int nmax)
     return b;
 }
 
+static int atoms_per_wave(int nmax)
+{
+    return nmax <= 128 ? AtomsPerWave<128>::value : (nmax <= 256 ? AtomsPerWave<256>::value : AtomsPerWave<512>::value);
+}
+
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad)
 {
     nmax = sf_bucket(nmax);
-    return fp64 ? lds_bytes<double>(nmax, nsfmax, g4max, grad) : lds_bytes<float>(nmax, nsfmax, g4max, grad);
+    const int ga = atoms_per_wave(nmax);
+    return fp64 ? lds_bytes<double>(nmax, ga, nsfmax, g4max, grad) : lds_bytes<float>(nmax, ga, nsfmax, g4max, grad);
 }
 
 }  // namespace hdnnp

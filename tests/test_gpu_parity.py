@@ -157,6 +157,40 @@ def test_flat_and_linear_inputs(pkg, wl, oracles):
     np.testing.assert_allclose(F, r["force"], rtol=0, atol=1e-10)
 
 
+def test_general_zeta_against_oracle(pkg, wl, oracle, tmp_path):
+    """G4 functions with zeta = 3 (integral, not a power of two) and 2.5
+    (non-integral) take the general power paths of the kernels; the example
+    potential with such zeta patched in is compared with the oracle."""
+    import shutil
+    d = tmp_path / "pot"
+    shutil.copytree(MA_PARAMS, d, copy_function=shutil.copyfile)   # the tree may be read-only
+    lines = (d / "input.nn").read_text().splitlines()
+    k = 0
+    for n, line in enumerate(lines):
+        f = line.split()
+        if f and f[0] == "symfunction_short" and f[2] == "3":
+            if k % 4 == 1:
+                f[-2] = "3.0"
+            elif k % 4 == 2:
+                f[-2] = "2.5"
+            lines[n] = "  ".join(f)
+            k += 1
+    (d / "input.nn").write_text("\n".join(lines) + "\n")
+    path = str(d) + "/"
+    o = oracle.Oracle(oracle.read_potential(path))
+    s = golden_structure(wl, golden("eval_ma_cell8"))
+    r = o.evaluate(s)
+    h = pkg.parseRuNNerInput(path, "fp64")
+    E, F, L = h.energy_and_forces(s)
+    assert abs(E - r["energy"]) <= 1e-10 * max(1.0, abs(r["energy"]))
+    np.testing.assert_allclose(F, r["force"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(L, r["dEdLat"], rtol=0, atol=1e-9 * max(1.0, np.abs(r["dEdLat"]).max()))
+    hm = pkg.parseRuNNerInput(path, "mixed")
+    Em, Fm, _ = hm.energy_and_forces(s)
+    assert np.sqrt(np.mean((Fm - F) ** 2)) <= 1e-5
+    assert abs(Em - E) / s.nat <= 1e-5
+
+
 def _run(args):
     return subprocess.run(args, capture_output=True, text=True, timeout=300)
 
