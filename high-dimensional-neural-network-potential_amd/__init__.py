@@ -28,7 +28,7 @@ _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
-EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
+EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
@@ -63,6 +63,7 @@ def load_library():
         lib.hdnnp_create.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(vp)]
         lib.hdnnp_evaluate.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.hdnnp_evaluate_device.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, _dp, _dp, vp, _dp, vp]
+        lib.hdnnp_evaluate_batch.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp]
         lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
         lib.hdnnp_get_symfunctions.argtypes = [vp, ctypes.c_int32, _dp, _dp, _dp]
@@ -149,6 +150,27 @@ class Hdnnp:
         self._check(self.lib.hdnnp_evaluate(self.ctx, nat, _p(Z, _ip), _p(pos, _dp), int(bool(s.periodic)),
                                             _p(lat, _dp), ctypes.byref(E), _p(F, _dp), _p(L, _dp)))
         return E.value, F, L
+
+    def energy_and_forces_batch(self, structures, want_dEdLat: bool = True):
+        """hdnnp_evaluate_batch: many structures in one evaluation. Returns
+        (E[nstruct], [F_k (nat_k, 3)], dEdLat[nstruct, 3, 3] | None)."""
+        ns = len(structures)
+        nat = np.array([int(np.asarray(s.Z).shape[0]) for s in structures], dtype=np.int32)
+        Z = np.ascontiguousarray(np.concatenate([np.asarray(s.Z, dtype=np.int32) for s in structures])
+                                 if ns else np.zeros(0, np.int32))
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(s.pos, dtype=np.float64).reshape(-1, 3)
+                                                   for s in structures]) if ns else np.zeros((0, 3)))
+        per = np.array([int(bool(s.periodic)) for s in structures], dtype=np.int32)
+        lat = np.ascontiguousarray(np.stack([np.asarray(s.lat, dtype=np.float64).reshape(3, 3)
+                                             for s in structures]) if ns else np.zeros((0, 3, 3)))
+        E = np.zeros(ns)
+        F = np.zeros((int(nat.sum()), 3))
+        L = np.zeros((ns, 3, 3)) if want_dEdLat else None
+        self._check(self.lib.hdnnp_evaluate_batch(self.ctx, ns, _p(nat, _ip), _p(Z, _ip), _p(pos, _dp),
+                                                  _p(per, _ip), _p(lat, _dp), _p(E, _dp), _p(F, _dp),
+                                                  _p(L, _dp)))
+        off = np.concatenate([[0], np.cumsum(nat)])
+        return E, [F[off[k]:off[k + 1]] for k in range(ns)], L
 
     def evaluate_device(self, Z_dev, pos_dev, force_dev, periodic: bool, lat, stream=None):
         """Device-resident evaluation: Z_dev/pos_dev/force_dev are device

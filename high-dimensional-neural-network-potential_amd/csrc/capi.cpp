@@ -65,6 +65,15 @@ int hdnnp_evaluate(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *
     return guarded(ctx, [&] { ctx->eng->evaluate_host(nat, Z, pos, periodic, lat, energy, force, dEdLat); });
 }
 
+int hdnnp_evaluate_batch(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z, const double *pos,
+                         const int32_t *periodic, const double *lat, double *energy, double *force, double *dEdLat)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng->evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, energy, force, dEdLat);
+    });
+}
+
 int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev, const double *pos_dev,
                           int32_t periodic, const double *lat, double *energy, double *force_dev,
                           double *dEdLat, void *stream)

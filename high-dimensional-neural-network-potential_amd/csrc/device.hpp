@@ -63,22 +63,37 @@ struct DevModel {
     int fallback_el;             // usable element given to flagged atoms (results discarded)
 };
 
-// Cell grid of one evaluation (host-computed, reference formulas).
-struct Geo {
+// Cell grid of one structure of an evaluation (host-computed, reference
+// formulas). The structures of a batch are concatenated: atoms
+// [atom_off, atom_off + nat), cells [cell_off, cell_off + ng0*ng1*ng2).
+struct StructGeo {
     double lat[9];      // list-frame lattice, lat[3k+a] = component a of vector k
     double inv[9];      // its inverse (inv3DM form)
     double origin[3];   // artificial-box origin (nonperiodic), else 0
-    double rc2;         // (largest cutoff)^2
     int periodic;
     int ng[3];          // grid cells per lattice vector
     int nc[3];          // stencil half-widths (getNcells of the grid)
-    int nat;
+    int S;              // stencil cells, (2nc0+1)(2nc1+1)(2nc2+1)
+    int atom_off, nat;
+    int cell_off, ncell;
+    long long ps_off;   // first pos_start row of its atoms
+};
+
+// Kernel argument: the structures of one evaluation.
+struct Geo {
+    const StructGeo *sg;    // [nstruct], device memory
+    double rc2;             // (largest cutoff)^2
+    int nstruct;
+    int ncell;              // cells of all structures
+    int nat;                // atoms of all structures
 };
 
 // Device work arrays of one evaluation.
 struct Work {
-    int nat;
+    int nat;             // atoms of all structures
     const double *pos;   // [nat][3] caller positions
+    int *sid;            // [nat] structure of each atom
+    int *csid;           // [ncell] structure of each cell
     const int *Z;        // [nat]
     int *eidx;           // element rank per atom
     double4 *wpos;       // wrapped positions in the list frame
@@ -95,8 +110,8 @@ struct Work {
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)
     int *ent_s;          // [nat*cap] stencil-cell index of the entry
     int *rev;            // [nat*cap] partner entry in the neighbour's list, -1 if none
-    int nstencil;        // stencil cells per grid cell, S
-    unsigned short *pos_start;   // [nat*S] first list slot of each stencil cell, or null
+    unsigned short *pos_start;   // per atom, S rows (StructGeo::ps_off): first list slot of
+                                 // each stencil cell, or null
     int *perm;           // atoms sorted by element
     int *elem_off;       // [kMaxEl+1] start of each element in perm
     int *elem_count;     // [kMaxEl]
@@ -105,7 +120,7 @@ struct Work {
     double *Ei;          // [nat]
     double *vir;         // [nat][9]
     double *force;       // [nat][3]
-    double *result;      // [16]: energy, W[9]
+    double *result;      // [nstruct][10]: energy, W[9] of each structure
     double *bbox;        // [6]
 };
 
@@ -135,10 +150,11 @@ __host__ __device__ inline void unpack_shift(int p, int &x, int &y, int &z)
 // ---- launchers (kernels_nl.hip) ----
 void launch_map_elements(const DevModel &m, Work &w, hipStream_t s);
 void launch_bbox(Work &w, hipStream_t s);
+void launch_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells, hipStream_t s);
 void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s);
 void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s);
-void launch_nl_rev(Work &w, hipStream_t s);
+void launch_nl_rev(const Geo &g, Work &w, hipStream_t s);
 void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s);
 void launch_fill_int(int *p, int v, long n, hipStream_t s);
 
@@ -156,6 +172,9 @@ size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, hipStream_t s);
-void launch_reduce(Work &w, double *partial, hipStream_t s);   // partial: 2560 doubles
+// per-structure sums of Ei and vir: chunks [c.x, c.y) of atoms never cross a
+// structure; structure s owns chunks [sc[s], sc[s+1]); partial: 10 per chunk
+void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
+                   hipStream_t s);
 
 }  // namespace hdnnp

@@ -123,9 +123,10 @@ Engine::Engine(HostModel m, int precision) : model_(std::move(m)), precision_(pr
 Engine::~Engine()
 {
     for (auto &d : model_mem_) d.release();
-    for (DevMem *d : {&zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
+    for (DevMem *d : {&geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
+                      &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
                       &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
-                      &vir_, &misc_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &red_})
+                      &vir_, &misc_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_})
         d->release();
     if (pinned_) (void)hipHostFree(pinned_);
     for (auto &e : ev_)
@@ -148,7 +149,6 @@ void Engine::init_device()
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(5, "no HIP device available");
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipHostMalloc(&pinned_, 4096, hipHostMallocDefault));
 
     const HostModel &m = model_;
     DevModel d{};
@@ -314,7 +314,9 @@ static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // i
 static constexpr size_t kDblOffset = 256;                                    // bytes
 // largest per-(atom, stencil cell) slot table; beyond it partners are found by bisection
 static constexpr size_t kPosStartBytes = size_t(1) << 30;
-static constexpr int kDblResult = 0, kDblBbox = 16;                          // doubles
+static constexpr int kDblBbox = 16;                                          // doubles
+// atoms per block of the first reduction stage
+static constexpr int kReduceChunk = 512;
 
 template <typename R>
 void Engine::compute(Work &w, const Geo &g, hipStream_t s)
@@ -342,27 +344,116 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s)
     HIPCHK(hipEventRecord(ev_[5], s));
 
     launch_force_gather_t<R>(w, gent, s);
-    red_.ensure(8 * 2560);
-    launch_reduce(w, red_.as<double>(), s);
     HIPCHK(hipGetLastError());
 }
 
-void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
-                 double *force_dev, hipStream_t s, double *energy, double *dEdLat, int retry)
+// pinned host staging, grown on demand (only touched between synchronised runs)
+void *Engine::staging(size_t bytes)
 {
-    if (nat < 0) throw Error(1, "nat < 0");
-    if (periodic && !lat) throw Error(1, "periodic structure without lattice");
+    if (bytes > pinned_bytes_) {
+        if (pinned_) HIPCHK(hipHostFree(pinned_));
+        pinned_ = nullptr;
+        pinned_bytes_ = 0;
+        const size_t want = std::max<size_t>(4096, bytes + bytes / 2);
+        HIPCHK(hipHostMalloc(&pinned_, want, hipHostMallocDefault));
+        pinned_bytes_ = want;
+    }
+    return pinned_;
+}
+
+// Cell grid of one structure (neighborLists.f90:131-169): lattice (periodic)
+// or artificial box from the bounding box (nonperiodic), grid cells of at
+// least 0.6 rc, stencil half-widths from getNcells of the grid.
+static void structure_grid(const StructIn &in, double rc, StructGeo &G)
+{
+    std::memset(&G, 0, sizeof G);
+    G.periodic = in.periodic ? 1 : 0;
+    if (in.periodic) {
+        std::memcpy(G.lat, in.lat, sizeof G.lat);
+    } else {                                     // neighborLists.f90:131-138
+        for (int a = 0; a < 3; ++a) {
+            G.origin[a] = in.bbox[a] - 1.e-4;
+            G.lat[a + 3 * a] = (in.bbox[3 + a] - G.origin[a]) + 1.e-4;
+        }
+    }
+    inv3(G.lat, G.inv);
+    double grid[9];
+    const double target = rc * 0.6;               // neighborLists.f90:142-146
+    for (int a = 0; a < 3; ++a) {
+        const double *v = G.lat + 3 * a;
+        const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (!(len > 0) || !std::isfinite(len)) throw Error(1, "degenerate lattice");
+        const double ngd = std::floor(len / target) + 1;
+        if (ngd > 1e6) throw Error(6, "cell grid too large");
+        G.ng[a] = (int)ngd;
+        for (int k = 0; k < 3; ++k) grid[3 * a + k] = v[k] / G.ng[a];
+    }
+    get_ncells(grid, rc, G.nc);                   // neighborLists.f90:169
+    // nonperiodic: offsets outside the grid are skipped anyway, so the stencil
+    // is clamped to the grid (same cells visited; flat inputs stay cheap)
+    if (!in.periodic)
+        for (int a = 0; a < 3; ++a) G.nc[a] = std::min(G.nc[a], G.ng[a] - 1);
+    const long ncell = (long)G.ng[0] * G.ng[1] * G.ng[2];
+    if (ncell > (1L << 26)) throw Error(6, "cell grid too large");
+    const long S = (long)(2 * G.nc[0] + 1) * (2 * G.nc[1] + 1) * (2 * G.nc[2] + 1);
+    if (S > (1L << 22)) throw Error(6, "neighbour stencil too large");
+    G.ncell = (int)ncell;
+    G.S = (int)S;
+    G.nat = in.nat;
+}
+
+void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,
+                 hipStream_t s, double *energy, double *dEdLat, int retry)
+{
+    const int nstruct = (int)st.size();
     const double rc = model_.maxrc;
     if (!(rc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
     if (dm_.fallback_el < 0) throw Error(3, elem_why_[0].empty() ? "no element of the potential can be evaluated"
                                                                  : elem_why_[0]);
-    if (nat == 0) {
-        *energy = 0;
-        if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double));
-        last_nat_ = 0;
-        last_total_ = 0;
-        return;
+    long nat_l = 0;
+    for (const StructIn &in : st) {
+        if (in.nat < 0) throw Error(1, "nat < 0");
+        nat_l += in.nat;
     }
+    if (nat_l >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
+    const int nat = (int)nat_l;
+    for (int k = 0; k < nstruct; ++k) energy[k] = 0;
+    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double) * nstruct);
+    last_nat_ = 0;
+    last_total_ = 0;
+    if (nat == 0) return;
+
+    // per-structure grids, offsets and the neighbour-capacity estimate
+    hgeo_.resize(nstruct);
+    const bool fp64 = precision_ == 0;
+    int lds_lim = 0;
+    for (int c = 64; c <= 1024; c *= 2)
+        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536 &&
+            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, false) <= 65536)
+            lds_lim = c;
+    if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
+    long cell_off = 0, ps_total = 0;
+    int atom_off = 0;
+    double est = 0;
+    for (int k = 0; k < nstruct; ++k) {
+        StructGeo &G = hgeo_[k];
+        structure_grid(st[k], rc, G);
+        G.atom_off = atom_off;
+        G.cell_off = (int)cell_off;
+        G.ps_off = ps_total;
+        atom_off += G.nat;
+        cell_off += G.ncell;
+        ps_total += (long)G.nat * G.S;
+        if (cell_off > (1L << 28)) throw Error(6, "cell grids too large");
+        if (G.nat > 0) {
+            const double vol = std::fabs(det3(G.lat));
+            double e = 4.0 / 3.0 * M_PI * rc * rc * rc * G.nat / std::max(vol, 1e-300);
+            if (!G.periodic) e = std::min(e, (double)(G.nat - 1));
+            est = std::max(est, e);
+        }
+    }
+    const int ncell = (int)cell_off;
+
     HIPCHK(hipEventRecord(ev_[0], s));
     // per-atom buffers
     eidx_.ensure(4 * (size_t)nat);
@@ -376,6 +467,48 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     eidx_sorted_.ensure(4 * (size_t)nat);
     Ei_.ensure(8 * (size_t)nat);
     vir_.ensure(72 * (size_t)nat);
+    sid_.ensure(4 * (size_t)nat);
+    csid_.ensure(4 * (size_t)ncell);
+
+    // reduction chunks: never across a structure
+    std::vector<int2> chunks;
+    std::vector<int> sc(nstruct + 1);
+    for (int k = 0; k < nstruct; ++k) {
+        sc[k] = (int)chunks.size();
+        const StructGeo &G = hgeo_[k];
+        for (int a = 0; a < G.nat; a += kReduceChunk)
+            chunks.push_back(make_int2(G.atom_off + a, G.atom_off + std::min(G.nat, a + kReduceChunk)));
+    }
+    sc[nstruct] = (int)chunks.size();
+    const int nchunk = (int)chunks.size();
+
+    // host -> device tables through the pinned staging buffer
+    const size_t geo_b = sizeof(StructGeo) * nstruct, ch_b = sizeof(int2) * nchunk, sc_b = 4 * (nstruct + 1);
+    const size_t res_b = 8 * 10 * (size_t)nstruct;
+    const size_t o_geo = 0, o_ch = (geo_b + 255) / 256 * 256, o_sc = o_ch + (ch_b + 255) / 256 * 256,
+                 o_res = o_sc + (sc_b + 255) / 256 * 256, o_flags = o_res + (res_b + 255) / 256 * 256,
+                 o_bbox = o_flags + 256;
+    char *hs = (char *)staging(o_bbox + 256);
+    // the tables are uploaded only when they differ from the last upload
+    // (repeated evaluations of one geometry, as in MD, skip the copies)
+    const bool same = uploaded_.size() == o_res && geo_.p && chunks_.p && sc_.p &&
+                      std::memcmp(uploaded_.data(), hgeo_.data(), geo_b) == 0 &&
+                      std::memcmp(uploaded_.data() + o_ch, chunks.data(), ch_b) == 0 &&
+                      std::memcmp(uploaded_.data() + o_sc, sc.data(), sc_b) == 0;
+    res_.ensure(res_b);
+    partial_.ensure(8 * 10 * (size_t)std::max(nchunk, 1));
+    if (!same) {
+        std::memcpy(hs + o_geo, hgeo_.data(), geo_b);
+        std::memcpy(hs + o_ch, chunks.data(), ch_b);
+        std::memcpy(hs + o_sc, sc.data(), sc_b);
+        geo_.ensure(geo_b);
+        chunks_.ensure(std::max<size_t>(ch_b, 8));
+        sc_.ensure(sc_b);
+        HIPCHK(hipMemcpyAsync(geo_.p, hs + o_geo, geo_b, hipMemcpyHostToDevice, s));
+        if (nchunk) HIPCHK(hipMemcpyAsync(chunks_.p, hs + o_ch, ch_b, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(sc_.p, hs + o_sc, sc_b, hipMemcpyHostToDevice, s));
+        uploaded_.assign(hs, hs + o_res);
+    }
 
     int *mi = misc_.as<int>();
     double *md = (double *)((char *)misc_.p + kDblOffset);
@@ -383,6 +516,8 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     w.nat = nat;
     w.pos = pos_dev;
     w.Z = Z_dev;
+    w.sid = sid_.as<int>();
+    w.csid = csid_.as<int>();
     w.eidx = eidx_.as<int>();
     w.wpos = wpos_.as<double4>();
     w.cellkey = cellkey_.as<int>();
@@ -396,50 +531,25 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     w.Ei = Ei_.as<double>();
     w.vir = vir_.as<double>();
     w.force = force_dev;
-    w.result = md + kDblResult;
+    w.result = res_.as<double>();
     w.bbox = md + kDblBbox;
 
     HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
     launch_map_elements(dm_, w, s);
 
     Geo g{};
-    g.periodic = periodic ? 1 : 0;
-    g.nat = nat;
+    g.sg = geo_.as<StructGeo>();
     g.rc2 = rc * rc;
-    double *pin = (double *)pinned_;
-    if (periodic) {
-        std::memcpy(g.lat, lat, sizeof g.lat);
-    } else {                                     // neighborLists.f90:131-138
-        launch_bbox(w, s);
-        HIPCHK(hipMemcpyAsync(pin, w.bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::memset(g.lat, 0, sizeof g.lat);
-        for (int a = 0; a < 3; ++a) {
-            g.origin[a] = pin[a] - 1.e-4;
-            g.lat[a + 3 * a] = (pin[3 + a] - g.origin[a]) + 1.e-4;
-        }
+    g.nstruct = nstruct;
+    g.ncell = ncell;
+    g.nat = nat;
+    if (nstruct == 1) {
+        launch_fill_int(w.sid, 0, nat, s);
+        launch_fill_int(w.csid, 0, ncell, s);
+    } else {
+        launch_segment_ids(w.sid, nat, g.sg, nstruct, false, s);
+        launch_segment_ids(w.csid, ncell, g.sg, nstruct, true, s);
     }
-    inv3(g.lat, g.inv);
-    double grid[9];
-    const double target = rc * 0.6;               // neighborLists.f90:142-146
-    for (int a = 0; a < 3; ++a) {
-        const double *v = g.lat + 3 * a;
-        const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (!(len > 0) || !std::isfinite(len)) throw Error(1, "degenerate lattice");
-        const double ngd = std::floor(len / target) + 1;
-        if (ngd > 1e6) throw Error(6, "cell grid too large");
-        g.ng[a] = (int)ngd;
-        for (int k = 0; k < 3; ++k) grid[3 * a + k] = v[k] / g.ng[a];
-    }
-    get_ncells(grid, rc, g.nc);                   // neighborLists.f90:169
-    // nonperiodic: offsets outside the grid are skipped anyway, so the stencil
-    // is clamped to the grid (same cells visited; flat inputs stay cheap)
-    if (!periodic)
-        for (int a = 0; a < 3; ++a) g.nc[a] = std::min(g.nc[a], g.ng[a] - 1);
-    const long ncell = (long)g.ng[0] * g.ng[1] * g.ng[2];
-    if (ncell > (1L << 26)) throw Error(6, "cell grid too large");
-    const long S = (long)(2 * g.nc[0] + 1) * (2 * g.nc[1] + 1) * (2 * g.nc[2] + 1);
-    if (S > (1L << 22)) throw Error(6, "neighbour stencil too large");
     int bits = 1;
     while ((1L << bits) < ncell) ++bits;
 
@@ -458,7 +568,7 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     cell_end_.ensure(4 * (size_t)ncell);
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
-    launch_cell_range(g, w, (int)ncell, s);
+    launch_cell_range(g, w, ncell, s);
     // atoms sorted by element (MLP tiles); needs no host round trip
     bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
@@ -471,17 +581,7 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     // estimate from the mean density, rounded up to a staging size of the
     // symmetry-function kernels. An atom with more neighbours raises flags[1]
     // and the evaluation is repeated with room for it.
-    const bool fp64 = precision_ == 0;
-    int lds_lim = 0;
-    for (int c = 64; c <= 1024; c *= 2)
-        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536 &&
-            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, false) <= 65536)
-            lds_lim = c;
-    if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
     {
-        const double vol = std::fabs(det3(g.lat));
-        double est = 4.0 / 3.0 * M_PI * rc * rc * rc * nat / std::max(vol, 1e-300);
-        if (!periodic) est = std::min(est, (double)(nat - 1));
         const double want = std::max((double)cap_hint_, std::min(1.2 * est + 8.0, 1e6));
         int cap = 64;
         while (cap < want && cap < lds_lim) cap *= 2;
@@ -499,14 +599,13 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
     w.ent_shift = ent_shift_.as<int>();
     w.ent_s = ent_s_.as<int>();
     w.rev = rev_.as<int>();
-    w.nstencil = (int)S;
     w.pos_start = nullptr;
-    if ((size_t)nat * S * 2 <= kPosStartBytes) {
-        pos_start_.ensure((size_t)nat * S * 2);
+    if ((size_t)ps_total * 2 <= kPosStartBytes) {
+        pos_start_.ensure((size_t)ps_total * 2);
         w.pos_start = pos_start_.as<unsigned short>();
     }
-    launch_nl_build(g, w, (int)ncell, s);
-    launch_nl_rev(w, s);
+    launch_nl_build(g, w, ncell, s);
+    launch_nl_rev(g, w, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[2], s));
 
@@ -514,29 +613,31 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
         compute<double>(w, g, s);
     else
         compute<float>(w, g, s);
+    launch_reduce(w, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), s);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[6], s));
-    int *pi = (int *)pinned_;
-    HIPCHK(hipMemcpyAsync(pin + 64, w.result, 10 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(pi + 40, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+    double *hres = (double *)(hs + o_res);
+    int *hflags = (int *)(hs + o_flags);
+    HIPCHK(hipMemcpyAsync(hres, w.result, res_b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hflags, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    const int *flags = pi + 40;
-    if (flags[0]) throw Error(4, "structure contains element Z=" + std::to_string(flags[0]) +
-                                     " that the potential does not describe");
-    if (flags[3]) {
+    if (hflags[0]) throw Error(4, "structure contains element Z=" + std::to_string(hflags[0]) +
+                                      " that the potential does not describe");
+    if (hflags[3]) {
         for (int e = 0; e < dm_.nel; ++e)
-            if (model_.el[e].Z == flags[3]) throw Error(3, elem_why_[e]);
-        throw Error(3, "element Z=" + std::to_string(flags[3]) + " cannot be evaluated");
+            if (model_.el[e].Z == hflags[3]) throw Error(3, elem_why_[e]);
+        throw Error(3, "element Z=" + std::to_string(hflags[3]) + " cannot be evaluated");
     }
-    if (flags[1] > 0) {
-        if (flags[1] > lds_lim)
-            throw Error(6, "an atom has " + std::to_string(flags[1]) + " neighbours; this build stages at most " +
+    if (hflags[1] > 0) {
+        if (hflags[1] > lds_lim)
+            throw Error(6, "an atom has " + std::to_string(hflags[1]) + " neighbours; this build stages at most " +
                                std::to_string(lds_lim) + " per atom in LDS");
         if (retry == 0) throw Error(6, "neighbour capacity did not converge");
-        cap_hint_ = std::max(cap_hint_, flags[1]);
-        run(nat, Z_dev, pos_dev, periodic, lat, force_dev, s, energy, dEdLat, retry - 1);
+        cap_hint_ = std::max(cap_hint_, hflags[1]);
+        run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, retry - 1);
         return;
     }
-    if (flags[2]) throw Error(6, "periodic image shift beyond +-127 cells");
+    if (hflags[2]) throw Error(6, "periodic image shift beyond +-127 cells");
     last_nat_ = nat;
     last_cap_ = w.cap;
     last_total_ = -1;
@@ -545,40 +646,73 @@ void Engine::run(int nat, const int *Z_dev, const double *pos_dev, int periodic,
         HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
         ms_[k] = ms;
     }
-    const double *res = pin + 64;
-    *energy = res[0];
-    if (dEdLat) {
-        std::memset(dEdLat, 0, 9 * sizeof(double));
-        if (periodic) {    // dEdLat(a,x) = sum_b W(a,b) inv(lat)(x,b), stored at [3x + a]
+    for (int k = 0; k < nstruct; ++k) {
+        const double *res = hres + 10 * (size_t)k;
+        energy[k] = res[0];
+        if (dEdLat && st[k].periodic) {   // dEdLat(a,x) = sum_b W(a,b) inv(lat)(x,b), stored at [3x + a]
             double il[9];
-            inv3(lat, il);
+            inv3(st[k].lat, il);
+            double *dl = dEdLat + 9 * (size_t)k;
             for (int a = 0; a < 3; ++a)
                 for (int x = 0; x < 3; ++x) {
                     double acc = 0;
                     for (int b = 0; b < 3; ++b) acc += res[1 + 3 * a + b] * il[x + 3 * b];
-                    dEdLat[3 * x + a] = acc;
+                    dl[3 * x + a] = acc;
                 }
         }
+    }
+}
+
+void Engine::evaluate_batch_host(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
+                                 const double *lat, double *energy, double *force, double *dEdLat)
+{
+    if (!dev_ready_) init_device();
+    if (nstruct < 0) throw Error(1, "nstruct < 0");
+    if (nstruct == 0) return;
+    if (!nat || !periodic || !energy) throw Error(1, "null array");
+    std::vector<StructIn> st(nstruct);
+    long total = 0;
+    for (int k = 0; k < nstruct; ++k) {
+        StructIn &in = st[k];
+        in.nat = nat[k];
+        in.periodic = periodic[k] ? 1 : 0;
+        if (in.nat < 0) throw Error(1, "nat < 0");
+        if (in.periodic) {
+            if (!lat) throw Error(1, "periodic structure without lattice");
+            std::memcpy(in.lat, lat + 9 * (size_t)k, sizeof in.lat);
+        } else if (in.nat > 0) {           // bounding box of the raw positions (neighborLists.f90:131-138)
+            const double *p = pos + 3 * (size_t)total;
+            for (int a = 0; a < 3; ++a) in.bbox[a] = in.bbox[3 + a] = p[a];
+            for (int i = 1; i < in.nat; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    in.bbox[a] = std::min(in.bbox[a], p[3 * i + a]);
+                    in.bbox[3 + a] = std::max(in.bbox[3 + a], p[3 * i + a]);
+                }
+        }
+        total += in.nat;
+    }
+    if (total > 0 && (!Z || !pos || !force)) throw Error(1, "null array");
+    if (total >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
+    const size_t n = (size_t)std::max<long>(total, 1);
+    zin_.ensure(4 * n);
+    posin_.ensure(24 * n);
+    fout_.ensure(24 * n);
+    if (total > 0) {
+        HIPCHK(hipMemcpyAsync(zin_.p, Z, 4 * (size_t)total, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)total, hipMemcpyHostToDevice, stream_));
+    }
+    run(st, zin_.as<int>(), posin_.as<double>(), fout_.as<double>(), stream_, energy, dEdLat, 1);
+    if (total > 0) {
+        HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)total, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
     }
 }
 
 void Engine::evaluate_host(int nat, const int *Z, const double *pos, int periodic, const double *lat,
                            double *energy, double *force, double *dEdLat)
 {
-    if (!dev_ready_) init_device();
     if (nat > 0 && (!Z || !pos || !force)) throw Error(1, "null array");
-    zin_.ensure(4 * (size_t)std::max(nat, 1));
-    posin_.ensure(24 * (size_t)std::max(nat, 1));
-    fout_.ensure(24 * (size_t)std::max(nat, 1));
-    if (nat > 0) {
-        HIPCHK(hipMemcpyAsync(zin_.p, Z, 4 * (size_t)nat, hipMemcpyHostToDevice, stream_));
-        HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)nat, hipMemcpyHostToDevice, stream_));
-    }
-    run(nat, zin_.as<int>(), posin_.as<double>(), periodic, lat, fout_.as<double>(), stream_, energy, dEdLat, 1);
-    if (nat > 0) {
-        HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)nat, hipMemcpyDeviceToHost, stream_));
-        HIPCHK(hipStreamSynchronize(stream_));
-    }
+    evaluate_batch_host(1, &nat, Z, pos, &periodic, lat, energy, force, dEdLat);
 }
 
 void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
@@ -586,7 +720,26 @@ void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, i
                              hipStream_t stream)
 {
     if (!dev_ready_) init_device();
-    run(nat, Z_dev, pos_dev, periodic, lat, force_dev, stream ? stream : stream_, energy, dEdLat, 1);
+    if (nat < 0) throw Error(1, "nat < 0");
+    if (periodic && !lat) throw Error(1, "periodic structure without lattice");
+    hipStream_t s = stream ? stream : stream_;
+    std::vector<StructIn> st(1);
+    st[0].nat = nat;
+    st[0].periodic = periodic ? 1 : 0;
+    if (periodic) {
+        std::memcpy(st[0].lat, lat, sizeof st[0].lat);
+    } else if (nat > 0) {                  // neighborLists.f90:131-138, on the device
+        Work w{};
+        w.nat = nat;
+        w.pos = pos_dev;
+        w.bbox = (double *)((char *)misc_.p + kDblOffset) + kDblBbox;
+        launch_bbox(w, s);
+        double *hb = (double *)staging(64);
+        HIPCHK(hipMemcpyAsync(hb, w.bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::memcpy(st[0].bbox, hb, sizeof st[0].bbox);
+    }
+    run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
 }
 
 int64_t Engine::neighbor_count()

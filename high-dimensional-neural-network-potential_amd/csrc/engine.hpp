@@ -22,6 +22,14 @@ struct DevMem {
     template <typename T> T *as() const { return (T *)p; }
 };
 
+// One structure of an evaluation as given by the caller.
+struct StructIn {
+    int nat = 0;
+    int periodic = 0;
+    double lat[9] = {};    // periodic: lattice, lat[3k+a] = component a of vector k
+    double bbox[6] = {};   // nonperiodic: min and max of the raw positions
+};
+
 class Engine {
   public:
     Engine(HostModel m, int precision);
@@ -35,6 +43,11 @@ class Engine {
     // host arrays in and out (hdnnp_evaluate)
     void evaluate_host(int nat, const int *Z, const double *pos, int periodic, const double *lat,
                        double *energy, double *force, double *dEdLat);
+    // nstruct structures concatenated: nat[k] atoms each, Z/pos/force host
+    // arrays over all atoms, periodic[k], lat[9k..] (periodic ones),
+    // energy[k], dEdLat[9k..] (may be null)
+    void evaluate_batch_host(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
+                             const double *lat, double *energy, double *force, double *dEdLat);
     // device arrays Z, pos, force; lat/energy/dEdLat on the host
     void evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
                          const double *lat, double *energy, double *force_dev, double *dEdLat,
@@ -47,9 +60,11 @@ class Engine {
 
   private:
     void init_device();
+    // The structures st (atoms concatenated in Z_dev / pos_dev / force_dev).
     // retry: how often an evaluation may be repeated with a larger neighbour capacity
-    void run(int nat, const int *Z_dev, const double *pos_dev, int periodic, const double *lat,
-             double *force_dev, hipStream_t s, double *energy, double *dEdLat, int retry);
+    void run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,
+             hipStream_t s, double *energy, double *dEdLat, int retry);
+    void *staging(size_t bytes);
     template <typename R>
     void compute(Work &w, const Geo &g, hipStream_t s);
 
@@ -69,8 +84,12 @@ class Engine {
     DevMem zin_, posin_, fout_;
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
     DevMem count_, perm_, eidx_sorted_, Ei_, vir_, misc_;
-    DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_, red_;
+    DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_;
+    DevMem geo_, sid_, csid_, chunks_, sc_, res_, partial_;
+    std::vector<StructGeo> hgeo_;
+    std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
+    size_t pinned_bytes_ = 0;
     int last_nat_ = 0, last_cap_ = 0;
     int64_t last_total_ = 0;   // -1: not yet summed
     int cap_hint_ = 0;         // most neighbours of one atom seen so far

@@ -73,12 +73,13 @@ void launch_force_gather_t(Work &w, const R *gent, hipStream_t s)
 template void launch_force_gather_t<double>(Work &, const double *, hipStream_t);
 template void launch_force_gather_t<float>(Work &, const float *, hipStream_t);
 
-// Two-stage FP64 reduction, result[0] = sum Ei, result[1..9] = sum vir.
-// Fixed order for a given nat: block b sums a fixed contiguous chunk (thread
-// t takes every 256th atom of it, then a tree over threads); one block then
-// sums the block partials with the same tree.
-constexpr int kRedBlocks = 256;
-
+// Two-stage FP64 reduction per structure: result[10 s] = sum of Ei,
+// result[10 s + 1 + k] = sum of vir[k] over the atoms of structure s.
+// Stage 1: one block per chunk of atoms (chunks never cross a structure;
+// thread t takes every 256th atom of the chunk, then a tree over threads).
+// Stage 2: one wave per structure adds its chunks' partials in a fixed order.
+// The order depends only on the structure's atom count, so results are
+// reproducible and a structure gets the same sums alone or in a batch.
 __device__ inline void tree10(double (*red)[256], double *a)
 {
 #pragma unroll
@@ -92,38 +93,50 @@ __device__ inline void tree10(double (*red)[256], double *a)
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_partial(Work w, double *partial, int chunk)
+__global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chunks, double *partial)
 {
     __shared__ double red[10][256];
     double a[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) a[k] = 0;
-    const int i0 = blockIdx.x * chunk, i1 = min(w.nat, i0 + chunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    const int2 c = chunks[blockIdx.x];
+    for (int i = c.x + threadIdx.x; i < c.y; i += 256) {
         a[0] += w.Ei[i];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[1 + k] += w.vir[9 * (size_t)i + k];
     }
     tree10(red, a);
-    if (threadIdx.x < 10) partial[10 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 10) partial[10 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(256) void k_reduce_final(Work w, const double *partial, int nb)
+// one wave per structure: lane l adds chunks l, l+64, ... of the structure,
+// then a fixed butterfly over the lanes
+__global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, const double *partial)
 {
-    __shared__ double red[10][256];
+    const int s = blockIdx.x, lane = threadIdx.x;
     double a[10];
 #pragma unroll
-    for (int k = 0; k < 10; ++k) a[k] = (int)threadIdx.x < nb ? partial[10 * threadIdx.x + k] : 0.0;
-    tree10(red, a);
-    if (threadIdx.x < 10) w.result[threadIdx.x] = red[threadIdx.x][0];
+    for (int k = 0; k < 10; ++k) a[k] = 0;
+    for (int c = sc[s] + lane; c < sc[s + 1]; c += 64)
+#pragma unroll
+        for (int k = 0; k < 10; ++k) a[k] += partial[10 * (size_t)c + k];
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+        for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o);
+    if (lane < 10) {
+        double v = a[0];
+#pragma unroll
+        for (int k = 1; k < 10; ++k) v = lane == k ? a[k] : v;
+        w.result[10 * (size_t)s + lane] = v;
+    }
 }
 
-void launch_reduce(Work &w, double *partial, hipStream_t s)
+void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
+                   hipStream_t s)
 {
-    const int nb = std::min(kRedBlocks, (w.nat + 255) / 256);
-    const int chunk = (w.nat + nb - 1) / nb;
-    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, s, w, partial, chunk);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, w, (const double *)partial, nb);
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_reduce_partial, dim3(nchunk), dim3(256), 0, s, w, chunks, partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(nstruct), dim3(64), 0, s, w, sc, (const double *)partial);
 }
 
 }  // namespace hdnnp

@@ -107,32 +107,55 @@ void launch_bbox(Work &w, hipStream_t s)
     hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, w);
 }
 
+// Structure of each atom (cells = false) or each cell (cells = true) of a
+// batch: the last structure whose first atom / cell is <= x.
+__global__ __launch_bounds__(256) void k_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    int lo = 0, hi = nstruct - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((cells ? sg[mid].cell_off : sg[mid].atom_off) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    out[x] = lo;
+}
+
+void launch_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_segment_ids, dim3((n + 255) / 256), dim3(256), 0, s, out, n, sg, nstruct, cells);
+}
+
 // Wrap into the list frame and bin: periodic input is wrapped twice (as
 // hdnnps.f90:381 and neighborLists.f90:156 do); nonperiodic input is shifted
 // by the artificial origin and wrapped once in the artificial box. Cell index
-// int(s*ngrid) (neighborLists.f90:161-166), clamped into the grid.
+// int(s*ngrid) (neighborLists.f90:161-166), clamped into the grid, offset by
+// the structure's first cell.
 __global__ __launch_bounds__(256) void k_wrap_bin(Geo g, Work w)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w.nat) return;
+    const StructGeo &G = g.sg[w.sid[i]];
     double x = w.pos[3 * i + 0], y = w.pos[3 * i + 1], z = w.pos[3 * i + 2];
-    if (g.periodic) {
-        wrap1(g.lat, g.inv, x, y, z);
-        wrap1(g.lat, g.inv, x, y, z);
+    if (G.periodic) {
+        wrap1(G.lat, G.inv, x, y, z);
+        wrap1(G.lat, G.inv, x, y, z);
     } else {
-        x = x - g.origin[0];
-        y = y - g.origin[1];
-        z = z - g.origin[2];
-        wrap1(g.lat, g.inv, x, y, z);
+        x = x - G.origin[0];
+        y = y - G.origin[1];
+        z = z - G.origin[2];
+        wrap1(G.lat, G.inv, x, y, z);
     }
     w.wpos[i] = make_double4(x, y, z, 0.0);
     double s0, s1, s2;
-    mv3(g.inv, x, y, z, s0, s1, s2);
-    int c0 = (int)(s0 * g.ng[0]), c1 = (int)(s1 * g.ng[1]), c2 = (int)(s2 * g.ng[2]);
-    c0 = min(max(c0, 0), g.ng[0] - 1);
-    c1 = min(max(c1, 0), g.ng[1] - 1);
-    c2 = min(max(c2, 0), g.ng[2] - 1);
-    w.cellkey[i] = (c0 * g.ng[1] + c1) * g.ng[2] + c2;
+    mv3(G.inv, x, y, z, s0, s1, s2);
+    int c0 = (int)(s0 * G.ng[0]), c1 = (int)(s1 * G.ng[1]), c2 = (int)(s2 * G.ng[2]);
+    c0 = min(max(c0, 0), G.ng[0] - 1);
+    c1 = min(max(c1, 0), G.ng[1] - 1);
+    c2 = min(max(c2, 0), G.ng[2] - 1);
+    w.cellkey[i] = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
 }
 
 void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s)
@@ -156,9 +179,10 @@ void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s)
     hipLaunchKernelGGL(k_cell_range, dim3((w.nat + 255) / 256), dim3(256), 0, s, w);
 }
 
-// One stencil cell of atom i: wrapped cell id and image shift, in the
-// reference's order of wrapping (z, then y, then x; neighborLists.f90:186-209).
-// Returns false when the cell lies outside a nonperiodic grid.
+// One stencil cell of atom i: wrapped cell id (within the structure's grid)
+// and image shift, in the reference's order of wrapping (z, then y, then x;
+// neighborLists.f90:186-209). Returns false when the cell lies outside a
+// nonperiodic grid.
 struct StencilCell {
     int cell;
     int sx, sy, sz;
@@ -166,7 +190,7 @@ struct StencilCell {
     bool centre;
 };
 
-__device__ inline bool stencil_cell(const Geo &g, int key, int s, StencilCell &c)
+__device__ inline bool stencil_cell(const StructGeo &g, int key, int s, StencilCell &c)
 {
     const int sz = 2 * g.nc[2] + 1, sy = 2 * g.nc[1] + 1;
     const int jz = s % sz - g.nc[2];
@@ -241,7 +265,9 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
     const int c = blockIdx.x;
     const int a0 = w.cell_start[c], A = w.cell_end[c] - a0;
     if (A <= 0) return;
-    const int S = w.nstencil;
+    const StructGeo &G = g.sg[w.csid[c]];
+    const int S = G.S;
+    const int key = c - G.cell_off;
     for (int g0 = 0; g0 < A; g0 += 64) {
         const int Ag = min(64, A - g0);
         __syncthreads();                      // previous group done with ia / pa
@@ -255,9 +281,9 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
             StencilCell sc{};
             int m = 0, start = 0;
             const bool in_s = s0 + lane < S;
-            if (in_s && stencil_cell(g, c, s0 + lane, sc)) {
-                start = w.cell_start[sc.cell];
-                m = w.cell_end[sc.cell] - start;
+            if (in_s && stencil_cell(G, key, s0 + lane, sc)) {
+                start = w.cell_start[G.cell_off + sc.cell];
+                m = w.cell_end[G.cell_off + sc.cell] - start;
             }
             const int incl = wave_incl_scan_int(m, lane);
             const int tot = __shfl(incl, 63);
@@ -313,7 +339,8 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
                     if (w.pos_start && first_here) {
                         const int k = my_pref - t0;
                         const int ps = na + __popcll(mask & ((1ull << k) - 1));
-                        w.pos_start[(size_t)ai * S + s0 + lane] = (unsigned short)min(ps, 65535);
+                        w.pos_start[G.ps_off + (long long)(ai - G.atom_off) * S + s0 + lane] =
+                            (unsigned short)min(ps, 65535);
                     }
                     if (lane == a) n += __popcll(mask);
                 }
@@ -322,7 +349,9 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
             if (w.pos_start)
                 for (int a = 0; a < Ag; ++a) {
                     const int na = __shfl(n, a);
-                    if (in_s && my_pref >= tot) w.pos_start[(size_t)ia[a] * S + s0 + lane] = (unsigned short)min(na, 65535);
+                    if (in_s && my_pref >= tot)
+                        w.pos_start[G.ps_off + (long long)(ia[a] - G.atom_off) * S + s0 + lane] =
+                            (unsigned short)min(na, 65535);
                 }
         }
         if (lane < Ag) {
@@ -344,19 +373,20 @@ void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s)
 // by bisection. The strict distance test is evaluated separately in each
 // direction, so a pair at the exact cutoff can lack its partner: rev = -1.
 // One thread per entry slot; slot q belongs to atom q >> cap_log2.
-__global__ __launch_bounds__(256) void k_nl_rev(Work w)
+__global__ __launch_bounds__(256) void k_nl_rev(Geo g, Work w)
 {
     const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= (long)w.nat * w.cap) return;
     const int i = (int)(q >> w.cap_log2);
     if ((int)(q - ((long)i << w.cap_log2)) >= w.count[i]) return;
-    const int S = w.nstencil;
+    const StructGeo &G = g.sg[w.sid[i]];
+    const int S = G.S;
     const int j = w.ent_j[q];
     const int sp = S - 1 - w.ent_s[q];
     const int bj = j * w.cap, nj = w.count[j];
     int r = -1;
     if (w.pos_start) {
-        for (int t = w.pos_start[(size_t)j * S + sp]; t < nj; ++t) {
+        for (int t = w.pos_start[G.ps_off + (long long)(j - G.atom_off) * S + sp]; t < nj; ++t) {
             if (w.ent_s[bj + t] != sp) break;
             if (w.ent_j[bj + t] == i) {
                 r = bj + t;
@@ -376,10 +406,10 @@ __global__ __launch_bounds__(256) void k_nl_rev(Work w)
     w.rev[q] = r;
 }
 
-void launch_nl_rev(Work &w, hipStream_t s)
+void launch_nl_rev(const Geo &g, Work &w, hipStream_t s)
 {
     const long slots = (long)w.nat * w.cap;
-    hipLaunchKernelGGL(k_nl_rev, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_nl_rev, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, g, w);
 }
 
 // Element segments of the element-sorted atom order: elem_off[e] = first

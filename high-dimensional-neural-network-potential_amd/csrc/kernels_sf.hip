@@ -331,16 +331,35 @@ __device__ __forceinline__ void elem_tile(const Work &w, int nel, int blk, int g
     }
 }
 
+// Lattice (list frame) of the structure of atom i.
+struct Lat9 {
+    double v[9];
+};
+__device__ __forceinline__ Lat9 atom_lattice(const Geo &g, const Work &w, int i)
+{
+    Lat9 L;
+    if (i < 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L.v[k] = 0.0;
+        return L;
+    }
+    const CAS StructGeo *G = (const CAS StructGeo *)g.sg + w.sid[i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L.v[k] = G->lat[k];
+    return L;
+}
+
 // Geometry of one list entry: d = R_i - (R_j + L.h) (FP64), r = |d|.
-__device__ __forceinline__ void entry_geometry(const Geo &g, const Work &w, const double4 &pi, int e,
+__device__ __forceinline__ void entry_geometry(const Lat9 &lt, const Work &w, const double4 &pi, int e,
                                                double &d0, double &d1, double &d2, double &r, int &j)
 {
+    const double *lat = lt.v;
     j = w.ent_j[e];
     int sx, sy, sz;
     unpack_shift(w.ent_shift[e], sx, sy, sz);
-    const double l0 = ((g.lat[6] * sz) + g.lat[3] * sy) + g.lat[0] * sx;
-    const double l1 = ((g.lat[7] * sz) + g.lat[4] * sy) + g.lat[1] * sx;
-    const double l2 = ((g.lat[8] * sz) + g.lat[5] * sy) + g.lat[2] * sx;
+    const double l0 = ((lat[6] * sz) + lat[3] * sy) + lat[0] * sx;
+    const double l1 = ((lat[7] * sz) + lat[4] * sy) + lat[1] * sx;
+    const double l2 = ((lat[8] * sz) + lat[5] * sy) + lat[2] * sx;
     const double4 pj = w.wpos[j];
     d0 = pi.x - (pj.x + l0);
     d1 = pi.y - (pj.y + l1);
@@ -367,6 +386,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
     const int n = i >= 0 ? w.count[i] : 0;
     const int off = i >= 0 ? i * w.cap : 0;
     const double4 pi = i >= 0 ? w.wpos[i] : make_double4(0, 0, 0, 0);
+    const Lat9 lt = atom_lattice(g, w, i);
     const R rrc = R(1) / (R)rce;
     double k0[NK], k1[NK], k2[NK], kr[NK];
     int kf[NK];
@@ -382,7 +402,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
         double d0 = 0, d1 = 0, d2 = 0, r = 1e300;
         if (q < n) {
             int j;
-            entry_geometry(g, w, pi, off + q, d0, d1, d2, r, j);
+            entry_geometry(lt, w, pi, off + q, d0, d1, d2, r, j);
             f = r < rce ? w.eidx[j] : -1;
         }
         if (kKeep) {
@@ -426,7 +446,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
             r = 1e300;
             if (q < n) {
                 int j;
-                entry_geometry(g, w, pi, off + q, d0, d1, d2, r, j);
+                entry_geometry(lt, w, pi, off + q, d0, d1, d2, r, j);
                 f = r < rce ? w.eidx[j] : -1;
             }
         }

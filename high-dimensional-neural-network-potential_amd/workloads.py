@@ -119,6 +119,23 @@ def cluster(nat: int, seed: int, box: float = 20.0, dmin: float = 1.8,
     return Structure(pos, Z, np.zeros((3, 3)), False)
 
 
+def batch_mix(nstruct: int, seed: int = 2024, nmax_cell: int = 4, cl_min: int = 8, cl_max: int = 64):
+    """Batch of independent structures: half perovskite supercells
+    n1 x n2 x n3 with each n uniform in 1..nmax_cell (12 to 12*nmax_cell^3
+    atoms, jitter 0.2 bohr), half nonperiodic C/N/H clusters of cl_min..cl_max
+    atoms in a 20-bohr box (minimum distance 1.8 bohr), interleaved."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nstruct):
+        sub = int(rng.integers(1 << 30))
+        if k % 2 == 0:
+            n = rng.integers(1, nmax_cell + 1, size=3)
+            out.append(perovskite(int(n[0]), int(n[1]), int(n[2]), seed=sub))
+        else:
+            out.append(cluster(int(rng.integers(cl_min, cl_max + 1)), seed=sub))
+    return out
+
+
 def methylammonium() -> Structure:
     """The reference example molecule (example/input.xyz), nonperiodic."""
     return Structure(_MA_POS.copy(), _MA_Z.copy(), np.zeros((3, 3)), False)

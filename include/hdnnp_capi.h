@@ -58,6 +58,18 @@ int hdnnp_evaluate(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *
                    int32_t periodic, const double *lat, double *energy, double *force,
                    double *dEdLat);
 
+/* Many independent structures in one evaluation (the reference evaluates one
+ * structure per hdnnpEnergyAndForces call; this is that call for nstruct
+ * structures at once). Structure k has nat[k] atoms; Z, pos and force hold the
+ * atoms of all structures one after the other (pos/force [sum nat][3]);
+ * periodic[k]; lat[9k..9k+8] is read for periodic structures only (lat may be
+ * NULL when none is periodic); energy[nstruct]; dEdLat[9k..] per structure
+ * (may be NULL). Each result equals the result of hdnnp_evaluate on that
+ * structure alone, within rounding of the summation order of E. */
+int hdnnp_evaluate_batch(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z,
+                         const double *pos, const int32_t *periodic, const double *lat, double *energy,
+                         double *force, double *dEdLat);
+
 /* Same evaluation with Z, pos and force resident in device memory (HBM);
  * lat, energy and dEdLat are host pointers. `stream` is a hipStream_t or
  * NULL for the context's own stream; the call returns after the results are

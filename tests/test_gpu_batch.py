@@ -1,0 +1,62 @@
+"""Batched evaluation (hdnnp_evaluate_batch): many independent structures of
+uneven size in one evaluation. Each structure's neighbour list, symmetry
+functions, MLP rows, gradient rows and reduction chunks are the same as in an
+evaluation of that structure alone, so the batch must reproduce the
+single-structure results bit for bit (FP64 and mixed)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import PEROV_PARAMS, golden, golden_structure
+
+pytestmark = pytest.mark.gpu
+
+
+def _structures(wl):
+    empty = wl.Structure(np.zeros((0, 3)), np.zeros(0, np.int32))
+    return [wl.perovskite(1, 1, 1, seed=11), wl.cluster(8, seed=1), wl.perovskite(2, 1, 1, seed=12),
+            wl.cluster(40, seed=2), empty, wl.perovskite(1, 2, 3, seed=13), wl.cluster(64, seed=3),
+            golden_structure(wl, golden("eval_perov2"))]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_single(pkg, wl, precision):
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    ss = _structures(wl)
+    print(f"batch of {len(ss)} ({sum(s.nat for s in ss)} atoms), {precision}", flush=True)
+    E, F, L = h.energy_and_forces_batch(ss)
+    for k, s in enumerate(ss):
+        if s.nat == 0:
+            assert E[k] == 0.0 and F[k].shape == (0, 3) and not L[k].any()
+            continue
+        e, f, l = h.energy_and_forces(s)
+        assert E[k] == e, (k, E[k], e)
+        assert np.array_equal(F[k], f), k
+        if s.periodic:
+            assert np.array_equal(L[k], l), k
+        else:
+            assert not L[k].any()
+
+
+def test_batch_against_reference_golden(pkg, wl):
+    """A structure with golden reference values keeps them inside a batch."""
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    d = golden("eval_perov2")
+    s = golden_structure(wl, d)
+    E, F, L = h.energy_and_forces_batch([wl.cluster(20, seed=5), s, wl.perovskite(1, 1, 2, seed=6)])
+    assert abs(E[1] - float(d["energy"])) <= 1e-10 * max(1.0, abs(float(d["energy"])))
+    np.testing.assert_allclose(F[1], d["force"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(L[1], d["dEdLat"], rtol=0, atol=1e-9 * max(1.0, np.abs(d["dEdLat"]).max()))
+
+
+def test_large_mixed_batch(pkg, wl):
+    """300 structures of the batch recipe (perovskite supercells up to 4x4x4 and
+    8-64 atom clusters); 30 of them re-evaluated alone."""
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed")
+    ss = wl.batch_mix(300, seed=77)
+    print(f"batch of {len(ss)} ({sum(s.nat for s in ss)} atoms)", flush=True)
+    E, F, L = h.energy_and_forces_batch(ss)
+    for k in np.random.default_rng(1).choice(len(ss), 30, replace=False):
+        e, f, l = h.energy_and_forces(ss[k])
+        assert E[k] == e and np.array_equal(F[k], f), k
