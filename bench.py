@@ -179,6 +179,57 @@ def run_mode(pkg, torch, s, precision, steps, warmup):
                    stage=np.array(stage))
 
 
+def bench_batch(args):
+    """Batched evaluation of many independent structures (half perovskite
+    supercells n1 x n2 x n3, n in 1..4, half 8-64 atom C/N/H clusters): one
+    hdnnp_evaluate_batch_device call per step, atoms resident in device memory
+    (per-structure sizes, flags and lattices on the host)."""
+    import torch
+    pkg = load_package()
+    ss = pkg.workloads.batch_mix(args.nstruct, seed=2024)
+    nat_s = np.array([s.nat for s in ss], dtype=np.int32)
+    nat = int(nat_s.sum())
+    per = np.array([int(s.periodic) for s in ss], dtype=np.int32)
+    lats = np.stack([np.asarray(s.lat, dtype=np.float64).reshape(3, 3) for s in ss])
+    dev = torch.device("cuda:0")
+    Z = torch.from_numpy(np.concatenate([s.Z for s in ss]).astype(np.int32)).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([s.pos for s in ss]))).to(dev)
+    force = torch.empty((nat, 3), dtype=torch.float64, device=dev)
+    print(f"workload: batch of {len(ss)} structures, {nat} atoms", file=sys.stderr, flush=True)
+    res = {}
+    for mode in args.modes.split(","):
+        h = pkg.parseRuNNerInput(PARAMS, mode)
+        for _ in range(args.warmup):
+            h.evaluate_batch_torch(nat_s, Z, pos, per, lats, force)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            E, _, L = h.evaluate_batch_torch(nat_s, Z, pos, per, lats, force)
+            lat.append(time.perf_counter() - a)
+        torch.cuda.synchronize()
+        t = float(np.median(lat))
+        res[mode] = {"atoms_per_s": nat / t, "structures_per_s": len(ss) / t, "ms_per_batch": t * 1e3,
+                     "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+                     "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy()}
+        print(f"{mode}: {t * 1e3:.3f} ms per batch -> {nat / t:.4g} atoms/s, {len(ss) / t:.4g} structures/s; "
+              f"stage ms {res[mode]['stage_ms']}", file=sys.stderr, flush=True)
+    rmse = None
+    if "fp64" in res and "mixed" in res:
+        rmse = float(np.sqrt(np.mean((res["mixed"]["F"] - res["fp64"]["F"]) ** 2)))
+    head = "mixed" if ("mixed" in res and rmse is not None and rmse <= FORCE_RMSE_BOUND) else next(iter(res))
+    out = {"metric": "atoms/s (E+F+stress), batched independent structures", "value": res[head]["atoms_per_s"],
+           "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res[head]["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32 (symmetry functions, MLP); f64 (wrap, list, sums)",
+           "data": "synthetic (seeded structures, seeded random-weight potential)",
+           "config": {"workload": f"batch of {len(ss)} structures ({nat} atoms): perovskite n1xn2xn3 (n in 1..4) "
+                                  f"and 8-64 atom clusters, atoms resident in device memory", "precision": head},
+           "force_rmse_mixed_vs_fp64": rmse,
+           "modes": {m: {k: v for k, v in r.items() if k != "F"} for m, r in res.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,9 +239,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--workload", default="cell", choices=["cell", "batch"],
+                    help="cell: one perovskite supercell (headline); batch: many independent structures")
+    ap.add_argument("--nstruct", type=int, default=4096, help="structures of the batch workload")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "batch":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_batch(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "

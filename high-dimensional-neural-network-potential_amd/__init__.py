@@ -14,6 +14,7 @@ raises.
 from __future__ import annotations
 
 import ctypes
+import sys
 import os
 
 import numpy as np
@@ -28,7 +29,7 @@ _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
-EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
+EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
@@ -58,12 +59,18 @@ def load_library():
         if not os.path.exists(LIB_PATH):
             raise HdnnpError(5, f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C "
                                 f"{HERE})")
+        # torch bundles its own HIP runtime; two runtimes share the device in
+        # one process only when torch's initialises first, so bring it up
+        # before the library if torch is in use here
+        if "torch" in sys.modules:
+            sys.modules["torch"].cuda.is_available()
         lib = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p
         lib.hdnnp_create.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(vp)]
         lib.hdnnp_evaluate.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.hdnnp_evaluate_device.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_evaluate_batch.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp]
+        lib.hdnnp_evaluate_batch_device.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
         lib.hdnnp_get_symfunctions.argtypes = [vp, ctypes.c_int32, _dp, _dp, _dp]
@@ -171,6 +178,25 @@ class Hdnnp:
                                                   _p(L, _dp)))
         off = np.concatenate([[0], np.cumsum(nat)])
         return E, [F[off[k]:off[k + 1]] for k in range(ns)], L
+
+    def evaluate_batch_torch(self, nat, Z, pos, periodic, lat, force=None, stream=None):
+        """hdnnp_evaluate_batch_device: nat/periodic (nstruct,), lat (nstruct, 3, 3)
+        host arrays; Z (int32), pos (float64 [sum nat, 3]) torch tensors on the GPU.
+        Returns (E[nstruct], force tensor, dEdLat[nstruct, 3, 3])."""
+        import torch
+        nat = np.ascontiguousarray(nat, dtype=np.int32)
+        per = np.ascontiguousarray(periodic, dtype=np.int32)
+        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        ns = nat.shape[0]
+        if force is None:
+            force = torch.empty((int(nat.sum()), 3), dtype=torch.float64, device=pos.device)
+        E = np.zeros(ns)
+        L = np.zeros((ns, 3, 3))
+        self._check(self.lib.hdnnp_evaluate_batch_device(
+            self.ctx, ns, _p(nat, _ip), ctypes.c_void_p(Z.data_ptr()), ctypes.c_void_p(pos.data_ptr()),
+            _p(per, _ip), _p(lat, _dp), _p(E, _dp), ctypes.c_void_p(force.data_ptr()), _p(L, _dp),
+            ctypes.c_void_p(stream or 0)))
+        return E, force, L
 
     def evaluate_device(self, Z_dev, pos_dev, force_dev, periodic: bool, lat, stream=None):
         """Device-resident evaluation: Z_dev/pos_dev/force_dev are device

@@ -74,6 +74,17 @@ int hdnnp_evaluate_batch(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, co
     });
 }
 
+int hdnnp_evaluate_batch_device(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z_dev,
+                                const double *pos_dev, const int32_t *periodic, const double *lat, double *energy,
+                                double *force_dev, double *dEdLat, void *stream)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng->evaluate_batch_device(nstruct, nat, Z_dev, pos_dev, periodic, lat, energy, force_dev, dEdLat,
+                                        (hipStream_t)stream);
+    });
+}
+
 int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev, const double *pos_dev,
                           int32_t periodic, const double *lat, double *energy, double *force_dev,
                           double *dEdLat, void *stream)

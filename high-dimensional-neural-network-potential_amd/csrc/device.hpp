@@ -121,7 +121,6 @@ struct Work {
     double *vir;         // [nat][9]
     double *force;       // [nat][3]
     double *result;      // [nstruct][10]: energy, W[9] of each structure
-    double *bbox;        // [6]
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
@@ -149,7 +148,7 @@ __host__ __device__ inline void unpack_shift(int p, int &x, int &y, int &z)
 
 // ---- launchers (kernels_nl.hip) ----
 void launch_map_elements(const DevModel &m, Work &w, hipStream_t s);
-void launch_bbox(Work &w, hipStream_t s);
+void launch_bbox(const double *pos, const int *off, const int *which, int nwhich, double *out, hipStream_t s);
 void launch_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells, hipStream_t s);
 void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s);
 void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s);

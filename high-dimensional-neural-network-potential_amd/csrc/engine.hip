@@ -123,7 +123,7 @@ Engine::Engine(HostModel m, int precision) : model_(std::move(m)), precision_(pr
 Engine::~Engine()
 {
     for (auto &d : model_mem_) d.release();
-    for (DevMem *d : {&geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
+    for (DevMem *d : {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
                       &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
                       &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
                       &vir_, &misc_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_})
@@ -314,7 +314,6 @@ static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // i
 static constexpr size_t kDblOffset = 256;                                    // bytes
 // largest per-(atom, stencil cell) slot table; beyond it partners are found by bisection
 static constexpr size_t kPosStartBytes = size_t(1) << 30;
-static constexpr int kDblBbox = 16;                                          // doubles
 // atoms per block of the first reduction stage
 static constexpr int kReduceChunk = 512;
 
@@ -511,7 +510,6 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     }
 
     int *mi = misc_.as<int>();
-    double *md = (double *)((char *)misc_.p + kDblOffset);
     Work w{};
     w.nat = nat;
     w.pos = pos_dev;
@@ -532,7 +530,6 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     w.vir = vir_.as<double>();
     w.force = force_dev;
     w.result = res_.as<double>();
-    w.bbox = md + kDblBbox;
 
     HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
     launch_map_elements(dm_, w, s);
@@ -715,31 +712,58 @@ void Engine::evaluate_host(int nat, const int *Z, const double *pos, int periodi
     evaluate_batch_host(1, &nat, Z, pos, &periodic, lat, energy, force, dEdLat);
 }
 
+void Engine::evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev, const double *pos_dev,
+                                   const int *periodic, const double *lat, double *energy, double *force_dev,
+                                   double *dEdLat, hipStream_t stream)
+{
+    if (!dev_ready_) init_device();
+    if (nstruct < 0) throw Error(1, "nstruct < 0");
+    if (nstruct == 0) return;
+    if (!nat || !periodic || !energy) throw Error(1, "null array");
+    hipStream_t s = stream ? stream : stream_;
+    std::vector<StructIn> st(nstruct);
+    std::vector<int> off(nstruct + 1, 0), which;
+    for (int k = 0; k < nstruct; ++k) {
+        StructIn &in = st[k];
+        in.nat = nat[k];
+        in.periodic = periodic[k] ? 1 : 0;
+        if (in.nat < 0) throw Error(1, "nat < 0");
+        if ((long)off[k] + in.nat >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
+        off[k + 1] = off[k] + in.nat;
+        if (in.periodic) {
+            if (!lat) throw Error(1, "periodic structure without lattice");
+            std::memcpy(in.lat, lat + 9 * (size_t)k, sizeof in.lat);
+        } else if (in.nat > 0) {
+            which.push_back(k);
+        }
+    }
+    if (off[nstruct] > 0 && (!Z_dev || !pos_dev || !force_dev)) throw Error(1, "null array");
+    if (!which.empty()) {   // bounding boxes of the nonperiodic structures, on the device
+        const int nw = (int)which.size();
+        const size_t ob = 4 * (size_t)(nstruct + 1), wb = 4 * (size_t)nw, bb = 48 * (size_t)nw;
+        char *hs = (char *)staging(ob + wb + bb + 512);
+        std::memcpy(hs, off.data(), ob);
+        std::memcpy(hs + (ob + 255) / 256 * 256, which.data(), wb);
+        double *hb = (double *)(hs + (ob + 255) / 256 * 256 + (wb + 255) / 256 * 256);
+        bbox_.ensure(ob + wb + bb + 512);
+        char *db = (char *)bbox_.p;
+        HIPCHK(hipMemcpyAsync(db, hs, (char *)hb - hs, hipMemcpyHostToDevice, s));
+        launch_bbox(pos_dev, (const int *)db, (const int *)(db + ((ob + 255) / 256 * 256)), nw,
+                    (double *)(db + ((char *)hb - hs)), s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hb, db + ((char *)hb - hs), bb, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int t = 0; t < nw; ++t) std::memcpy(st[which[t]].bbox, hb + 6 * (size_t)t, 6 * sizeof(double));
+    }
+    run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
+}
+
 void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
                              const double *lat, double *energy, double *force_dev, double *dEdLat,
                              hipStream_t stream)
 {
-    if (!dev_ready_) init_device();
-    if (nat < 0) throw Error(1, "nat < 0");
     if (periodic && !lat) throw Error(1, "periodic structure without lattice");
-    hipStream_t s = stream ? stream : stream_;
-    std::vector<StructIn> st(1);
-    st[0].nat = nat;
-    st[0].periodic = periodic ? 1 : 0;
-    if (periodic) {
-        std::memcpy(st[0].lat, lat, sizeof st[0].lat);
-    } else if (nat > 0) {                  // neighborLists.f90:131-138, on the device
-        Work w{};
-        w.nat = nat;
-        w.pos = pos_dev;
-        w.bbox = (double *)((char *)misc_.p + kDblOffset) + kDblBbox;
-        launch_bbox(w, s);
-        double *hb = (double *)staging(64);
-        HIPCHK(hipMemcpyAsync(hb, w.bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::memcpy(st[0].bbox, hb, sizeof st[0].bbox);
-    }
-    run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
+    evaluate_batch_device(1, &nat, Z_dev, pos_dev, &periodic, lat, energy, force_dev, dEdLat, stream);
 }
 
 int64_t Engine::neighbor_count()

@@ -48,6 +48,11 @@ class Engine {
     // energy[k], dEdLat[9k..] (may be null)
     void evaluate_batch_host(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
                              const double *lat, double *energy, double *force, double *dEdLat);
+    // the same with Z, pos and force in device memory (nat, periodic, lat,
+    // energy, dEdLat on the host)
+    void evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev, const double *pos_dev,
+                               const int *periodic, const double *lat, double *energy, double *force_dev,
+                               double *dEdLat, hipStream_t stream);
     // device arrays Z, pos, force; lat/energy/dEdLat on the host
     void evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
                          const double *lat, double *energy, double *force_dev, double *dEdLat,
@@ -85,7 +90,7 @@ class Engine {
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
     DevMem count_, perm_, eidx_sorted_, Ei_, vir_, misc_;
     DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_;
-    DevMem geo_, sid_, csid_, chunks_, sc_, res_, partial_;
+    DevMem geo_, sid_, csid_, chunks_, sc_, res_, partial_, bbox_;
     std::vector<StructGeo> hgeo_;
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;

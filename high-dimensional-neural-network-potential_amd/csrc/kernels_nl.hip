@@ -74,15 +74,18 @@ void launch_map_elements(const DevModel &m, Work &w, hipStream_t s)
     hipLaunchKernelGGL(k_map_elements, dim3((w.nat + 255) / 256), dim3(256), 0, s, m, w);
 }
 
-// Bounding box of the raw positions (nonperiodic artificial lattice,
-// neighborLists.f90:131-138): bbox[0..2] = min, bbox[3..5] = max.
-__global__ __launch_bounds__(1024) void k_bbox(Work w)
+// Bounding boxes of the raw positions of nonperiodic structures (artificial
+// lattice, neighborLists.f90:131-138): block b covers structure which[b],
+// atoms [off[s], off[s+1]); out[6b..6b+5] = min x, y, z, max x, y, z.
+__global__ __launch_bounds__(256) void k_bbox(const double *pos, const int *off, const int *which, double *out)
 {
-    __shared__ double red[6][1024];
+    __shared__ double red[6][256];
+    const int st = which[blockIdx.x];
+    const int i0 = off[st], i1 = off[st + 1];
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int i = threadIdx.x; i < w.nat; i += blockDim.x)
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
         for (int k = 0; k < 3; ++k) {
-            const double v = w.pos[3 * i + k];
+            const double v = pos[3 * (size_t)i + k];
             mn[k] = fmin(mn[k], v);
             mx[k] = fmax(mx[k], v);
         }
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(1024) void k_bbox(Work w)
         red[3 + k][threadIdx.x] = mx[k];
     }
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o)
             for (int k = 0; k < 3; ++k) {
                 red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + o]);
@@ -99,12 +102,12 @@ __global__ __launch_bounds__(1024) void k_bbox(Work w)
             }
         __syncthreads();
     }
-    if (threadIdx.x < 6) w.bbox[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 6) out[6 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-void launch_bbox(Work &w, hipStream_t s)
+void launch_bbox(const double *pos, const int *off, const int *which, int nwhich, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, w);
+    if (nwhich > 0) hipLaunchKernelGGL(k_bbox, dim3(nwhich), dim3(256), 0, s, pos, off, which, out);
 }
 
 // Structure of each atom (cells = false) or each cell (cells = true) of a

@@ -70,6 +70,13 @@ int hdnnp_evaluate_batch(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, co
                          const double *pos, const int32_t *periodic, const double *lat, double *energy,
                          double *force, double *dEdLat);
 
+/* hdnnp_evaluate_batch with Z_dev, pos_dev and force_dev resident in device
+ * memory (nat, periodic, lat, energy and dEdLat are host arrays). `stream` as
+ * for hdnnp_evaluate_device. No counterpart in the reference. */
+int hdnnp_evaluate_batch_device(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z_dev,
+                                const double *pos_dev, const int32_t *periodic, const double *lat,
+                                double *energy, double *force_dev, double *dEdLat, void *stream);
+
 /* Same evaluation with Z, pos and force resident in device memory (HBM);
  * lat, energy and dEdLat are host pointers. `stream` is a hipStream_t or
  * NULL for the context's own stream; the call returns after the results are

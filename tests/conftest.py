@@ -43,6 +43,14 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def pkg():
+    # torch ships its own HIP runtime; it has to come up before the library's
+    # (see load_library), so on a GPU machine initialise it first
+    if os.path.exists("/dev/kfd"):
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
     return load_package()
 
 

@@ -60,3 +60,21 @@ def test_large_mixed_batch(pkg, wl):
     for k in np.random.default_rng(1).choice(len(ss), 30, replace=False):
         e, f, l = h.energy_and_forces(ss[k])
         assert E[k] == e and np.array_equal(F[k], f), k
+
+
+def test_device_batch_equals_host_batch(pkg, wl):
+    """hdnnp_evaluate_batch_device (atoms in device memory, bounding boxes of the
+    nonperiodic structures computed on the device) gives the host batch's results."""
+    import torch
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    ss = [s for s in _structures(wl)]
+    E, F, L = h.energy_and_forces_batch(ss)
+    nat = np.array([s.nat for s in ss], dtype=np.int32)
+    dev = torch.device("cuda:0")
+    Z = torch.from_numpy(np.concatenate([s.Z for s in ss]).astype(np.int32)).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([s.pos.reshape(-1, 3) for s in ss]))).to(dev)
+    Ed, Fd, Ld = h.evaluate_batch_torch(nat, Z, pos, [int(s.periodic) for s in ss],
+                                        np.stack([np.asarray(s.lat).reshape(3, 3) for s in ss]))
+    assert np.array_equal(E, Ed)
+    assert np.array_equal(np.concatenate(F), Fd.cpu().numpy())
+    assert np.array_equal(L, Ld)
