@@ -29,7 +29,8 @@ _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
-EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device", "hdnnp_neighbor_count",
+EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device",
+           "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
@@ -70,6 +71,10 @@ def load_library():
         lib.hdnnp_evaluate.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.hdnnp_evaluate_device.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_evaluate_batch.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp]
+        lib.hdnnp_create_committee.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                               ctypes.POINTER(vp)]
+        lib.hdnnp_committee_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        lib.hdnnp_last_committee.argtypes = [vp, ctypes.c_int32, _dp, _dp]
         lib.hdnnp_evaluate_batch_device.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
@@ -94,17 +99,38 @@ def _p(a, t):
 class Hdnnp:
     """Handle of one potential (the reference's type(hdnnp), hdnnps.f90:13-21)."""
 
-    def __init__(self, param_dir: str, precision="fp64"):
+    def __init__(self, param_dir, precision="fp64"):
+        """param_dir: one parameter directory, or a list of them for a committee
+        (globalHdnnps.f90: the first one's symmetry functions, every member's
+        networks; evaluations return the committee mean)."""
         self.lib = load_library()
         self.ctx = ctypes.c_void_p()
         self.precision = _PREC[precision]
-        rc = self.lib.hdnnp_create(os.fsencode(param_dir), self.precision, ctypes.byref(self.ctx))
+        if isinstance(param_dir, (list, tuple)):
+            dirs = (ctypes.c_char_p * len(param_dir))(*[os.fsencode(d) for d in param_dir])
+            rc = self.lib.hdnnp_create_committee(len(param_dir), dirs, self.precision, ctypes.byref(self.ctx))
+        else:
+            rc = self.lib.hdnnp_create(os.fsencode(param_dir), self.precision, ctypes.byref(self.ctx))
         if rc:
             raise HdnnpError(rc, self.lib.hdnnp_last_error(None).decode())
         info = ModelInfo()
         self._check(self.lib.hdnnp_get_model_info(self.ctx, ctypes.byref(info)))
         self.info = info
         self.maxsf = max(info.nsf[e] for e in range(info.nelements))
+
+    @property
+    def committee_size(self) -> int:
+        n = ctypes.c_int32()
+        self._check(self.lib.hdnnp_committee_size(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    def committee(self, nstruct: int = 1):
+        """Of the last evaluation: (std of the member energies [nstruct],
+        member energies [nstruct, N])."""
+        sd = np.zeros(nstruct)
+        me = np.zeros((nstruct, self.committee_size))
+        self._check(self.lib.hdnnp_last_committee(self.ctx, nstruct, _p(sd, _dp), _p(me, _dp)))
+        return sd, me
 
     def _check(self, rc):
         if rc:
@@ -248,6 +274,23 @@ def parseRuNNerInput(path: str, precision="fp64") -> Hdnnp:
 
 def hdnnpEnergyAndForces(handle: Hdnnp, ats, want_dEdLat: bool = True):
     return handle.energy_and_forces(ats, want_dEdLat)
+
+
+def initGlobalHdnnpsFromInputFile(path: str = "nnps.txt", precision="fp64") -> Hdnnp:
+    """globalHdnnps.f90:13-29: the committee listed in `path` (first line the
+    number of potentials, then one parameter directory per line)."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        dirs = [f.readline().strip() for _ in range(n)]
+    return Hdnnp(dirs, precision)
+
+
+def globalHdnnpEnergyAndForces(handle: Hdnnp, ats):
+    """globalHdnnps.f90:31-70: committee mean E, F, dEdLat and the standard
+    deviation of the member energies."""
+    E, F, L = handle.energy_and_forces(ats)
+    sd, _ = handle.committee(1)
+    return E, F, L, float(sd[0])
 
 
 def dEdLat2Stress(lat, dEdLat) -> np.ndarray:

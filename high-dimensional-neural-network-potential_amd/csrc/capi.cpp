@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "runner.hpp"
@@ -46,15 +47,61 @@ int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out)
     }
     *out = nullptr;
     return guarded(nullptr, [&] {
-        hdnnp::HostModel m = hdnnp::read_runner(param_dir);
+        std::vector<hdnnp::HostModel> ms;
+        ms.push_back(hdnnp::read_runner(param_dir));
         auto *c = new hdnnp_ctx;
         try {
-            c->eng = new hdnnp::Engine(std::move(m), precision);
+            c->eng = new hdnnp::Engine(std::move(ms), precision);
         } catch (...) {
             delete c;
             throw;
         }
         *out = c;
+    });
+}
+
+int hdnnp_create_committee(int32_t n, const char *const *param_dirs, int32_t precision, hdnnp_ctx **out)
+{
+    if (!out || !param_dirs || n < 1) {
+        g_create_error = "hdnnp_create_committee: null argument or n < 1";
+        return HDNNP_ERR_ARG;
+    }
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        std::vector<hdnnp::HostModel> ms;
+        for (int k = 0; k < n; ++k) {
+            if (!param_dirs[k]) throw hdnnp::Error(HDNNP_ERR_ARG, "hdnnp_create_committee: null directory");
+            ms.push_back(hdnnp::read_runner(param_dirs[k]));
+        }
+        auto *c = new hdnnp_ctx;
+        try {
+            c->eng = new hdnnp::Engine(std::move(ms), precision);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int hdnnp_committee_size(const hdnnp_ctx *ctx, int32_t *n)
+{
+    if (!ctx || !n) return HDNNP_ERR_ARG;
+    *n = ctx->eng->committee_size();
+    return HDNNP_OK;
+}
+
+int hdnnp_last_committee(hdnnp_ctx *ctx, int32_t nstruct, double *std_dev, double *member_energies)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        const std::vector<double> &sd = ctx->eng->energy_std();
+        const std::vector<double> &me = ctx->eng->member_energies();
+        if (nstruct != (int32_t)sd.size())
+            throw hdnnp::Error(HDNNP_ERR_ARG, "nstruct differs from the last evaluation's (" +
+                                                  std::to_string(sd.size()) + ")");
+        if (std_dev) std::copy(sd.begin(), sd.end(), std_dev);
+        if (member_energies) std::copy(me.begin(), me.end(), member_energies);
     });
 }
 

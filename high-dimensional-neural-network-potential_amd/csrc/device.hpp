@@ -166,7 +166,8 @@ void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, const R 
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad);
 // ---- kernels_mlp.hip: 16 atoms per wavefront, maxw = widest layer/input ----
 template <typename R>
-void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, hipStream_t s);
+void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, double *Ei, R scale,
+                  bool accumulate, hipStream_t s);
 size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>

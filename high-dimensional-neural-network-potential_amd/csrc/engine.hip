@@ -110,10 +110,31 @@ __global__ void k_iota(int *p, int n)
 
 // ---------------------------------------------------------------------------
 
-Engine::Engine(HostModel m, int precision) : model_(std::move(m)), precision_(precision)
+Engine::Engine(std::vector<HostModel> members, int precision) : precision_(precision)
 {
+    if (members.empty()) throw Error(1, "no potential given");
+    model_ = std::move(members[0]);
+    members_.assign(std::make_move_iterator(members.begin() + 1), std::make_move_iterator(members.end()));
     if (precision_ != 0 && precision_ != 1) throw Error(1, "precision must be HDNNP_FP64 or HDNNP_MIXED");
     if ((int)model_.el.size() > kMaxEl) throw Error(6, "too many elements for this build");
+    // committee members (globalHdnnps.f90:31-70) reuse the first potential's
+    // symmetry functions and scaling and differ in their networks and atom
+    // energies: every element of the first needs a network with as many inputs
+    for (size_t k = 0; k < members_.size(); ++k) {
+        const HostModel &mk = members_[k];
+        for (const ElementModel &em : model_.el) {
+            const int r = mk.rank_of_Z(em.Z);
+            if (r < 0) throw Error(3, "committee member " + std::to_string(k + 2) + " has no network for Z=" +
+                                          std::to_string(em.Z));
+            const ElementModel &ek = mk.el[r];
+            if (ek.layers.empty() || ek.layers[0].nin != (int)em.sfs.size())
+                throw Error(3, "committee member " + std::to_string(k + 2) + ": network of Z=" +
+                                   std::to_string(em.Z) + " does not take the first potential's " +
+                                   std::to_string(em.sfs.size()) + " symmetry functions");
+            if (ek.layers.size() != mk.el[0].layers.size())
+                throw Error(3, "committee member " + std::to_string(k + 2) + ": networks differ in depth");
+        }
+    }
     for (size_t e = 0; e < model_.el.size(); ++e) {
         elem_valid_[e] = model_.el[e].valid;
         elem_why_[e] = model_.el[e].why;
@@ -268,6 +289,7 @@ void Engine::init_device()
             ei[e].atom_energy = d.atom_energy[e];
         }
         d.einfo = upload(model_mem_, ei);
+        ei_host_ = ei;
     }
     d.zlut = upload(model_mem_, zlut);
     d.sf_mean = upload(model_mem_, mean);
@@ -305,6 +327,37 @@ void Engine::init_device()
         d.W32[e] = upload(model_mem_, wf);
     }
     dm_ = d;
+    dmm_.assign(1, d);
+    maxw_m_.assign(1, mlp_maxw_);
+    // committee members: the first potential's tables with the member's networks
+    for (const HostModel &mk : members_) {
+        DevModel dk = d;
+        std::vector<ElemInfo> eik = ei_host_;
+        dk.nlayers = (int)mk.el[0].layers.size();
+        if (dk.nlayers > kMaxLay) throw Error(6, "network too deep for this build");
+        int maxw = d.nsfmax;
+        for (int l = 0; l < dk.nlayers; ++l) {
+            dk.nodes[l] = mk.el[0].layers[l].nout;
+            dk.act[l] = mk.el[0].layers[l].act;
+            maxw = std::max(maxw, dk.nodes[l]);
+        }
+        for (int e = 0; e < d.nel; ++e) {
+            const ElementModel &ek = mk.el[mk.rank_of_Z(m.el[e].Z)];
+            eik[e].atom_energy = ek.atom_energy;
+            dk.atom_energy[e] = ek.atom_energy;
+            std::vector<double> w;
+            for (auto &L : ek.layers) {
+                w.insert(w.end(), L.W.begin(), L.W.end());
+                w.insert(w.end(), L.b.begin(), L.b.end());
+            }
+            std::vector<float> wf(w.begin(), w.end());
+            dk.W64[e] = upload(model_mem_, w);
+            dk.W32[e] = upload(model_mem_, wf);
+        }
+        dk.einfo = upload(model_mem_, eik);
+        dmm_.push_back(dk);
+        maxw_m_.push_back(maxw);
+    }
     misc_.ensure(1024);
     dev_ready_ = true;
 }
@@ -332,9 +385,15 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s)
 
     // one block per 16-atom tile of one element; the element counts stay on
     // the device, so the grid covers the most tiles any split can need and
-    // surplus blocks exit at once
-    if (mlp_lds_bytes(sizeof(R) == 8, dm_, mlp_maxw_) > 65536) throw Error(6, "MLP too wide for this build");
-    launch_mlp_t<R>(dm_, w, mlp_maxw_, (nat + 15) / 16 + dm_.nel, G, dEdG, s);
+    // surplus blocks exit at once. A committee runs every member's network on
+    // the same G: each writes its atomic energies, dE/dG is averaged (forces
+    // and dE/dh are linear in it).
+    const int nm = (int)dmm_.size();
+    for (int k = 0; k < nm; ++k) {
+        if (mlp_lds_bytes(sizeof(R) == 8, dmm_[k], maxw_m_[k]) > 65536) throw Error(6, "MLP too wide for this build");
+        launch_mlp_t<R>(dmm_[k], w, maxw_m_[k], (nat + 15) / 16 + dm_.nel, G, dEdG, w.Ei + (size_t)k * nat,
+                        R(1) / R(nm), k > 0, s);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[4], s));
 
@@ -464,7 +523,7 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     count_.ensure(4 * (size_t)nat);
     perm_.ensure(4 * (size_t)nat);
     eidx_sorted_.ensure(4 * (size_t)nat);
-    Ei_.ensure(8 * (size_t)nat);
+    Ei_.ensure(8 * (size_t)nat * dmm_.size());
     vir_.ensure(72 * (size_t)nat);
     sid_.ensure(4 * (size_t)nat);
     csid_.ensure(4 * (size_t)ncell);
@@ -483,7 +542,8 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
 
     // host -> device tables through the pinned staging buffer
     const size_t geo_b = sizeof(StructGeo) * nstruct, ch_b = sizeof(int2) * nchunk, sc_b = 4 * (nstruct + 1);
-    const size_t res_b = 8 * 10 * (size_t)nstruct;
+    const int nm = (int)dmm_.size();
+    const size_t res_b = 8 * 10 * (size_t)nstruct * nm;
     const size_t o_geo = 0, o_ch = (geo_b + 255) / 256 * 256, o_sc = o_ch + (ch_b + 255) / 256 * 256,
                  o_res = o_sc + (sc_b + 255) / 256 * 256, o_flags = o_res + (res_b + 255) / 256 * 256,
                  o_bbox = o_flags + 256;
@@ -610,7 +670,12 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
         compute<double>(w, g, s);
     else
         compute<float>(w, g, s);
-    launch_reduce(w, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), s);
+    for (int k = 0; k < nm; ++k) {       // per member: sum of its E_i (and the common virial)
+        Work wk = w;
+        wk.Ei = w.Ei + (size_t)k * nat;
+        wk.result = w.result + 10 * (size_t)k * nstruct;
+        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), s);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[6], s));
     double *hres = (double *)(hs + o_res);
@@ -643,9 +708,24 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
         HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
         ms_[k] = ms;
     }
+    // committee mean and population standard deviation of the energies
+    // (globalHdnnps.f90:57-67)
+    member_E_.assign((size_t)nstruct * nm, 0.0);
+    std_E_.assign(nstruct, 0.0);
+    for (int k = 0; k < nstruct; ++k) {
+        double mean = 0;
+        for (int j = 0; j < nm; ++j) {
+            member_E_[(size_t)k * nm + j] = hres[10 * ((size_t)j * nstruct + k)];
+            mean += member_E_[(size_t)k * nm + j];
+        }
+        mean /= nm;
+        double var = 0;
+        for (int j = 0; j < nm; ++j) var += (member_E_[(size_t)k * nm + j] - mean) * (member_E_[(size_t)k * nm + j] - mean);
+        std_E_[k] = std::sqrt(var / nm);
+        energy[k] = nm == 1 ? hres[10 * (size_t)k] : mean;
+    }
     for (int k = 0; k < nstruct; ++k) {
         const double *res = hres + 10 * (size_t)k;
-        energy[k] = res[0];
         if (dEdLat && st[k].periodic) {   // dEdLat(a,x) = sum_b W(a,b) inv(lat)(x,b), stored at [3x + a]
             double il[9];
             inv3(st[k].lat, il);
@@ -821,7 +901,16 @@ void Engine::get_symfunctions(int ld, double *G, double *dEdG, double *Ei)
     };
     if (G) fetch(G_, G);
     if (dEdG) fetch(dEdG_, dEdG);
-    if (Ei) HIPCHK(hipMemcpy(Ei, Ei_.p, 8 * (size_t)nat, hipMemcpyDeviceToHost));
+    if (Ei) {   // committee mean of the atomic energies
+        const int nm = (int)dmm_.size();
+        std::vector<double> all((size_t)nat * nm);
+        HIPCHK(hipMemcpy(all.data(), Ei_.p, 8 * all.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nat; ++i) {
+            double a = 0;
+            for (int k = 0; k < nm; ++k) a += all[(size_t)k * nat + i];
+            Ei[i] = nm == 1 ? all[i] : a / nm;
+        }
+    }
 }
 
 void Engine::timings(double *ms6) const

@@ -32,7 +32,13 @@ struct StructIn {
 
 class Engine {
   public:
-    Engine(HostModel m, int precision);
+    // members[0] is the potential; further members form a committee with it
+    // (their networks on the first potential's symmetry functions)
+    Engine(std::vector<HostModel> members, int precision);
+    int committee_size() const { return 1 + (int)members_.size(); }
+    // last evaluation: member energies [nstruct][N] and their standard deviation
+    const std::vector<double> &member_energies() const { return member_E_; }
+    const std::vector<double> &energy_std() const { return std_E_; }
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -74,6 +80,11 @@ class Engine {
     void compute(Work &w, const Geo &g, hipStream_t s);
 
     HostModel model_;
+    std::vector<HostModel> members_;          // committee members 2..N
+    std::vector<DevModel> dmm_;               // device models of all members (dmm_[0] = dm_)
+    std::vector<int> maxw_m_;                 // widest layer of each member's networks
+    std::vector<ElemInfo> ei_host_;
+    std::vector<double> member_E_, std_E_;
     int precision_;
     bool dev_ready_ = false;
     hipStream_t stream_ = nullptr;

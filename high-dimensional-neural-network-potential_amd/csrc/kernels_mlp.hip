@@ -90,8 +90,11 @@ struct MlpShape {
     int total;              // LDS elements
 };
 
+// Ei_out[i] = E_i of this network; dEdG = scale * dE_i/dG, or added to dEdG
+// when accumulate is set (committee mean over several networks).
 template <typename R>
-__global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, const R *G, R *dEdG)
+__global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, const R *G, R *dEdG, double *Ei_out,
+                                            R scale, int accumulate)
 {
     extern __shared__ __align__(16) char smem[];
     typedef typename Mma<R>::V V;
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         out = tmp;
     }
     // atomic energy (hdnnps.f90:526); the output layer has one node
-    if (lane < tn) w.Ei[w.perm[t0 + lane]] = (double)in[lane * ld] + m.einfo[e].atom_energy;
+    if (lane < tn) Ei_out[w.perm[t0 + lane]] = (double)in[lane * ld] + m.einfo[e].atom_energy;
 
     // backward, layer_backward (:116-158) with gradOut = 1
     R *g = out;
@@ -211,7 +214,11 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
     }
     for (int t = lane; t < T * ldg; t += 64) {
         const int a = t / ldg, s = t - a * ldg;
-        if (a < tn) dEdG[(size_t)w.perm[t0 + a] * ldg + s] = s < nsf ? g[a * ld + s] : R(0);
+        if (a < tn) {
+            R *d = dEdG + (size_t)w.perm[t0 + a] * ldg + s;
+            const R v = s < nsf ? scale * g[a * ld + s] : R(0);
+            *d = accumulate ? *d + v : v;
+        }
     }
 }
 
@@ -225,7 +232,8 @@ size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw)
 }
 
 template <typename R>
-void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, hipStream_t s)
+void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, double *Ei, R scale,
+                  bool accumulate, hipStream_t s)
 {
     MlpShape sh{};
     sh.nlayers = m.nlayers;
@@ -239,10 +247,13 @@ void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G,
         off += 16 * sh.ldc[l];
     }
     sh.total = 2 * 16 * sh.ld + off;
-    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total, s, m, w, sh, G, dEdG);
+    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total, s, m, w, sh, G, dEdG, Ei,
+                       scale, accumulate ? 1 : 0);
 }
 
-template void launch_mlp_t<double>(const DevModel &, Work &, int, int, const double *, double *, hipStream_t);
-template void launch_mlp_t<float>(const DevModel &, Work &, int, int, const float *, float *, hipStream_t);
+template void launch_mlp_t<double>(const DevModel &, Work &, int, int, const double *, double *, double *, double,
+                                   bool, hipStream_t);
+template void launch_mlp_t<float>(const DevModel &, Work &, int, int, const float *, float *, double *, float, bool,
+                                  hipStream_t);
 
 }  // namespace hdnnp

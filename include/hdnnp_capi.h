@@ -50,6 +50,23 @@ enum {
  * memory is set up on the first evaluation. */
 int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out);
 
+/* Replaces initGlobalHdnnpsFromInputFile (src/potentials/globalHdnnps.f90:13-29,
+ * which reads the directories from nnps.txt): a committee of n potentials.
+ * The first potential's symmetry functions and scaling serve all members;
+ * member k contributes its networks and atom energies (hdnnpCalcNNs with the
+ * first handle's symmetry functions, globalHdnnps.f90:47-55). Evaluations of
+ * the context (hdnnp_evaluate and its variants) then return the committee
+ * mean of E, F and dEdLat (globalHdnnpEnergyAndForces, :31-70). n = 1 is the
+ * same as hdnnp_create. */
+int hdnnp_create_committee(int32_t n, const char *const *param_dirs, int32_t precision, hdnnp_ctx **out);
+int hdnnp_committee_size(const hdnnp_ctx *ctx, int32_t *n);
+
+/* Of the last evaluation (nstruct structures; 1 for the single-structure
+ * calls): std_dev[nstruct] = sqrt(mean over members of (E_k - mean)^2)
+ * (the `std` argument of globalHdnnpEnergyAndForces) and
+ * member_energies[nstruct][n]. Either pointer may be NULL. */
+int hdnnp_last_committee(hdnnp_ctx *ctx, int32_t nstruct, double *std_dev, double *member_energies);
+
 /* Replaces hdnnpEnergyAndForces(handle, ats, energy, force, dEdLat)
  * (src/potentials/hdnnps.f90:333-371). Host arrays in, host arrays out.
  * dEdLat may be NULL; for nonperiodic input it is set to zero (the reference

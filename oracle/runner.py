@@ -12,6 +12,7 @@ elemSymToNum (src/constants/constants.f90:95-111) and the stable merge sorts
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 from dataclasses import dataclass, field
 
@@ -354,6 +355,26 @@ class Oracle:
         if rc:
             raise RunnerError(f"orc_atoms failed with status {rc}")
         return Ei, G, D
+
+
+def committee(members, s):
+    """globalHdnnpEnergyAndForces (src/potentials/globalHdnnps.f90:31-70): the
+    neighbour lists and scaled symmetry functions of the first potential
+    (:44-46), then hdnnpCalcNNs of every member on them (:47-49; only the
+    member's networks and atom energies enter, hdnnps.f90:523-526); returns the
+    mean energy, forces and dEdLat, the population standard deviation of the
+    member energies (:57-67) and the member energies."""
+    first = members[0]
+    res = []
+    for p in members:
+        hybrid = dataclasses.replace(first, layers=p.layers, atom_energy=p.atom_energy, acts=p.acts,
+                                     nodes=p.nodes)
+        res.append(Oracle(hybrid).evaluate(s))
+    E = np.array([r["energy"] for r in res])
+    mean = E.sum() / len(E)
+    return {"energy": mean, "force": sum(r["force"] for r in res) / len(res),
+            "dEdLat": sum(r["dEdLat"] for r in res) / len(res),
+            "std": float(np.sqrt(((E - mean) ** 2).sum() / len(E))), "members": E}
 
 
 def neighbor_entries(s, rc: float) -> np.ndarray:

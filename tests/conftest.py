@@ -80,3 +80,32 @@ def golden_structure(wl, d):
 
 EVAL_CASES = ["ma", "ma_cell1", "ma_tric2", "ma_cell8", "cluster30", "perov1", "perov2", "perov3",
               "perov132", "perov6"]
+
+
+def committee_member(src, dst, seed, weight_noise=0.05, scaling_factor=None):
+    """A committee member for tests: the potential in `src` with its network
+    weights perturbed (w * (1 + weight_noise * N(0,1)), seeded) and, when
+    scaling_factor is given, the min/max columns of scaling.data scaled by it.
+    Written as RuNNer files to `dst` (a directory path ending in '/')."""
+    import glob
+    import shutil
+    os.makedirs(dst, exist_ok=True)
+    for f in glob.glob(os.path.join(src, "*")):
+        shutil.copyfile(f, os.path.join(dst, os.path.basename(f)))
+    rng = np.random.default_rng(seed)
+    for f in sorted(glob.glob(os.path.join(dst, "weights.*.data"))):
+        w = np.loadtxt(f)
+        w = w * (1.0 + weight_noise * rng.standard_normal(w.shape))
+        np.savetxt(f, w, fmt="%20.12f")
+    if scaling_factor is not None:
+        path = os.path.join(dst, "scaling.data")
+        out = []
+        for line in open(path).read().splitlines():
+            p = line.split()
+            if len(p) == 5:          # element, SF index, min, max, mean
+                p[2] = repr(float(p[2]) * scaling_factor)
+                p[3] = repr(float(p[3]) * scaling_factor)
+                line = "  ".join(p)
+            out.append(line)
+        open(path, "w").write("\n".join(out) + "\n")
+    return dst

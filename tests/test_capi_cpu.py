@@ -177,3 +177,12 @@ def test_fortran_driver_is_built():
     assert os.access(exe, os.X_OK)
     mods = os.path.join(ROOT, "high-dimensional-neural-network-potential_amd", "lib", "hdnnps.mod")
     assert os.path.exists(mods)
+
+
+def test_committee_rejects_mismatched_member(pkg):
+    """Committee members need a network for every element of the first
+    potential, taking its symmetry functions (checked at creation, no GPU)."""
+    from conftest import MA_PARAMS, PEROV_PARAMS
+    with pytest.raises(pkg.HdnnpError) as ei:
+        pkg.Hdnnp([PEROV_PARAMS, MA_PARAMS], "fp64")
+    assert ei.value.code == 3 and "committee member 2" in str(ei.value)

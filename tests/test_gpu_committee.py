@@ -1,0 +1,59 @@
+"""Committee evaluation (hdnnp_create_committee; globalHdnnps.f90:13-70)
+against the oracle's restatement, plus batch and one-member identities.
+Tolerances as the single-potential FP64 tests: E 1e-10 relative, F and
+dEdLat 1e-9, energy std and member energies 1e-10."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import PEROV_PARAMS, committee_member, golden, golden_structure
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def members(tmp_path_factory):
+    d = tmp_path_factory.mktemp("committee")
+    m2 = committee_member(PEROV_PARAMS, str(d / "m2") + "/", seed=2)
+    m3 = committee_member(PEROV_PARAMS, str(d / "m3") + "/", seed=3, scaling_factor=1.3)
+    return [PEROV_PARAMS, m2, m3]
+
+
+def test_committee_matches_oracle(pkg, wl, oracle, members):
+    s = golden_structure(wl, golden("eval_perov2"))
+    c = oracle.committee([oracle.read_potential(d) for d in members], s)
+    h = pkg.Hdnnp(members, "fp64")
+    assert h.committee_size == 3
+    E, F, L = h.energy_and_forces(s)
+    sd, me = h.committee(1)
+    assert abs(E - c["energy"]) <= 1e-10 * max(1.0, abs(c["energy"]))
+    np.testing.assert_allclose(F, c["force"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(L, c["dEdLat"], rtol=0, atol=1e-9 * max(1.0, np.abs(c["dEdLat"]).max()))
+    assert abs(sd[0] - c["std"]) <= 1e-10
+    np.testing.assert_allclose(me[0], c["members"], rtol=1e-10, atol=0)
+    hm = pkg.Hdnnp(members, "mixed")
+    Em, Fm, _ = hm.energy_and_forces(s)
+    assert np.sqrt(np.mean((Fm - F) ** 2)) <= 1e-5
+    assert abs(Em - E) / s.nat <= 1e-5
+    Eg, Fg, Lg, sdg = pkg.globalHdnnpEnergyAndForces(h, s)
+    assert Eg == E and np.array_equal(Fg, F) and sdg == sd[0]
+
+
+def test_committee_of_one_is_the_potential(pkg, wl):
+    s = golden_structure(wl, golden("eval_perov3"))
+    a = pkg.Hdnnp(PEROV_PARAMS, "fp64").energy_and_forces(s)
+    b = pkg.Hdnnp([PEROV_PARAMS], "fp64").energy_and_forces(s)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_committee_batch_equals_single(pkg, wl, members):
+    h = pkg.Hdnnp(members, "mixed")
+    ss = [wl.perovskite(1, 1, 2, seed=21), wl.cluster(30, seed=22), golden_structure(wl, golden("eval_perov2"))]
+    E, F, L = h.energy_and_forces_batch(ss)
+    sd, me = h.committee(len(ss))
+    for k, s in enumerate(ss):
+        e, f, l = h.energy_and_forces(s)
+        sdk, mek = h.committee(1)
+        assert E[k] == e and np.array_equal(F[k], f)
+        assert sd[k] == sdk[0] and np.array_equal(me[k], mek[0])

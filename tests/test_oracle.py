@@ -268,3 +268,33 @@ def test_planted_mistakes_are_caught(oracle, wl):
     }
     for name, mut in mutations.items():
         assert abs(energy_with(mut) - E) > 1e-8, name
+
+
+# ---------------------------------------------------------------- committee
+def test_committee_oracle_pinned(oracle, wl, tmp_path):
+    """The committee restatement (globalHdnnps.f90:31-70) against what its
+    definition fixes: members that share the first potential's symmetry
+    functions give the plain mean of their own evaluations and np.std of the
+    energies; a member's own scaling.data is ignored; one member is the
+    potential itself."""
+    from conftest import committee_member
+    base = POTENTIALS["methylammonium"]
+    m2 = committee_member(base, str(tmp_path / "m2") + "/", seed=2)
+    m3 = committee_member(base, str(tmp_path / "m3") + "/", seed=3)
+    m3s = committee_member(base, str(tmp_path / "m3s") + "/", seed=3, scaling_factor=1.5)
+    P = [oracle.read_potential(p) for p in (base, m2, m3)]
+    s = golden_structure(wl, golden("eval_ma_cell8"))
+    single = [oracle.Oracle(p).evaluate(s) for p in P]
+    E = np.array([r["energy"] for r in single])
+    c = oracle.committee(P, s)
+    assert abs(c["energy"] - E.mean()) <= 1e-12 * abs(E.mean())
+    np.testing.assert_allclose(c["force"], sum(r["force"] for r in single) / 3, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(c["dEdLat"], sum(r["dEdLat"] for r in single) / 3, rtol=0, atol=1e-12)
+    assert abs(c["std"] - E.std()) <= 1e-12
+    assert E.std() > 1e-6                        # the members do differ
+    c2 = oracle.committee([P[0], oracle.read_potential(m3s)], s)
+    assert c2["members"][1] == single[2]["energy"]
+    alone = oracle.Oracle(oracle.read_potential(m3s)).evaluate(s)["energy"]
+    assert abs(alone - single[2]["energy"]) > 1e-6    # the scaling would matter alone
+    c1 = oracle.committee([P[0]], s)
+    assert c1["energy"] == single[0]["energy"] and c1["std"] == 0.0
