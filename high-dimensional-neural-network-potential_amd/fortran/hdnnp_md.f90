@@ -6,6 +6,8 @@
 !       momenta read from a file (the reference draws unseeded random ones)
 !   hdnnp_md eval <paramdir/> <structure> <out> [mixed]
 !       one hdnnpEnergyAndForces call with dEdLat, plus dEdLat2Stress
+!   hdnnp_md committee <nnps.txt> <structure> <out>
+!       one globalHdnnpEnergyAndForces call (committee mean, energy std)
 !
 ! Structure file: "nat periodic" / three lattice-vector lines / "Z x y z" rows.
 program hdnnp_md
@@ -13,6 +15,7 @@ program hdnnp_md
     use constants
     use atomicStructure
     use hdnnps
+    use globalHdnnps
     implicit none
     character(len=512) :: mode, pdir, a3, a4, a5, a6, a7
     type(hdnnp) :: h
@@ -35,8 +38,11 @@ program hdnnp_md
         if (trim(a5) == 'mixed') prec = HDNNP_MIXED
         call parseRuNNerInput(h, trim(pdir), prec)
         call runEval(h, trim(a3), trim(a4))
+    case ('committee')
+        call initGlobalHdnnpsFromInputFile(file=trim(pdir))
+        call runCommittee(trim(a3), trim(a4))
     case default
-        print*, 'usage: hdnnp_md md|eval ...'
+        print*, 'usage: hdnnp_md md|eval|committee ...'
         error stop 2
     end select
     call hdnnpDestroy(h)
@@ -81,6 +87,47 @@ contains
         end do
         close(u)
     end subroutine runMD
+
+    subroutine readStructure(sfile, ats)
+        character(len=*), intent(in) :: sfile
+        type(atStruct), intent(out) :: ats
+        integer :: u, i, iper
+        open(newunit=u, file=sfile, action='read', status='old')
+        read(u, *) ats%nat, iper
+        ats%periodic = (iper /= 0)
+        do i = 1, 3
+            read(u, *) ats%lat(:, i)
+        end do
+        allocate(ats%ats(3, ats%nat), ats%el(ats%nat))
+        do i = 1, ats%nat
+            read(u, *) ats%el(i), ats%ats(:, i)
+        end do
+        close(u)
+    end subroutine readStructure
+
+    subroutine runCommittee(sfile, fname)
+        character(len=*), intent(in) :: sfile, fname
+        type(atStruct) :: ats
+        real(dp) :: energy, dEdLat(3,3), sd
+        real(dp), allocatable :: force(:,:)
+        integer :: u, i
+        call readStructure(sfile, ats)
+        allocate(force(3, ats%nat))
+        call globalHdnnpEnergyAndForces(ats, energy, force, dEdLat, sd)
+        open(newunit=u, file=fname, action='write', status='replace')
+        write(u, '(A,I4)') 'members ', nGlobalHdnnps
+        write(u, '(A,ES25.16E3)') 'energy ', energy
+        write(u, '(A,ES25.16E3)') 'std ', sd
+        write(u, '(A)') 'force'
+        do i = 1, ats%nat
+            write(u, '(3ES25.16E3)') force(:, i)
+        end do
+        write(u, '(A)') 'dEdLat'
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') dEdLat(i, :)
+        end do
+        close(u)
+    end subroutine runCommittee
 
     subroutine runEval(h, sfile, fname)
         type(hdnnp), intent(inout) :: h

@@ -115,6 +115,21 @@ module hdnnp_capi
             import :: c_ptr
             type(c_ptr), value :: ctx
         end subroutine
+        integer(c_int) function hdnnp_create_committee(n, dirs, precision, ctx) &
+                bind(C, name='hdnnp_create_committee')
+            import :: c_int, c_ptr
+            integer(c_int), value :: n, precision
+            type(c_ptr), intent(in) :: dirs(*)
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function hdnnp_last_committee(ctx, nstruct, std_dev, member_energies) &
+                bind(C, name='hdnnp_last_committee')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nstruct
+            real(c_double), intent(out) :: std_dev
+            type(c_ptr), value :: member_energies
+        end function
         integer(c_int) function hdnnp_get_model_info(ctx, info) bind(C, name='hdnnp_get_model_info')
             import :: c_int, c_ptr, hdnnp_model_info
             type(c_ptr), value :: ctx
@@ -296,8 +311,6 @@ contains
         character(len=*) :: path
         integer, intent(in), optional :: precision
         integer(c_int) :: prec, rc
-        type(hdnnp_model_info) :: info
-        integer :: e
         prec = HDNNP_FP64
         if (present(precision)) prec = int(precision, c_int)
         if (c_associated(handle%ctx)) call hdnnp_destroy(handle%ctx)
@@ -306,6 +319,15 @@ contains
             print*, c_message(hdnnp_last_error(c_null_ptr))
             error stop 1
         end if
+        call fillHandle(handle)
+    end subroutine parseRuNNerInput
+
+    ! the handle's public fields from the library context
+    subroutine fillHandle(handle)
+        type(hdnnp), intent(inout) :: handle
+        type(hdnnp_model_info) :: info
+        integer(c_int) :: rc
+        integer :: e
         rc = hdnnp_get_model_info(handle%ctx, info)
         handle%maxRc = info%maxrc
         handle%emin = info%emin
@@ -316,7 +338,7 @@ contains
         do e = 1, info%nelements
             handle%atomenergies(info%Z(e)) = info%atom_energy(e)
         end do
-    end subroutine parseRuNNerInput
+    end subroutine fillHandle
 
     ! hdnnpEnergyAndForces (hdnnps.f90:333-371)
     subroutine hdnnpEnergyAndForces(handle, ats, energy, force, dEdLat)
@@ -350,3 +372,83 @@ contains
     end subroutine hdnnpDestroy
 
 end module hdnnps
+
+
+! Committee of potentials with the reference's interface
+! (src/potentials/globalHdnnps.f90). The members are one library context: the
+! first potential's neighbour lists and symmetry functions, every member's
+! networks; results are the committee means.
+module globalHdnnps
+    use precision
+    use atomicStructure
+    use hdnnp_capi
+    use hdnnps
+    implicit none
+
+    integer :: nGlobalHdnnps = 0
+    type(hdnnp) :: globalCommittee
+
+contains
+
+    ! initGlobalHdnnpsFromInputFile (globalHdnnps.f90:13-29): nnps.txt holds
+    ! the number of potentials, then one parameter directory per line.
+    ! `precision` (optional, not in the reference) as in parseRuNNerInput.
+    subroutine initGlobalHdnnpsFromInputFile(precision, file)
+        integer, intent(in), optional :: precision
+        character(len=*), intent(in), optional :: file
+        integer :: nnunit, i, k, n
+        character(len=512) :: txt
+        character(kind=c_char), allocatable, target :: names(:,:)
+        type(c_ptr), allocatable :: ptrs(:)
+        integer(c_int) :: prec, rc
+
+        prec = HDNNP_FP64
+        if (present(precision)) prec = int(precision, c_int)
+        if (present(file)) then
+            open(newunit=nnunit, file=file, action='read', status='old')
+        else
+            open(newunit=nnunit, file='nnps.txt', action='read', status='old')
+        end if
+        read(nnunit, *) n
+        allocate(names(len(txt) + 1, n), ptrs(n))
+        do i = 1, n
+            read(nnunit, '(A)') txt
+            txt = adjustl(txt)
+            do k = 1, len_trim(txt)
+                names(k, i) = txt(k:k)
+            end do
+            names(len_trim(txt) + 1, i) = c_null_char
+            ptrs(i) = c_loc(names(1, i))
+        end do
+        close(nnunit)
+        if (c_associated(globalCommittee%ctx)) call hdnnp_destroy(globalCommittee%ctx)
+        rc = hdnnp_create_committee(int(n, c_int), ptrs, prec, globalCommittee%ctx)
+        if (rc /= 0) then
+            print*, c_message(hdnnp_last_error(c_null_ptr))
+            error stop 1
+        end if
+        call fillHandle(globalCommittee)
+        nGlobalHdnnps = n
+    end subroutine initGlobalHdnnpsFromInputFile
+
+    ! globalHdnnpEnergyAndForces (globalHdnnps.f90:31-70): committee mean
+    ! energy, forces and dEdLat; std = population standard deviation of the
+    ! member energies
+    subroutine globalHdnnpEnergyAndForces(ats, energy, forces, dEdLat, std)
+        type(atStruct), intent(inout) :: ats
+        real(dp), intent(out) :: forces(3, ats%nat), energy
+        real(dp), intent(out), optional :: std, dEdLat(3,3)
+        real(dp) :: sd
+        integer(c_int) :: rc
+        if (present(dEdLat)) then
+            call hdnnpEnergyAndForces(globalCommittee, ats, energy, forces, dEdLat)
+        else
+            call hdnnpEnergyAndForces(globalCommittee, ats, energy, forces)
+        end if
+        if (present(std)) then
+            rc = hdnnp_last_committee(globalCommittee%ctx, 1_c_int, sd, c_null_ptr)
+            std = sd
+        end if
+    end subroutine globalHdnnpEnergyAndForces
+
+end module globalHdnnps
