@@ -30,7 +30,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device",
-           "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee", "hdnnp_neighbor_count",
+           "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee",
+           "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
@@ -71,6 +72,13 @@ def load_library():
         lib.hdnnp_evaluate.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.hdnnp_evaluate_device.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_evaluate_batch.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp]
+        lib.hdnnp_md_create.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _dp, _dp, _ip, _dp, ctypes.c_double,
+                                        ctypes.c_double, ctypes.POINTER(vp)]
+        lib.hdnnp_md_step.argtypes = [vp, ctypes.c_int32, _dp, _dp]
+        lib.hdnnp_md_get.argtypes = [vp, _dp, _dp, _dp]
+        lib.hdnnp_md_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        lib.hdnnp_md_destroy.argtypes = [vp]
+        lib.hdnnp_md_destroy.restype = None
         lib.hdnnp_create_committee.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
                                                ctypes.POINTER(vp)]
         lib.hdnnp_committee_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
@@ -274,6 +282,76 @@ def parseRuNNerInput(path: str, precision="fp64") -> Hdnnp:
 
 def hdnnpEnergyAndForces(handle: Hdnnp, ats, want_dEdLat: bool = True):
     return handle.energy_and_forces(ats, want_dEdLat)
+
+
+# constants.f90 of the reference: mass unit and the standard atomic weights
+# (amu) of the elements used here
+MASS_UNIT_TO_ELECTRON_MASS = 1822.888513874368
+FEMTOSECONDS_TO_ATOMIC = 41.34137333519408
+_ATOMIC_MASS = {1: 1.008, 2: 4.002, 3: 6.94, 4: 9.012, 5: 10.81, 6: 12.011, 7: 14.007, 8: 15.999, 9: 18.998,
+                10: 20.1797, 11: 22.989, 12: 24.305, 13: 26.981, 14: 28.085, 15: 30.973, 16: 32.06, 17: 35.45,
+                18: 39.948, 53: 126.904, 82: 207.2}
+
+
+def atomic_masses(Z) -> np.ndarray:
+    """getAtomicMass(el) * massUnit2ElectronMass (electron masses)."""
+    return np.array([_ATOMIC_MASS[int(z)] for z in Z]) * MASS_UNIT_TO_ELECTRON_MASS
+
+
+class MdSystems:
+    """hdnnp_md_*: velocity Verlet (hdnnpExample.f90:58-72) of several systems
+    on the device with the forces of `handle`; momenta[k] (nat_k, 3) per system;
+    skin > 0 reuses neighbour lists within rc + skin (Verlet skin)."""
+
+    def __init__(self, handle: Hdnnp, structures, momenta, dt: float, skin: float = 0.0, masses=None):
+        self.h = handle
+        self.lib = handle.lib
+        nat = np.array([s.nat for s in structures], dtype=np.int32)
+        Z = np.ascontiguousarray(np.concatenate([np.asarray(s.Z, np.int32) for s in structures]))
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(s.pos, np.float64).reshape(-1, 3)
+                                                   for s in structures]))
+        mom = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p in momenta]))
+        mass = np.ascontiguousarray(atomic_masses(Z) if masses is None else np.asarray(masses, np.float64))
+        per = np.array([int(bool(s.periodic)) for s in structures], dtype=np.int32)
+        lat = np.ascontiguousarray(np.stack([np.asarray(s.lat, np.float64).reshape(3, 3) for s in structures]))
+        self.nsys = len(structures)
+        self.nat = int(nat.sum())
+        self._off = np.concatenate([[0], np.cumsum(nat)])
+        self.md = ctypes.c_void_p()
+        handle._check(self.lib.hdnnp_md_create(handle.ctx, self.nsys, _p(nat, _ip), _p(Z, _ip), _p(pos, _dp),
+                                               _p(mom, _dp), _p(mass, _dp), _p(per, _ip), _p(lat, _dp),
+                                               float(dt), float(skin), ctypes.byref(self.md)))
+
+    def step(self, n: int = 1):
+        """n steps; returns (epot[nsys], ekin[nsys]) of the last one."""
+        ep = np.zeros(self.nsys)
+        ek = np.zeros(self.nsys)
+        self.h._check(self.lib.hdnnp_md_step(self.md, n, _p(ep, _dp), _p(ek, _dp)))
+        return ep, ek
+
+    def get(self):
+        pos = np.zeros((self.nat, 3))
+        mom = np.zeros((self.nat, 3))
+        f = np.zeros((self.nat, 3))
+        self.h._check(self.lib.hdnnp_md_get(self.md, _p(pos, _dp), _p(mom, _dp), _p(f, _dp)))
+        return pos, mom, f
+
+    @property
+    def stats(self):
+        st, nb = ctypes.c_int64(), ctypes.c_int64()
+        self.h._check(self.lib.hdnnp_md_stats(self.md, ctypes.byref(st), ctypes.byref(nb)))
+        return st.value, nb.value
+
+    def close(self):
+        if getattr(self, "md", None) and self.md.value:
+            self.lib.hdnnp_md_destroy(self.md)
+            self.md = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def initGlobalHdnnpsFromInputFile(path: str = "nnps.txt", precision="fp64") -> Hdnnp:

@@ -9,11 +9,17 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "md.hpp"
 #include "runner.hpp"
 
 struct hdnnp_ctx {
     hdnnp::Engine *eng = nullptr;
     std::string err;
+};
+
+struct hdnnp_md {
+    hdnnp_ctx *ctx = nullptr;     // owns the engine the systems are stepped with
+    hdnnp::MdSystems *sys = nullptr;
 };
 
 static thread_local std::string g_create_error;
@@ -82,6 +88,52 @@ int hdnnp_create_committee(int32_t n, const char *const *param_dirs, int32_t pre
         }
         *out = c;
     });
+}
+
+int hdnnp_md_create(hdnnp_ctx *ctx, int32_t nsys, const int32_t *nat, const int32_t *Z, const double *pos,
+                    const double *mom, const double *mass, const int32_t *periodic, const double *lat, double dt,
+                    double skin, hdnnp_md **out)
+{
+    if (!ctx || !out) return HDNNP_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *m = new hdnnp_md;
+        m->ctx = ctx;
+        try {
+            m->sys = new hdnnp::MdSystems(*ctx->eng, nsys, nat, Z, pos, mom, mass, periodic, lat, dt, skin);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int hdnnp_md_step(hdnnp_md *md, int32_t nsteps, double *epot, double *ekin)
+{
+    if (!md) return HDNNP_ERR_ARG;
+    return guarded(md->ctx, [&] { md->sys->step(nsteps, epot, ekin); });
+}
+
+int hdnnp_md_get(hdnnp_md *md, double *pos, double *mom, double *force)
+{
+    if (!md) return HDNNP_ERR_ARG;
+    return guarded(md->ctx, [&] { md->sys->get(pos, mom, force); });
+}
+
+int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds)
+{
+    if (!md) return HDNNP_ERR_ARG;
+    if (steps) *steps = md->sys->steps();
+    if (list_builds) *list_builds = md->sys->builds();
+    return HDNNP_OK;
+}
+
+void hdnnp_md_destroy(hdnnp_md *md)
+{
+    if (!md) return;
+    delete md->sys;
+    delete md;
 }
 
 int hdnnp_committee_size(const hdnnp_ctx *ctx, int32_t *n)

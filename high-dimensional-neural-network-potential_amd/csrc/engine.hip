@@ -460,12 +460,15 @@ static void structure_grid(const StructIn &in, double rc, StructGeo &G)
     G.nat = in.nat;
 }
 
-void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,
-                 hipStream_t s, double *energy, double *dEdLat, int retry)
+void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
+                   double rc_list)
 {
     const int nstruct = (int)st.size();
-    const double rc = model_.maxrc;
-    if (!(rc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
+    const double rc = rc_list;
+    prep_ = Prepared{};
+    prep_.st = st;
+    ++build_gen_;
+    if (!(model_.maxrc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
     if (dm_.fallback_el < 0) throw Error(3, elem_why_[0].empty() ? "no element of the potential can be evaluated"
                                                                  : elem_why_[0]);
     long nat_l = 0;
@@ -475,10 +478,9 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     }
     if (nat_l >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
     const int nat = (int)nat_l;
-    for (int k = 0; k < nstruct; ++k) energy[k] = 0;
-    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double) * nstruct);
     last_nat_ = 0;
     last_total_ = 0;
+    prep_.nat = nat;
     if (nat == 0) return;
 
     // per-structure grids, offsets and the neighbour-capacity estimate
@@ -588,7 +590,7 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     w.flags = mi + kIntFlags;
     w.Ei = Ei_.as<double>();
     w.vir = vir_.as<double>();
-    w.force = force_dev;
+    w.force = nullptr;              // set by finish
     w.result = res_.as<double>();
 
     HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
@@ -665,7 +667,30 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     launch_nl_rev(g, w, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[2], s));
+    prep_.w = w;
+    prep_.g = g;
+    prep_.nchunk = nchunk;
+    prep_.lds_lim = lds_lim;
+    prep_.o_res = o_res;
+    prep_.o_flags = o_flags;
+    prep_.res_b = res_b;
+    prep_.valid = true;
+}
 
+bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat)
+{
+    const int nstruct = (int)prep_.st.size();
+    for (int k = 0; k < nstruct; ++k) energy[k] = 0;
+    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double) * nstruct);
+    if (prep_.nat == 0) return true;
+    if (!prep_.valid) throw Error(5, "no prepared evaluation");
+    const bool fp64 = precision_ == 0;
+    Work w = prep_.w;
+    w.force = force_dev;
+    const Geo g = prep_.g;
+    const int nat = prep_.nat, nchunk = prep_.nchunk, nm = (int)dmm_.size();
+    const std::vector<StructIn> &st = prep_.st;
+    char *hs = (char *)pinned_;
     if (fp64)
         compute<double>(w, g, s);
     else
@@ -678,9 +703,9 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[6], s));
-    double *hres = (double *)(hs + o_res);
-    int *hflags = (int *)(hs + o_flags);
-    HIPCHK(hipMemcpyAsync(hres, w.result, res_b, hipMemcpyDeviceToHost, s));
+    double *hres = (double *)(hs + prep_.o_res);
+    int *hflags = (int *)(hs + prep_.o_flags);
+    HIPCHK(hipMemcpyAsync(hres, w.result, prep_.res_b, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(hflags, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (hflags[0]) throw Error(4, "structure contains element Z=" + std::to_string(hflags[0]) +
@@ -691,13 +716,12 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
         throw Error(3, "element Z=" + std::to_string(hflags[3]) + " cannot be evaluated");
     }
     if (hflags[1] > 0) {
-        if (hflags[1] > lds_lim)
+        if (hflags[1] > prep_.lds_lim)
             throw Error(6, "an atom has " + std::to_string(hflags[1]) + " neighbours; this build stages at most " +
-                               std::to_string(lds_lim) + " per atom in LDS");
-        if (retry == 0) throw Error(6, "neighbour capacity did not converge");
+                               std::to_string(prep_.lds_lim) + " per atom in LDS");
         cap_hint_ = std::max(cap_hint_, hflags[1]);
-        run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, retry - 1);
-        return;
+        prep_.valid = false;
+        return false;
     }
     if (hflags[2]) throw Error(6, "periodic image shift beyond +-127 cells");
     last_nat_ = nat;
@@ -737,6 +761,18 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
                     dl[3 * x + a] = acc;
                 }
         }
+    }
+    return true;
+}
+
+void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,
+                 hipStream_t s, double *energy, double *dEdLat, int retry)
+{
+    const double rc = model_.maxrc;
+    for (;;) {
+        build(st, Z_dev, pos_dev, s, rc);
+        if (finish(force_dev, s, energy, dEdLat)) return;
+        if (retry-- <= 0) throw Error(6, "neighbour capacity did not converge");
     }
 }
 
@@ -802,23 +838,35 @@ void Engine::evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev
     if (!nat || !periodic || !energy) throw Error(1, "null array");
     hipStream_t s = stream ? stream : stream_;
     std::vector<StructIn> st(nstruct);
-    std::vector<int> off(nstruct + 1, 0), which;
+    long total = 0;
     for (int k = 0; k < nstruct; ++k) {
         StructIn &in = st[k];
         in.nat = nat[k];
         in.periodic = periodic[k] ? 1 : 0;
         if (in.nat < 0) throw Error(1, "nat < 0");
-        if ((long)off[k] + in.nat >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
-        off[k + 1] = off[k] + in.nat;
+        total += in.nat;
+        if (total >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
         if (in.periodic) {
             if (!lat) throw Error(1, "periodic structure without lattice");
             std::memcpy(in.lat, lat + 9 * (size_t)k, sizeof in.lat);
-        } else if (in.nat > 0) {
-            which.push_back(k);
         }
     }
-    if (off[nstruct] > 0 && (!Z_dev || !pos_dev || !force_dev)) throw Error(1, "null array");
-    if (!which.empty()) {   // bounding boxes of the nonperiodic structures, on the device
+    if (total > 0 && (!Z_dev || !pos_dev || !force_dev)) throw Error(1, "null array");
+    device_bboxes(st, pos_dev, s);
+    run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
+}
+
+// bounding boxes of the nonperiodic structures of st (atoms concatenated in
+// pos_dev), computed on the device (neighborLists.f90:131-138)
+void Engine::device_bboxes(std::vector<StructIn> &st, const double *pos_dev, hipStream_t s)
+{
+    const int nstruct = (int)st.size();
+    std::vector<int> off(nstruct + 1, 0), which;
+    for (int k = 0; k < nstruct; ++k) {
+        off[k + 1] = off[k] + st[k].nat;
+        if (!st[k].periodic && st[k].nat > 0) which.push_back(k);
+    }
+    if (!which.empty()) {
         const int nw = (int)which.size();
         const size_t ob = 4 * (size_t)(nstruct + 1), wb = 4 * (size_t)nw, bb = 48 * (size_t)nw;
         char *hs = (char *)staging(ob + wb + bb + 512);
@@ -835,7 +883,6 @@ void Engine::evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev
         HIPCHK(hipStreamSynchronize(s));
         for (int t = 0; t < nw; ++t) std::memcpy(st[which[t]].bbox, hb + 6 * (size_t)t, 6 * sizeof(double));
     }
-    run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
 }
 
 void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,

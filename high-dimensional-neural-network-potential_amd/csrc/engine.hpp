@@ -75,6 +75,26 @@ class Engine {
     // retry: how often an evaluation may be repeated with a larger neighbour capacity
     void run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,
              hipStream_t s, double *energy, double *dEdLat, int retry);
+  public:
+    // The two phases of run. build: cell grids, sorts and neighbour lists
+    // within rc_list (>= the potential's cutoff) of the positions in pos_dev
+    // (wrapped copies in wpos()). finish: symmetry functions of the current
+    // wrapped positions, networks, forces and sums; false (nothing written)
+    // when an atom had more neighbours than the list capacity -- build again.
+    void build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
+               double rc_list);
+    bool finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat);
+    double4 *wpos() { return wpos_.as<double4>(); }
+    // bounding boxes of the nonperiodic structures (for build), from device positions
+    void device_bboxes(std::vector<StructIn> &st, const double *pos_dev, hipStream_t s);
+    int prepared_nat() const { return prep_.nat; }
+    // count of builds so far: a caller that reuses lists checks that no other
+    // evaluation on this engine has built since its own build
+    unsigned long long build_generation() const { return build_gen_; }
+    void ensure_device() { if (!dev_ready_) init_device(); }
+    hipStream_t stream() const { return stream_; }
+
+  private:
     void *staging(size_t bytes);
     template <typename R>
     void compute(Work &w, const Geo &g, hipStream_t s);
@@ -103,6 +123,15 @@ class Engine {
     DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_;
     DevMem geo_, sid_, csid_, chunks_, sc_, res_, partial_, bbox_;
     std::vector<StructGeo> hgeo_;
+    struct Prepared {
+        Work w{};
+        Geo g{};
+        std::vector<StructIn> st;
+        int nat = 0, nchunk = 0, lds_lim = 0;
+        size_t o_res = 0, o_flags = 0, res_b = 0;
+        bool valid = false;
+    } prep_;
+    unsigned long long build_gen_ = 0;
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
     size_t pinned_bytes_ = 0;

@@ -1,0 +1,112 @@
+// Velocity-Verlet kernels of the on-device molecular dynamics and the Verlet
+// skin bookkeeping.
+//
+// Reference: the MD loop of hdnnpExample.f90:58-72,
+//   ats = ats + p / m * dt + 0.5 * forces / m * dt**2
+//   lastForces = forces;  forces = F(ats)
+//   p = p + dt / 2 * (forces + lastForces)
+//   ekin = sum(p**2 / m) / 2
+// with the same operation order per component.
+#include "device.hpp"
+
+namespace hdnnp {
+
+__global__ __launch_bounds__(256) void k_md_drift(double *pos, const double *mom, const double *F, const double *mass,
+                                                  double dt, int n3)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3) return;
+    const double m = mass[x / 3];
+    pos[x] = (pos[x] + mom[x] / m * dt) + 0.5 * F[x] / m * (dt * dt);
+}
+
+__global__ __launch_bounds__(256) void k_md_kick(double *mom, const double *F, const double *Fold, double dt, int n3)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3) return;
+    mom[x] = mom[x] + dt / 2.0 * (F[x] + Fold[x]);
+}
+
+// largest squared displacement since the last list build; maxd2 holds the
+// bits of a non-negative double, which order like unsigned integers
+__global__ __launch_bounds__(256) void k_md_disp(const double *pos, const double *posb, int nat,
+                                                 unsigned long long *maxd2)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < nat) {
+        const double a = pos[3 * i] - posb[3 * i], b = pos[3 * i + 1] - posb[3 * i + 1],
+                     c = pos[3 * i + 2] - posb[3 * i + 2];
+        d2 = a * a + b * b + c * c;
+    }
+    red[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(maxd2, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// wrapped positions for a reused list: those of the build plus the
+// displacement since (no re-wrapping, so the image shifts stay valid)
+__global__ __launch_bounds__(256) void k_md_refresh(double4 *wpos, const double4 *wposb, const double *pos,
+                                                    const double *posb, int nat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    const double4 b = wposb[i];
+    wpos[i] = make_double4(b.x + (pos[3 * i] - posb[3 * i]), b.y + (pos[3 * i + 1] - posb[3 * i + 1]),
+                           b.z + (pos[3 * i + 2] - posb[3 * i + 2]), 0.0);
+}
+
+// kinetic energy of each system: one block per system, fixed-order tree
+__global__ __launch_bounds__(256) void k_md_ekin(const double *mom, const double *mass, const int *off,
+                                                 double *ekin)
+{
+    __shared__ double red[256];
+    const int s = blockIdx.x;
+    double a = 0.0;
+    for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += 256) {
+        const double m = mass[i];
+        a += mom[3 * i] * mom[3 * i] / m + mom[3 * i + 1] * mom[3 * i + 1] / m +
+             mom[3 * i + 2] * mom[3 * i + 2] / m;
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ekin[s] = red[0] / 2.0;
+}
+
+void launch_md_drift(double *pos, const double *mom, const double *F, const double *mass, double dt, int nat,
+                     hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_drift, dim3((3 * nat + 255) / 256), dim3(256), 0, s, pos, mom, F, mass, dt, 3 * nat);
+}
+
+void launch_md_kick(double *mom, const double *F, const double *Fold, double dt, int nat, hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_kick, dim3((3 * nat + 255) / 256), dim3(256), 0, s, mom, F, Fold, dt, 3 * nat);
+}
+
+void launch_md_disp(const double *pos, const double *posb, int nat, unsigned long long *maxd2, hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_disp, dim3((nat + 255) / 256), dim3(256), 0, s, pos, posb, nat, maxd2);
+}
+
+void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, const double *posb, int nat,
+                       hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_refresh, dim3((nat + 255) / 256), dim3(256), 0, s, wpos, wposb, pos, posb, nat);
+}
+
+void launch_md_ekin(const double *mom, const double *mass, const int *off, int nsys, double *ekin, hipStream_t s)
+{
+    if (nsys > 0) hipLaunchKernelGGL(k_md_ekin, dim3(nsys), dim3(256), 0, s, mom, mass, off, ekin);
+}
+
+}  // namespace hdnnp

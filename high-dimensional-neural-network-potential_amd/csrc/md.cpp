@@ -1,0 +1,134 @@
+// Host side of the on-device molecular dynamics (see md.hpp).
+#include "md.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hdnnp {
+
+#define MDCHK(x)                                                                               \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            throw Error(5, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);     \
+    } while (0)
+
+MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const double *pos, const double *mom,
+                     const double *mass, const int *periodic, const double *lat, double dt, double skin)
+    : eng_(eng), dt_(dt), skin_(skin)
+{
+    if (nsys < 1 || !nat || !Z || !pos || !mom || !mass || !periodic) throw Error(1, "null argument or nsys < 1");
+    if (!(dt > 0) || !(skin >= 0)) throw Error(1, "dt must be > 0 and skin >= 0");
+    eng_.ensure_device();
+    st_.resize(nsys);
+    std::vector<int> off(nsys + 1, 0);
+    long total = 0;
+    for (int k = 0; k < nsys; ++k) {
+        if (nat[k] < 0) throw Error(1, "nat < 0");
+        st_[k].nat = nat[k];
+        st_[k].periodic = periodic[k] ? 1 : 0;
+        if (st_[k].periodic) {
+            if (!lat) throw Error(1, "periodic system without lattice");
+            std::memcpy(st_[k].lat, lat + 9 * (size_t)k, sizeof st_[k].lat);
+        }
+        total += nat[k];
+        if (total >= (1L << 30)) throw Error(6, "too many atoms");
+        off[k + 1] = (int)total;
+    }
+    nat_ = (int)total;
+    const size_t n = std::max<size_t>(1, nat_);
+    for (int i = 0; i < nat_; ++i)
+        if (!(mass[i] > 0)) throw Error(1, "masses must be positive");
+    Z_.ensure(4 * n);
+    pos_.ensure(24 * n);
+    mom_.ensure(24 * n);
+    mass_.ensure(8 * n);
+    F_.ensure(24 * n);
+    Fold_.ensure(24 * n);
+    posb_.ensure(24 * n);
+    wposb_.ensure(sizeof(double4) * n);
+    off_.ensure(4 * (nsys + 1));
+    ekin_.ensure(8 * nsys);
+    maxd2_.ensure(8);
+    MDCHK(hipMemcpy(Z_.p, Z, 4 * (size_t)nat_, hipMemcpyHostToDevice));
+    MDCHK(hipMemcpy(pos_.p, pos, 24 * (size_t)nat_, hipMemcpyHostToDevice));
+    MDCHK(hipMemcpy(mom_.p, mom, 24 * (size_t)nat_, hipMemcpyHostToDevice));
+    MDCHK(hipMemcpy(mass_.p, mass, 8 * (size_t)nat_, hipMemcpyHostToDevice));
+    MDCHK(hipMemcpy(off_.p, off.data(), 4 * (size_t)(nsys + 1), hipMemcpyHostToDevice));
+    epot_.assign(nsys, 0.0);
+    ekin_h_.assign(nsys, 0.0);
+    forces(nullptr);     // initial forces (the reference's call before the loop)
+}
+
+MdSystems::~MdSystems()
+{
+    for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_})
+        d->release();
+}
+
+// Forces of the current positions. With skin > 0 the last lists are reused
+// while every atom is within skin/2 of where it was at their build.
+void MdSystems::forces(double *epot)
+{
+    hipStream_t s = eng_.stream();
+    bool rebuild = !built_ || skin_ == 0.0 || eng_.build_generation() != gen_;
+    if (!rebuild) {
+        MDCHK(hipMemsetAsync(maxd2_.p, 0, 8, s));
+        launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, maxd2_.as<unsigned long long>(), s);
+        unsigned long long bits = 0;
+        MDCHK(hipMemcpyAsync(&bits, maxd2_.p, 8, hipMemcpyDeviceToHost, s));
+        MDCHK(hipStreamSynchronize(s));
+        double d2;
+        std::memcpy(&d2, &bits, 8);
+        rebuild = !(d2 <= 0.25 * skin_ * skin_);
+    }
+    for (int attempt = 0;; ++attempt) {
+        if (rebuild) {
+            eng_.device_bboxes(st_, pos_.as<double>(), s);
+            eng_.build(st_, Z_.as<int>(), pos_.as<double>(), s, eng_.model().maxrc + skin_);
+            MDCHK(hipMemcpyAsync(posb_.p, pos_.p, 24 * (size_t)nat_, hipMemcpyDeviceToDevice, s));
+            MDCHK(hipMemcpyAsync(wposb_.p, eng_.wpos(), sizeof(double4) * (size_t)nat_, hipMemcpyDeviceToDevice, s));
+            ++builds_;
+            built_ = true;
+            gen_ = eng_.build_generation();
+        } else {
+            launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
+        }
+        MDCHK(hipGetLastError());
+        if (eng_.finish(F_.as<double>(), s, epot_.data(), nullptr)) break;
+        if (attempt >= 2) throw Error(6, "neighbour capacity did not converge");
+        rebuild = true;   // capacity exceeded: build again with room for it
+    }
+    if (epot) std::copy(epot_.begin(), epot_.end(), epot);
+}
+
+void MdSystems::step(int nsteps, double *epot, double *ekin)
+{
+    if (nsteps < 0) throw Error(1, "nsteps < 0");
+    hipStream_t s = eng_.stream();
+    for (int t = 0; t < nsteps; ++t) {
+        launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
+        std::swap(F_, Fold_);
+        forces(nullptr);
+        launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
+        MDCHK(hipGetLastError());
+        ++steps_;
+    }
+    launch_md_ekin(mom_.as<double>(), mass_.as<double>(), off_.as<int>(), (int)st_.size(), ekin_.as<double>(), s);
+    MDCHK(hipMemcpyAsync(ekin_h_.data(), ekin_.p, 8 * st_.size(), hipMemcpyDeviceToHost, s));
+    MDCHK(hipStreamSynchronize(s));
+    if (epot) std::copy(epot_.begin(), epot_.end(), epot);
+    if (ekin) std::copy(ekin_h_.begin(), ekin_h_.end(), ekin);
+}
+
+void MdSystems::get(double *pos, double *mom, double *force)
+{
+    hipStream_t s = eng_.stream();
+    MDCHK(hipStreamSynchronize(s));
+    if (pos) MDCHK(hipMemcpy(pos, pos_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
+    if (mom) MDCHK(hipMemcpy(mom, mom_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
+    if (force) MDCHK(hipMemcpy(force, F_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
+}
+
+}  // namespace hdnnp

@@ -1,0 +1,49 @@
+// Molecular dynamics of many systems on the device: velocity Verlet
+// (hdnnpExample.f90:58-72) with the forces of the engine, positions and
+// momenta resident in device memory, and a Verlet skin: neighbour lists
+// within rc + skin are reused while no atom has moved more than skin / 2
+// since they were built (skin = 0 builds them at every step, as the
+// reference does). Systems and evaluations may share one engine: a system
+// whose lists another call has replaced since its build builds them again.
+#pragma once
+
+#include <vector>
+
+#include "engine.hpp"
+
+namespace hdnnp {
+
+class MdSystems {
+  public:
+    // nsys systems of nat[k] atoms; Z, pos, mom, mass over all atoms
+    // (pos/mom [sum nat][3], mass per atom in electron masses); periodic[k],
+    // lat[9k..] as for Engine::evaluate_batch_host; dt in atomic time units.
+    MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const double *pos, const double *mom,
+              const double *mass, const int *periodic, const double *lat, double dt, double skin);
+    ~MdSystems();
+    MdSystems(const MdSystems &) = delete;
+    MdSystems &operator=(const MdSystems &) = delete;
+
+    // nsteps velocity-Verlet steps; epot/ekin [nsys] of the last step (may be null)
+    void step(int nsteps, double *epot, double *ekin);
+    void get(double *pos, double *mom, double *force);   // host copies (any may be null)
+    long long steps() const { return steps_; }
+    long long builds() const { return builds_; }
+    int nsys() const { return (int)st_.size(); }
+    int nat() const { return nat_; }
+
+  private:
+    void forces(double *epot);
+
+    Engine &eng_;
+    std::vector<StructIn> st_;
+    int nat_ = 0;
+    double dt_, skin_;
+    DevMem Z_, pos_, mom_, mass_, F_, Fold_, posb_, wposb_, off_, ekin_, maxd2_;
+    std::vector<double> epot_, ekin_h_;
+    bool built_ = false;
+    unsigned long long gen_ = 0;   // engine build generation of this system's lists
+    long long steps_ = 0, builds_ = 0;
+};
+
+}  // namespace hdnnp

@@ -139,6 +139,30 @@ int hdnnp_get_sf_table(hdnnp_ctx *ctx, int32_t e, int32_t cap_rows, double *rows
  * hdnnps.f90:318-327): per layer W(i,j) input-major, then the biases. */
 int hdnnp_get_weights(hdnnp_ctx *ctx, int32_t e, int64_t cap, double *w, int64_t *count);
 
+/* Molecular dynamics on the device: the velocity-Verlet loop of the
+ * reference's example (hdnnpExample.f90:58-72) for nsys systems at once, with
+ * positions and momenta resident in device memory. nat[k] atoms per system;
+ * Z, pos, mom [sum nat][3] (bohr, atomic momentum units) and mass[sum nat]
+ * (electron masses) over all atoms; periodic[k] and lat[9k..] as for
+ * hdnnp_evaluate_batch; dt in atomic time units. skin (bohr) > 0: neighbour
+ * lists within rc + skin are reused while no atom has moved more than skin/2
+ * since their build (a Verlet skin; the reference rebuilds at every step,
+ * which skin = 0 does). The systems use the context's engine: do not
+ * evaluate on the context while a step runs. Any other build on the context
+ * (an evaluation, another MD object's step) replaces the lists, and the next
+ * step builds them again: give MD objects that should reuse their lists a
+ * context each. Forces of the initial positions are computed by
+ * hdnnp_md_create. */
+typedef struct hdnnp_md hdnnp_md;
+int hdnnp_md_create(hdnnp_ctx *ctx, int32_t nsys, const int32_t *nat, const int32_t *Z, const double *pos,
+                    const double *mom, const double *mass, const int32_t *periodic, const double *lat, double dt,
+                    double skin, hdnnp_md **out);
+/* nsteps steps; epot[nsys] and ekin[nsys] (= sum p^2/m / 2) of the last one (may be NULL) */
+int hdnnp_md_step(hdnnp_md *md, int32_t nsteps, double *epot, double *ekin);
+int hdnnp_md_get(hdnnp_md *md, double *pos, double *mom, double *force);
+int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds);
+void hdnnp_md_destroy(hdnnp_md *md);
+
 /* Replaces dEdLat2Stress(lat, dEdLat, stress) (atomicStructure.f90:679-686):
  * stress = -1/det(lat) * dEdLat . lat^T in the reference's Fortran terms. */
 int hdnnp_dEdLat_to_stress(const double *lat, const double *dEdLat, double *stress);
