@@ -34,7 +34,15 @@ EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_eval
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
-           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy")
+           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy",
+           "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_md_open",
+           "hdnnp_server_md_step", "hdnnp_server_md_wait", "hdnnp_server_md_get", "hdnnp_server_md_close",
+           "hdnnp_server_stats", "hdnnp_server_last_error", "hdnnp_server_destroy")
+
+SERVER_STATS = ("evaluations", "eval_atoms", "md_requests", "md_atom_steps", "eval_p50_ms", "eval_p99_ms",
+                "md_p50_ms", "md_p99_ms", "batches", "mean_batch", "peak_reserved_bytes", "budget_bytes",
+                "seconds", "max_busy_lanes", "refused", "md_steps")
+HDNNP_ERR_TIMEOUT = 7
 
 
 class HdnnpError(RuntimeError):
@@ -96,6 +104,22 @@ def load_library():
         lib.hdnnp_last_timings.argtypes = [vp, _dp]
         lib.hdnnp_destroy.argtypes = [vp]
         lib.hdnnp_destroy.restype = None
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        lib.hdnnp_server_create.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(vp)]
+        lib.hdnnp_server_submit.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, i64p]
+        lib.hdnnp_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+        lib.hdnnp_server_md_open.argtypes = [vp, ctypes.c_int32, _ip, _dp, _dp, _dp, ctypes.c_int32, _dp,
+                                             ctypes.c_double, ctypes.c_double, i64p]
+        lib.hdnnp_server_md_step.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, i64p]
+        lib.hdnnp_server_md_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
+        lib.hdnnp_server_md_get.argtypes = [vp, ctypes.c_int64, _dp, _dp, _dp]
+        lib.hdnnp_server_md_close.argtypes = [vp, ctypes.c_int64]
+        lib.hdnnp_server_stats.argtypes = [vp, ctypes.c_int32, _dp]
+        lib.hdnnp_server_last_error.argtypes = []
+        lib.hdnnp_server_last_error.restype = ctypes.c_char_p
+        lib.hdnnp_server_destroy.argtypes = [vp]
+        lib.hdnnp_server_destroy.restype = None
         _LIB = lib
     return _LIB
 
@@ -346,6 +370,106 @@ class MdSystems:
         if getattr(self, "md", None) and self.md.value:
             self.lib.hdnnp_md_destroy(self.md)
             self.md = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """hdnnp_server_*: a request server on the device. Evaluations are queued
+    and evaluated in continuous batches by `lanes` worker threads (each with
+    its own stream), MD sessions keep their state and neighbour lists on the
+    device, and admission is by the device-memory budget (bytes; 0: 80% of
+    the free memory). Results equal those of Hdnnp evaluations bit for bit."""
+
+    def __init__(self, param_dir, precision="mixed", lanes: int = 4, hbm_budget: int = 0,
+                 max_batch_atoms: int = 0):
+        self.lib = load_library()
+        dirs = list(param_dir) if isinstance(param_dir, (list, tuple)) else [param_dir]
+        arr = (ctypes.c_char_p * len(dirs))(*[os.fsencode(d) for d in dirs])
+        self.sv = ctypes.c_void_p()
+        self._check(self.lib.hdnnp_server_create(len(dirs), arr, _PREC[precision], int(lanes), int(hbm_budget),
+                                                 int(max_batch_atoms), ctypes.byref(self.sv)))
+        self._nat = {}
+
+    def _check(self, rc):
+        if rc:
+            raise HdnnpError(rc, self.lib.hdnnp_server_last_error().decode())
+
+    def submit(self, s) -> int:
+        """Queue the evaluation of structure s (.Z, .pos, .lat, .periodic); its ticket."""
+        Z = np.ascontiguousarray(s.Z, dtype=np.int32)
+        pos = np.ascontiguousarray(s.pos, dtype=np.float64).reshape(-1, 3)
+        lat = np.ascontiguousarray(s.lat, dtype=np.float64).reshape(3, 3)
+        t = ctypes.c_int64()
+        self._check(self.lib.hdnnp_server_submit(self.sv, len(Z), _p(Z, _ip), _p(pos, _dp), int(bool(s.periodic)),
+                                                 _p(lat, _dp), ctypes.byref(t)))
+        self._nat[t.value] = len(Z)
+        return t.value
+
+    def result(self, ticket: int, timeout: float = 0.0):
+        """(energy, forces (nat, 3), dEdLat (3, 3), energy std, latency ms) of a
+        ticket; waits for it (timeout in seconds, 0: no limit)."""
+        nat = self._nat.get(ticket)
+        f = np.zeros((nat or 0, 3))
+        d = np.zeros((3, 3))
+        e, sd, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        rc = self.lib.hdnnp_server_wait(self.sv, ticket, float(timeout), ctypes.byref(e),
+                                        _p(f, _dp) if nat else None, _p(d, _dp), ctypes.byref(sd),
+                                        ctypes.byref(lat))
+        if rc != HDNNP_ERR_TIMEOUT:
+            self._nat.pop(ticket, None)
+        self._check(rc)
+        return e.value, f, d, sd.value, lat.value
+
+    def md_open(self, s, momenta, dt: float, skin: float = 1.0, masses=None) -> int:
+        """Open an MD session of structure s with momenta (nat, 3); its id."""
+        Z = np.ascontiguousarray(s.Z, dtype=np.int32)
+        pos = np.ascontiguousarray(s.pos, dtype=np.float64).reshape(-1, 3)
+        mom = np.ascontiguousarray(momenta, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(atomic_masses(Z) if masses is None else masses, dtype=np.float64)
+        lat = np.ascontiguousarray(s.lat, dtype=np.float64).reshape(3, 3)
+        sid = ctypes.c_int64()
+        self._check(self.lib.hdnnp_server_md_open(self.sv, len(Z), _p(Z, _ip), _p(pos, _dp), _p(mom, _dp),
+                                                  _p(mass, _dp), int(bool(s.periodic)), _p(lat, _dp), float(dt),
+                                                  float(skin), ctypes.byref(sid)))
+        self._nat[("s", sid.value)] = len(Z)
+        return sid.value
+
+    def md_step(self, session: int, nsteps: int) -> int:
+        t = ctypes.c_int64()
+        self._check(self.lib.hdnnp_server_md_step(self.sv, session, int(nsteps), ctypes.byref(t)))
+        return t.value
+
+    def md_result(self, ticket: int, timeout: float = 0.0):
+        """(epot, ekin, latency ms) after an MD request."""
+        ep, ek, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.hdnnp_server_md_wait(self.sv, ticket, float(timeout), ctypes.byref(ep),
+                                                  ctypes.byref(ek), ctypes.byref(lat)))
+        return ep.value, ek.value, lat.value
+
+    def md_get(self, session: int):
+        nat = self._nat[("s", session)]
+        pos, mom, f = np.zeros((nat, 3)), np.zeros((nat, 3)), np.zeros((nat, 3))
+        self._check(self.lib.hdnnp_server_md_get(self.sv, session, _p(pos, _dp), _p(mom, _dp), _p(f, _dp)))
+        return pos, mom, f
+
+    def md_close(self, session: int):
+        self._check(self.lib.hdnnp_server_md_close(self.sv, session))
+        self._nat.pop(("s", session), None)
+
+    def stats(self, reset: bool = False) -> dict:
+        out = np.zeros(len(SERVER_STATS))
+        self._check(self.lib.hdnnp_server_stats(self.sv, int(reset), _p(out, _dp)))
+        return dict(zip(SERVER_STATS, out.tolist()))
+
+    def close(self):
+        if getattr(self, "sv", None) and self.sv.value:
+            self.lib.hdnnp_server_destroy(self.sv)
+            self.sv = ctypes.c_void_p()
 
     def __del__(self):
         try:

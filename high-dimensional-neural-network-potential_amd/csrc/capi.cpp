@@ -11,10 +11,15 @@
 #include "engine.hpp"
 #include "md.hpp"
 #include "runner.hpp"
+#include "server.hpp"
 
 struct hdnnp_ctx {
     hdnnp::Engine *eng = nullptr;
     std::string err;
+};
+
+struct hdnnp_server {
+    hdnnp::Server *sv = nullptr;
 };
 
 struct hdnnp_md {
@@ -23,6 +28,26 @@ struct hdnnp_md {
 };
 
 static thread_local std::string g_create_error;
+static thread_local std::string g_server_error;
+
+// server calls: the message goes to the calling thread's server error
+template <typename F>
+static int sguarded(F &&f)
+{
+    try {
+        f();
+        return HDNNP_OK;
+    } catch (const hdnnp::Error &e) {
+        g_server_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_server_error = "host out of memory";
+        return HDNNP_ERR_CAPACITY;
+    } catch (const std::exception &e) {
+        g_server_error = e.what();
+        return HDNNP_ERR_DEVICE;
+    }
+}
 
 template <typename F>
 static int guarded(hdnnp_ctx *ctx, F &&f)
@@ -328,6 +353,105 @@ void hdnnp_destroy(hdnnp_ctx *ctx)
     if (!ctx) return;
     delete ctx->eng;
     delete ctx;
+}
+
+int hdnnp_server_create(int32_t n, const char *const *param_dirs, int32_t precision, int32_t lanes,
+                        int64_t hbm_budget_bytes, int64_t max_batch_atoms, hdnnp_server **out)
+{
+    if (!out || !param_dirs || n < 1 || hbm_budget_bytes < 0) {
+        g_server_error = "hdnnp_server_create: null argument, n < 1 or negative budget";
+        return HDNNP_ERR_ARG;
+    }
+    *out = nullptr;
+    return sguarded([&] {
+        std::vector<hdnnp::HostModel> ms;
+        for (int k = 0; k < n; ++k) {
+            if (!param_dirs[k]) throw hdnnp::Error(HDNNP_ERR_ARG, "hdnnp_server_create: null directory");
+            ms.push_back(hdnnp::read_runner(param_dirs[k]));
+        }
+        auto *h = new hdnnp_server;
+        try {
+            h->sv = new hdnnp::Server(std::move(ms), precision, lanes, (size_t)hbm_budget_bytes, (long)max_batch_atoms);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int hdnnp_server_submit(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, int32_t periodic,
+                        const double *lat, int64_t *ticket)
+{
+    if (!sv || !ticket) return HDNNP_ERR_ARG;
+    return sguarded([&] { *ticket = sv->sv->submit(nat, Z, pos, periodic, lat); });
+}
+
+int hdnnp_server_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double *energy, double *force,
+                      double *dEdLat, double *energy_std, double *latency_ms)
+{
+    if (!sv) return HDNNP_ERR_ARG;
+    bool done = false;
+    const int rc = sguarded([&] { done = sv->sv->wait(ticket, timeout_s, energy, force, dEdLat, energy_std, latency_ms); });
+    if (rc == HDNNP_OK && !done) {
+        g_server_error = "timed out";
+        return HDNNP_ERR_TIMEOUT;
+    }
+    return rc;
+}
+
+int hdnnp_server_md_open(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, const double *mom,
+                         const double *mass, int32_t periodic, const double *lat, double dt, double skin,
+                         int64_t *session)
+{
+    if (!sv || !session) return HDNNP_ERR_ARG;
+    return sguarded([&] { *session = sv->sv->md_open(nat, Z, pos, mom, mass, periodic, lat, dt, skin); });
+}
+
+int hdnnp_server_md_step(hdnnp_server *sv, int64_t session, int32_t nsteps, int64_t *ticket)
+{
+    if (!sv || !ticket) return HDNNP_ERR_ARG;
+    return sguarded([&] { *ticket = sv->sv->md_step(session, nsteps); });
+}
+
+int hdnnp_server_md_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double *epot, double *ekin,
+                         double *latency_ms)
+{
+    if (!sv) return HDNNP_ERR_ARG;
+    bool done = false;
+    const int rc = sguarded([&] { done = sv->sv->md_wait(ticket, timeout_s, epot, ekin, latency_ms); });
+    if (rc == HDNNP_OK && !done) {
+        g_server_error = "timed out";
+        return HDNNP_ERR_TIMEOUT;
+    }
+    return rc;
+}
+
+int hdnnp_server_md_get(hdnnp_server *sv, int64_t session, double *pos, double *mom, double *force)
+{
+    if (!sv) return HDNNP_ERR_ARG;
+    return sguarded([&] { sv->sv->md_get(session, pos, mom, force); });
+}
+
+int hdnnp_server_md_close(hdnnp_server *sv, int64_t session)
+{
+    if (!sv) return HDNNP_ERR_ARG;
+    return sguarded([&] { sv->sv->md_close(session); });
+}
+
+int hdnnp_server_stats(hdnnp_server *sv, int32_t reset, double *out)
+{
+    if (!sv || !out) return HDNNP_ERR_ARG;
+    return sguarded([&] { sv->sv->stats(out, reset != 0); });
+}
+
+const char *hdnnp_server_last_error(void) { return g_server_error.c_str(); }
+
+void hdnnp_server_destroy(hdnnp_server *sv)
+{
+    if (!sv) return;
+    delete sv->sv;
+    delete sv;
 }
 
 }  // extern "C"

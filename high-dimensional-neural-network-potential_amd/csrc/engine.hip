@@ -141,14 +141,37 @@ Engine::Engine(std::vector<HostModel> members, int precision) : precision_(preci
     }
 }
 
+std::vector<DevMem *> Engine::work_buffers()
+{
+    return {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
+            &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
+            &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
+            &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_};
+}
+
+size_t Engine::device_bytes() const
+{
+    size_t b = 0;
+    for (DevMem *d : const_cast<Engine *>(this)->work_buffers()) b += d->bytes;
+    return b;
+}
+
+void Engine::release_work()
+{
+    if (stream_) HIPCHK(hipStreamSynchronize(stream_));
+    for (DevMem *d : work_buffers()) d->release();
+    uploaded_.clear();
+    prep_ = Prepared{};
+    ++build_gen_;
+    last_nat_ = 0;
+    last_total_ = 0;
+}
+
 Engine::~Engine()
 {
     for (auto &d : model_mem_) d.release();
-    for (DevMem *d : {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
-                      &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
-                      &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
-                      &vir_, &misc_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_})
-        d->release();
+    for (DevMem *d : work_buffers()) d->release();
+    misc_.release();
     if (pinned_) (void)hipHostFree(pinned_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -460,6 +483,53 @@ static void structure_grid(const StructIn &in, double rc, StructGeo &G)
     G.nat = in.nat;
 }
 
+// neighbour estimate of one structure from its mean density (as build uses it)
+static double neighbour_estimate(const StructGeo &G, double rc)
+{
+    if (G.nat == 0) return 0.0;
+    const double vol = std::fabs(det3(G.lat));
+    double e = 4.0 / 3.0 * M_PI * rc * rc * rc * G.nat / std::max(vol, 1e-300);
+    if (!G.periodic) e = std::min(e, (double)(G.nat - 1));
+    return e;
+}
+
+// entry capacity per atom: a power of two from 64 covering the estimate (or
+// the most neighbours this engine has seen), at most lim
+static int entry_capacity(double est, int hint, int lim)
+{
+    const double want = std::max((double)hint, std::min(1.2 * est + 8.0, 1e6));
+    int cap = 64;
+    while (cap < want && cap < lim) cap *= 2;
+    return cap;
+}
+
+void Engine::footprint_add(Footprint &f, const StructIn &in, double rc_list) const
+{
+    StructGeo G;
+    structure_grid(in, rc_list, G);
+    f.nat += in.nat;
+    f.ncell += G.ncell;
+    f.ps += (long)in.nat * G.S;
+    f.nstruct += 1;
+    f.est = std::max(f.est, neighbour_estimate(G, rc_list));
+}
+
+size_t Engine::footprint_bytes(const Footprint &f) const
+{
+    const size_t R = precision_ == 0 ? 8 : 4, nm = (size_t)committee_size();
+    size_t nsfmax = 0;
+    for (const ElementModel &em : model_.el) nsfmax = std::max(nsfmax, em.sfs.size());
+    const size_t cap = (size_t)entry_capacity(f.est, cap_hint_, 1024);
+    const size_t nat = (size_t)f.nat;
+    size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
+    b += nat * cap * (16 + 3 * R);                                                           // list slots
+    b += (size_t)f.ncell * 12;                                                               // cells
+    if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
+    b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure
+    b += (nat / kReduceChunk + (size_t)f.nstruct + 1) * 88;                                 // reduction chunks
+    return b + b / 4 + (size_t(8) << 20);   // growth slack of each buffer, sort temporaries
+}
+
 void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
                    double rc_list)
 {
@@ -505,12 +575,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
         cell_off += G.ncell;
         ps_total += (long)G.nat * G.S;
         if (cell_off > (1L << 28)) throw Error(6, "cell grids too large");
-        if (G.nat > 0) {
-            const double vol = std::fabs(det3(G.lat));
-            double e = 4.0 / 3.0 * M_PI * rc * rc * rc * G.nat / std::max(vol, 1e-300);
-            if (!G.periodic) e = std::min(e, (double)(G.nat - 1));
-            est = std::max(est, e);
-        }
+        est = std::max(est, neighbour_estimate(G, rc));
     }
     const int ncell = (int)cell_off;
 
@@ -640,14 +705,9 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     // estimate from the mean density, rounded up to a staging size of the
     // symmetry-function kernels. An atom with more neighbours raises flags[1]
     // and the evaluation is repeated with room for it.
-    {
-        const double want = std::max((double)cap_hint_, std::min(1.2 * est + 8.0, 1e6));
-        int cap = 64;
-        while (cap < want && cap < lds_lim) cap *= 2;
-        w.cap = cap;
-        w.cap_log2 = 0;
-        while ((1 << w.cap_log2) < cap) ++w.cap_log2;
-    }
+    w.cap = entry_capacity(est, cap_hint_, lds_lim);
+    w.cap_log2 = 0;
+    while ((1 << w.cap_log2) < w.cap) ++w.cap_log2;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
     const size_t slots = (size_t)nat * w.cap;
     ent_j_.ensure(4 * slots);

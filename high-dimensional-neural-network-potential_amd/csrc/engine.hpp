@@ -94,8 +94,23 @@ class Engine {
     void ensure_device() { if (!dev_ready_) init_device(); }
     hipStream_t stream() const { return stream_; }
 
+    // Device memory of the work buffers: bytes held, and their release (the
+    // next evaluation allocates again; the device copy of the potential stays)
+    size_t device_bytes() const;
+    void release_work();
+    // Device bytes an evaluation needs, from the sizes build and compute give
+    // their buffers: structures are added one at a time (lists within
+    // rc_list; a nonperiodic structure needs its bounding box)
+    struct Footprint {
+        long nat = 0, ncell = 0, ps = 0, nstruct = 0;
+        double est = 0;   // largest neighbour-count estimate
+    };
+    void footprint_add(Footprint &f, const StructIn &in, double rc_list) const;
+    size_t footprint_bytes(const Footprint &f) const;
+
   private:
     void *staging(size_t bytes);
+    std::vector<DevMem *> work_buffers();
     template <typename R>
     void compute(Work &w, const Geo &g, hipStream_t s);
 

@@ -67,6 +67,14 @@ MdSystems::~MdSystems()
         d->release();
 }
 
+size_t MdSystems::device_bytes() const
+{
+    size_t b = 0;
+    for (const DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_})
+        b += d->bytes;
+    return b;
+}
+
 // Forces of the current positions. With skin > 0 the last lists are reused
 // while every atom is within skin/2 of where it was at their build.
 void MdSystems::forces(double *epot)

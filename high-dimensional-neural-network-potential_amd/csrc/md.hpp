@@ -30,6 +30,7 @@ class MdSystems {
     long long steps() const { return steps_; }
     long long builds() const { return builds_; }
     int nsys() const { return (int)st_.size(); }
+    size_t device_bytes() const;   // the systems' own device state (not the engine's)
     int nat() const { return nat_; }
 
   private:

@@ -41,7 +41,8 @@ enum {
     HDNNP_ERR_MODEL = 3,    /* model not usable (unsorted/mixed-rc SFs...) */
     HDNNP_ERR_ELEMENT = 4,  /* structure holds an element the model lacks  */
     HDNNP_ERR_DEVICE = 5,   /* HIP runtime error                           */
-    HDNNP_ERR_CAPACITY = 6  /* a size limit of this build was exceeded     */
+    HDNNP_ERR_CAPACITY = 6, /* a size limit of this build was exceeded     */
+    HDNNP_ERR_TIMEOUT = 7   /* a server wait ran out of time               */
 };
 
 /* Replaces parseRuNNerInput(handle, path) (src/potentials/hdnnps.f90:34-233):
@@ -162,6 +163,57 @@ int hdnnp_md_step(hdnnp_md *md, int32_t nsteps, double *epot, double *ekin);
 int hdnnp_md_get(hdnnp_md *md, double *pos, double *mom, double *force);
 int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds);
 void hdnnp_md_destroy(hdnnp_md *md);
+
+/* Request server. The reference has no counterpart: its potential is called
+ * by one serial caller at a time (hdnnps.f90:333, hdnnpExample.f90:58-72).
+ * The server keeps the potential resident on the device for many callers:
+ *  - Evaluations (hdnnp_server_submit / _wait) are queued and batched
+ *    continuously. Each free lane takes every queued evaluation that fits,
+ *    in arrival order, up to max_batch_atoms atoms (0: 262144), and
+ *    evaluates them as one batch. Lanes are worker threads with their own
+ *    stream and buffers, so their work overlaps on the device.
+ *  - MD sessions (hdnnp_server_md_*) keep positions, momenta and neighbour
+ *    lists (with a Verlet skin) resident between requests.
+ *  - Admission is by device memory. The buffers of the lanes and the state
+ *    of the sessions stay within hbm_budget_bytes (0: 80% of the free
+ *    device memory). Work waits until it fits; work larger than the whole
+ *    budget is refused with HDNNP_ERR_CAPACITY.
+ *  - Results are the same alone and in any mix: a structure's energy,
+ *    forces and dEdLat equal those of hdnnp_evaluate bit for bit.
+ * Units and layouts are those of hdnnp_evaluate and hdnnp_md_create. Every
+ * ticket must be waited for once. Calls may come from any thread. Errors
+ * of server calls are reported by hdnnp_server_last_error, per thread. */
+typedef struct hdnnp_server hdnnp_server;
+int hdnnp_server_create(int32_t n, const char *const *param_dirs, int32_t precision, int32_t lanes,
+                        int64_t hbm_budget_bytes, int64_t max_batch_atoms, hdnnp_server **out);
+/* queue one evaluation (the inputs are copied) */
+int hdnnp_server_submit(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, int32_t periodic,
+                        const double *lat, int64_t *ticket);
+/* Wait for an evaluation's results (timeout_s <= 0: no limit;
+ * HDNNP_ERR_TIMEOUT leaves the ticket valid). Any output may be NULL;
+ * energy_std is the committee's energy spread (0 for one potential);
+ * latency_ms is the time from submission to completion. */
+int hdnnp_server_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double *energy, double *force,
+                      double *dEdLat, double *energy_std, double *latency_ms);
+/* open an MD session of one system (opened on a lane once it fits) */
+int hdnnp_server_md_open(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, const double *mom,
+                         const double *mass, int32_t periodic, const double *lat, double dt, double skin,
+                         int64_t *session);
+/* queue nsteps velocity-Verlet steps (0: just the energies of the current state) */
+int hdnnp_server_md_step(hdnnp_server *sv, int64_t session, int32_t nsteps, int64_t *ticket);
+int hdnnp_server_md_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double *epot, double *ekin,
+                         double *latency_ms);
+/* state after all the session's queued requests; close frees its device state */
+int hdnnp_server_md_get(hdnnp_server *sv, int64_t session, double *pos, double *mom, double *force);
+int hdnnp_server_md_close(hdnnp_server *sv, int64_t session);
+/* out[16]: evaluations, their atoms, MD requests, MD atom-steps, p50 and
+ * p99 evaluation latency (ms), p50 and p99 MD-request latency (ms),
+ * batches, mean structures per batch, peak reserved device bytes, budget,
+ * seconds since the last reset, most lanes busy at once, refused requests,
+ * MD steps. reset != 0 starts a new window. */
+int hdnnp_server_stats(hdnnp_server *sv, int32_t reset, double *out);
+const char *hdnnp_server_last_error(void);
+void hdnnp_server_destroy(hdnnp_server *sv);
 
 /* Replaces dEdLat2Stress(lat, dEdLat, stress) (atomicStructure.f90:679-686):
  * stress = -1/det(lat) * dEdLat . lat^T in the reference's Fortran terms. */

@@ -1,0 +1,153 @@
+"""Request server (hdnnp_server_*): continuous batching of evaluations and MD
+sessions from several client threads. Every result equals the one obtained
+alone (a context of its own, no other work) bit for bit; admission keeps the
+reserved device memory within the budget; an error stays with its request."""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+from conftest import PEROV_PARAMS, committee_member
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.5 * 41.34137333519408
+
+
+def _mix(wl, n, seed, nmax_cell=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        sub = int(rng.integers(1 << 30))
+        if k % 2 == 0:
+            c = rng.integers(1, nmax_cell + 1, size=3)
+            out.append(wl.perovskite(int(c[0]), int(c[1]), int(c[2]), seed=sub))
+        else:
+            out.append(wl.cluster(int(rng.integers(8, 65)), seed=sub))
+    return out
+
+
+def _alone(pkg, precision, structures):
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    return [h.energy_and_forces(s) for s in structures]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_server_mix_equals_alone(pkg, wl, precision):
+    """40 evaluations submitted by three client threads while two MD
+    sessions step: each evaluation equals its evaluation alone, each session
+    follows its trajectory alone, and the lanes ran at the same time."""
+    structs = _mix(wl, 40, seed=11)
+    want = _alone(pkg, precision, structs)
+    rng = np.random.default_rng(3)
+    md_structs = [wl.perovskite(2, 2, 1, seed=41), wl.perovskite(1, 2, 2, seed=42)]
+    md_p = [rng.standard_normal((s.nat, 3)) * 2.0 for s in md_structs]
+    md_alone = []
+    for s, p in zip(md_structs, md_p):
+        h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+        m = pkg.MdSystems(h, [s], [p], DT, skin=1.0)
+        e = [m.step(3) for _ in range(8)]
+        md_alone.append(([x[0][0] for x in e], [x[1][0] for x in e], m.get()))
+
+    sv = pkg.Server(PEROV_PARAMS, precision, lanes=3, max_batch_atoms=700)
+    sessions = [sv.md_open(s, p, DT, skin=1.0) for s, p in zip(md_structs, md_p)]
+    tickets = [None] * len(structs)
+
+    def client(part):
+        for k in part:
+            tickets[k] = sv.submit(structs[k])
+
+    md_tickets = [[], []]
+    threads = [threading.Thread(target=client, args=(range(c, len(structs), 3),)) for c in range(3)]
+    for t in threads:
+        t.start()
+    for _ in range(8):
+        for q, sid in enumerate(sessions):
+            md_tickets[q].append(sv.md_step(sid, 3))
+    for t in threads:
+        t.join()
+    for k, t in enumerate(tickets):
+        E, F, D, sd, lat = sv.result(t, timeout=120)
+        E0, F0, D0 = want[k]
+        assert E == E0 and np.array_equal(F, F0) and np.array_equal(D, D0), k
+        assert sd == 0.0 and lat > 0
+    for q, sid in enumerate(sessions):
+        res = [sv.md_result(t, timeout=120) for t in md_tickets[q]]
+        ep, ek, (pos0, mom0, f0) = md_alone[q]
+        assert [r[0] for r in res] == ep and [r[1] for r in res] == ek
+        pos, mom, f = sv.md_get(sid)
+        assert np.array_equal(pos, pos0) and np.array_equal(mom, mom0) and np.array_equal(f, f0)
+        sv.md_close(sid)
+    st = sv.stats()
+    print({k: st[k] for k in ("evaluations", "batches", "mean_batch", "max_busy_lanes", "eval_p50_ms",
+                              "eval_p99_ms", "md_p50_ms", "md_p99_ms", "peak_reserved_bytes")}, flush=True)
+    assert st["evaluations"] == len(structs) and st["md_requests"] == 16 and st["md_steps"] == 48
+    assert st["max_busy_lanes"] >= 2
+    assert st["eval_p50_ms"] <= st["eval_p99_ms"]
+    sv.close()
+
+
+def test_server_budget_admission(pkg, wl):
+    """A small budget: requests wait for memory instead of failing, the
+    reserved bytes never exceed the budget, requests and sessions larger than
+    the whole budget are refused, and the results are unchanged."""
+    budget = 28 << 20
+    structs = [wl.perovskite(4, 4, 3, seed=60 + k) for k in range(6)] + _mix(wl, 10, seed=12)
+    want = _alone(pkg, "mixed", structs)
+    sv = pkg.Server(PEROV_PARAMS, "mixed", lanes=4, hbm_budget=budget, max_batch_atoms=600)
+    tickets = [sv.submit(s) for s in structs]
+    big = wl.perovskite(16, 16, 8, seed=1)
+    with pytest.raises(pkg.HdnnpError) as ei:
+        sv.submit(big)
+    assert ei.value.code == 6 and "budget" in str(ei.value)
+    with pytest.raises(pkg.HdnnpError) as ei:
+        sv.md_open(big, np.zeros((big.nat, 3)), DT)
+    assert ei.value.code == 6
+    for k, t in enumerate(tickets):
+        E, F, D, _, _ = sv.result(t, timeout=120)
+        assert E == want[k][0] and np.array_equal(F, want[k][1]) and np.array_equal(D, want[k][2])
+    st = sv.stats()
+    print({k: st[k] for k in ("batches", "max_busy_lanes", "peak_reserved_bytes", "refused")}, flush=True)
+    assert 0 < st["peak_reserved_bytes"] <= budget
+    assert st["refused"] == 2
+    sv.close()
+
+
+def test_server_error_stays_with_its_request(pkg, wl):
+    """An element the potential lacks fails that request only; the others
+    of its batch are evaluated as alone."""
+    structs = _mix(wl, 6, seed=13)
+    want = _alone(pkg, "fp64", structs)
+    bad = wl.cluster(20, seed=5)
+    bad.Z = bad.Z.copy()
+    bad.Z[3] = 79
+    sv = pkg.Server(PEROV_PARAMS, "fp64", lanes=1)
+    ts = [sv.submit(s) for s in structs[:3]] + [sv.submit(bad)] + [sv.submit(s) for s in structs[3:]]
+    with pytest.raises(pkg.HdnnpError) as ei:
+        sv.result(ts[3], timeout=120)
+    assert ei.value.code == 4 and "Z=79" in str(ei.value)
+    for k, t in enumerate(ts[:3] + ts[4:]):
+        E, F, D, _, _ = sv.result(t, timeout=120)
+        assert E == want[k][0] and np.array_equal(F, want[k][1]) and np.array_equal(D, want[k][2])
+    with pytest.raises(pkg.HdnnpError):
+        sv.result(ts[0])                      # a ticket is taken once
+    sv.close()
+
+
+def test_server_committee(pkg, wl, tmp_path):
+    """A server of two potentials returns the committee mean and spread of a
+    committee context."""
+    m2 = str(tmp_path / "m2") + "/"
+    committee_member(PEROV_PARAMS, m2, seed=7)
+    structs = _mix(wl, 6, seed=14)
+    h = pkg.Hdnnp([PEROV_PARAMS, m2], "fp64")
+    sv = pkg.Server([PEROV_PARAMS, m2], "fp64", lanes=2)
+    ts = [sv.submit(s) for s in structs]
+    for s, t in zip(structs, ts):
+        E0, F0, D0 = h.energy_and_forces(s)
+        sd0 = h.committee(1)[0][0]
+        E, F, D, sd, _ = sv.result(t, timeout=120)
+        assert E == E0 and np.array_equal(F, F0) and np.array_equal(D, D0) and sd == sd0 and sd > 0
+    sv.close()
