@@ -12,6 +12,8 @@ force RMSE against the FP64 path is <= 1e-5 Ha/bohr, else the FP64 rate.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
     python bench.py --calibrate-copy   # plain 512 MiB device copy, for HBM counter calibration
+    python bench.py --workload batch   # 4096 independent structures per evaluation
+    python bench.py --workload serve   # request server: Poisson stream + 64 MD sessions
 
 `roofline.traffic` (HBM bytes per evaluation of the three symmetry-function /
 MLP kernels) is read from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
@@ -34,6 +36,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "high-dimensional-neural-network-potential_amd")
 PARAMS = os.path.join(PKG_DIR, "data", "perovskite")
+# same symmetry functions and network shapes with repulsive weights: velocity
+# Verlet on it conserves energy (on the random weights it does not)
+PARAMS_MD = os.path.join(PKG_DIR, "data", "perovskite_md")
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}      # MI355X FP64 / FP32 vector peaks
 FORCE_RMSE_BOUND = 1e-5                           # Ha/bohr
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "perovskite16_hbm_traffic.txt")
@@ -230,6 +235,140 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_window(pkg, mode, pool, md_cells, args):
+    """One serving run: a Poisson stream of evaluation requests drawn from
+    `pool`, and MD sessions (one client thread each, args.md_steps steps per
+    request) stepping for args.seconds. Returns the server's counters."""
+    import threading
+    sv = pkg.Server(PARAMS_MD, mode, lanes=args.lanes, max_batch_atoms=args.max_batch_atoms)
+    rng = np.random.default_rng(5)
+    # warm-up, then the closed-loop capacity of the lanes for this pool
+    for t in [sv.submit(s) for s in pool[:16]]:
+        sv.result(t)
+    sv.stats(reset=True)
+    a = time.perf_counter()
+    for t in [sv.submit(s) for s in pool]:
+        sv.result(t)
+    cap = sum(s.nat for s in pool) / (time.perf_counter() - a)
+    mean_nat = float(np.mean([s.nat for s in pool]))
+    rate = args.load * cap / mean_nat
+    kT = 300.0 * 3.166811563e-6                          # 300 K in Hartree
+    sessions = []
+    for k, s in enumerate(md_cells):
+        m = pkg.atomic_masses(s.Z)
+        p = np.sqrt(m * kT)[:, None] * rng.standard_normal((s.nat, 3))
+        sessions.append(sv.md_open(s, p, 0.5 * 41.34137333519408, skin=1.0))
+    for sid in sessions:
+        sv.md_result(sv.md_step(sid, 0))                # opened
+    stop = threading.Event()
+    errors = []
+
+    def md_client(sid):
+        try:
+            while not stop.is_set():
+                sv.md_result(sv.md_step(sid, args.md_steps))
+        except Exception as e:                            # reported after the run
+            errors.append(repr(e))
+
+    pending = []
+    lock = threading.Lock()
+    done_submitting = threading.Event()
+
+    def collector():
+        try:
+            while True:
+                with lock:
+                    t = pending.pop(0) if pending else None
+                if t is None:
+                    if done_submitting.is_set():
+                        return
+                    time.sleep(0.0005)
+                    continue
+                sv.result(t)
+        except Exception as e:
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=md_client, args=(sid,)) for sid in sessions]
+    threads.append(threading.Thread(target=collector))
+    sv.stats(reset=True)
+    t0 = time.perf_counter()
+    for th in threads:
+        th.start()
+    nsub, t_next = 0, t0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next - t0 >= args.seconds:
+            break
+        while True:
+            now = time.perf_counter()
+            if now >= t_next:
+                break
+            time.sleep(min(t_next - now, 0.001))
+        t = sv.submit(pool[int(rng.integers(len(pool)))])
+        with lock:
+            pending.append(t)
+        nsub += 1
+    stop.set()
+    done_submitting.set()
+    for th in threads:
+        th.join()
+    st = sv.stats()
+    wall = time.perf_counter() - t0
+    for sid in sessions:
+        sv.md_close(sid)
+    sv.close()
+    if errors:
+        raise RuntimeError(f"serving run failed: {errors[:3]}")
+    st.update({"wall_s": wall, "offered_req_per_s": rate, "closed_loop_atoms_per_s": cap, "submitted": nsub,
+               "atoms_per_s": (st["eval_atoms"] + st["md_atom_steps"]) / wall,
+               "eval_atoms_per_s": st["eval_atoms"] / wall, "md_atom_steps_per_s": st["md_atom_steps"] / wall})
+    return st
+
+
+def bench_serve(args):
+    """Serving: a Poisson stream of energy + forces + stress requests (atom
+    counts log-uniform in 8..20,000, alternately periodic perovskite supercells
+    and nonperiodic C/N/H clusters) at args.load of the lanes' closed-loop
+    capacity, while 64 MD sessions of 1536-atom perovskite cells (8x4x4, 300 K,
+    dt 0.5 fs, 1-bohr Verlet skin, state resident on the device) step
+    continuously. The potential is the repulsive variant of the synthetic one
+    (data/perovskite_md: same symmetry functions and network shapes, so the
+    same work per atom), on which the sessions conserve energy. Reports served
+    atoms/s (evaluated atoms + MD atom-steps) with p50/p99 latency."""
+    import torch
+    torch.cuda.is_available()
+    pkg = load_package()
+    pool = pkg.workloads.serving_requests(args.pool, seed=77)
+    md_cells = [pkg.workloads.perovskite(8, 4, 4, seed=900 + k) for k in range(args.sessions)]
+    print(f"workload: serving, pool of {len(pool)} requests (mean {np.mean([s.nat for s in pool]):.0f} atoms), "
+          f"{len(md_cells)} MD sessions x {md_cells[0].nat if md_cells else 0} atoms", file=sys.stderr, flush=True)
+    res = {}
+    for mode in args.modes.split(","):
+        st = serve_window(pkg, mode, pool, md_cells, args)
+        res[mode] = st
+        print(f"{mode}: {st['atoms_per_s']:.4g} atoms/s served ({st['evaluations']:.0f} evaluations, "
+              f"p50 {st['eval_p50_ms']:.2f} ms, p99 {st['eval_p99_ms']:.2f} ms; {st['md_steps']:.0f} MD steps, "
+              f"request p50 {st['md_p50_ms']:.2f} ms, p99 {st['md_p99_ms']:.2f} ms; mean batch "
+              f"{st['mean_batch']:.1f}; lanes busy at once {st['max_busy_lanes']:.0f})", file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in res else next(iter(res))
+    r = res[head]
+    out = {"metric": "atoms/s (E+F+stress) served; p50/p99 latency", "value": r["atoms_per_s"],
+           "unit": "atoms/s", "n_gpus": 1, "steps": int(r["evaluations"]), "warmup": 16,
+           "ms_per_step": r["eval_p50_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if head == "fp64" else "f32 (symmetry functions, MLP); f64 (wrap, list, sums)",
+           "data": "synthetic (seeded structures, seeded random-weight potential)",
+           "config": {"workload": f"Poisson stream at {args.load:.0%} of closed-loop capacity, requests of 8-20,000 "
+                                  f"atoms (log-uniform, half periodic), plus {args.sessions} MD sessions of 1536 "
+                                  f"atoms ({args.md_steps} steps per request, 300 K, dt 0.5 fs, skin 1 bohr); repulsive "
+                                  f"variant of the synthetic potential (same shapes)",
+                      "lanes": args.lanes, "max_batch_atoms": args.max_batch_atoms, "seconds": args.seconds,
+                      "precision": head},
+           "latency_ms": {"eval_p50": r["eval_p50_ms"], "eval_p99": r["eval_p99_ms"],
+                          "md_request_p50": r["md_p50_ms"], "md_request_p99": r["md_p99_ms"]},
+           "modes": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,18 +378,26 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--workload", default="cell", choices=["cell", "batch"],
-                    help="cell: one perovskite supercell (headline); batch: many independent structures")
+    ap.add_argument("--workload", default="cell", choices=["cell", "batch", "serve"],
+                    help="cell: one perovskite supercell (headline); batch: many independent structures; "
+                         "serve: request server under a Poisson stream with MD sessions")
     ap.add_argument("--nstruct", type=int, default=4096, help="structures of the batch workload")
+    ap.add_argument("--pool", type=int, default=128, help="serve: distinct request structures")
+    ap.add_argument("--sessions", type=int, default=64, help="serve: MD sessions")
+    ap.add_argument("--md-steps", type=int, default=10, help="serve: MD steps per session request")
+    ap.add_argument("--seconds", type=float, default=15.0, help="serve: length of each timed run")
+    ap.add_argument("--load", type=float, default=0.6, help="serve: offered load, fraction of capacity")
+    ap.add_argument("--lanes", type=int, default=4, help="serve: server lanes")
+    ap.add_argument("--max-batch-atoms", type=int, default=65536, help="serve: atoms per batch")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload == "batch":
+    if args.workload in ("batch", "serve"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        bench_batch(args)
+        (bench_batch if args.workload == "batch" else bench_serve)(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "

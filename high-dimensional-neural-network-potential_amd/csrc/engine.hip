@@ -739,18 +739,21 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
 
 bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat)
 {
-    const int nstruct = (int)prep_.st.size();
-    for (int k = 0; k < nstruct; ++k) energy[k] = 0;
-    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double) * nstruct);
-    if (prep_.nat == 0) return true;
+    finish_launch(force_dev, s);
+    return finish_collect(s, energy, dEdLat);
+}
+
+void Engine::finish_launch(double *force_dev, hipStream_t s)
+{
+    if (prep_.nat == 0) return;
     if (!prep_.valid) throw Error(5, "no prepared evaluation");
     const bool fp64 = precision_ == 0;
     Work w = prep_.w;
     w.force = force_dev;
+    prep_.w.force = force_dev;
     const Geo g = prep_.g;
     const int nat = prep_.nat, nchunk = prep_.nchunk, nm = (int)dmm_.size();
-    const std::vector<StructIn> &st = prep_.st;
-    char *hs = (char *)pinned_;
+    const int nstruct = (int)prep_.st.size();
     if (fp64)
         compute<double>(w, g, s);
     else
@@ -763,6 +766,19 @@ bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dE
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[6], s));
+}
+
+bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)
+{
+    const int nstruct = (int)prep_.st.size();
+    for (int k = 0; k < nstruct; ++k) energy[k] = 0;
+    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double) * nstruct);
+    if (prep_.nat == 0) return true;
+    if (!prep_.valid) throw Error(5, "no prepared evaluation");
+    const Work &w = prep_.w;
+    const int nat = prep_.nat, nm = (int)dmm_.size();
+    const std::vector<StructIn> &st = prep_.st;
+    char *hs = (char *)pinned_;
     double *hres = (double *)(hs + prep_.o_res);
     int *hflags = (int *)(hs + prep_.o_flags);
     HIPCHK(hipMemcpyAsync(hres, w.result, prep_.res_b, hipMemcpyDeviceToHost, s));

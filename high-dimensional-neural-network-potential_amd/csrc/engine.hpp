@@ -84,6 +84,10 @@ class Engine {
     void build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
                double rc_list);
     bool finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat);
+    // finish in two parts: the kernels (queued on s, nothing waited for),
+    // then the read-back, checks and sums (waits for s)
+    void finish_launch(double *force_dev, hipStream_t s);
+    bool finish_collect(hipStream_t s, double *energy, double *dEdLat);
     double4 *wpos() { return wpos_.as<double4>(); }
     // bounding boxes of the nonperiodic structures (for build), from device positions
     void device_bboxes(std::vector<StructIn> &st, const double *pos_dev, hipStream_t s);

@@ -61,6 +61,26 @@ __global__ __launch_bounds__(256) void k_md_refresh(double4 *wpos, const double4
                            b.z + (pos[3 * i + 2] - posb[3 * i + 2]), 0.0);
 }
 
+// state of a step (positions, momenta, forces) to or from one saved slot
+__global__ __launch_bounds__(256) void k_md_save(double *slot, const double *pos, const double *mom, const double *F,
+                                                 int n3)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3) return;
+    slot[x] = pos[x];
+    slot[n3 + x] = mom[x];
+    slot[2 * n3 + x] = F[x];
+}
+
+__global__ __launch_bounds__(256) void k_md_load(const double *slot, double *pos, double *mom, double *F, int n3)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3) return;
+    pos[x] = slot[x];
+    mom[x] = slot[n3 + x];
+    F[x] = slot[2 * n3 + x];
+}
+
 // kinetic energy of each system: one block per system, fixed-order tree
 __global__ __launch_bounds__(256) void k_md_ekin(const double *mom, const double *mass, const int *off,
                                                  double *ekin)
@@ -102,6 +122,16 @@ void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, c
                        hipStream_t s)
 {
     if (nat > 0) hipLaunchKernelGGL(k_md_refresh, dim3((nat + 255) / 256), dim3(256), 0, s, wpos, wposb, pos, posb, nat);
+}
+
+void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_save, dim3((3 * nat + 255) / 256), dim3(256), 0, s, slot, pos, mom, F, 3 * nat);
+}
+
+void launch_md_load(const double *slot, double *pos, double *mom, double *F, int nat, hipStream_t s)
+{
+    if (nat > 0) hipLaunchKernelGGL(k_md_load, dim3((3 * nat + 255) / 256), dim3(256), 0, s, slot, pos, mom, F, 3 * nat);
 }
 
 void launch_md_ekin(const double *mom, const double *mass, const int *off, int nsys, double *ekin, hipStream_t s)

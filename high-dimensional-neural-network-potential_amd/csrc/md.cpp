@@ -51,6 +51,11 @@ MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const 
     off_.ensure(4 * (nsys + 1));
     ekin_.ensure(8 * nsys);
     maxd2_.ensure(8);
+    const size_t q = (size_t(256) << 20) / (72 * n);   // saved states that fit in 256 MiB
+    maxk_ = (int)std::min<size_t>(kMaxChunk, q > 2 ? q - 1 : 1);
+    snap_.ensure(72 * n * (size_t)(maxk_ + 1));
+    d2_.ensure(8 * kMaxChunk);
+    MDCHK(hipHostMalloc((void **)&d2_host_, 8 * kMaxChunk, hipHostMallocDefault));
     MDCHK(hipMemcpy(Z_.p, Z, 4 * (size_t)nat_, hipMemcpyHostToDevice));
     MDCHK(hipMemcpy(pos_.p, pos, 24 * (size_t)nat_, hipMemcpyHostToDevice));
     MDCHK(hipMemcpy(mom_.p, mom, 24 * (size_t)nat_, hipMemcpyHostToDevice));
@@ -63,14 +68,16 @@ MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const 
 
 MdSystems::~MdSystems()
 {
-    for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_})
+    for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_, &snap_, &d2_})
         d->release();
+    if (d2_host_) (void)hipHostFree(d2_host_);
 }
 
 size_t MdSystems::device_bytes() const
 {
     size_t b = 0;
-    for (const DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_})
+    for (const DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_,
+                            &snap_, &d2_})
         b += d->bytes;
     return b;
 }
@@ -98,7 +105,10 @@ void MdSystems::forces(double *epot)
             MDCHK(hipMemcpyAsync(posb_.p, pos_.p, 24 * (size_t)nat_, hipMemcpyDeviceToDevice, s));
             MDCHK(hipMemcpyAsync(wposb_.p, eng_.wpos(), sizeof(double4) * (size_t)nat_, hipMemcpyDeviceToDevice, s));
             ++builds_;
+            if (built_ && since_ > 0) interval_ = since_;
+            since_ = 0;
             built_ = true;
+            careful_ = false;
             gen_ = eng_.build_generation();
         } else {
             launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
@@ -111,17 +121,76 @@ void MdSystems::forces(double *epot)
     if (epot) std::copy(epot_.begin(), epot_.end(), epot);
 }
 
+void MdSystems::careful_step()
+{
+    hipStream_t s = eng_.stream();
+    launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
+    std::swap(F_, Fold_);
+    forces(nullptr);
+    launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
+    MDCHK(hipGetLastError());
+    ++steps_;
+    ++since_;
+}
+
+int MdSystems::chunk(int k)
+{
+    hipStream_t s = eng_.stream();
+    const size_t slot = 9 * (size_t)nat_;    // doubles per saved state
+    double *sp = snap_.as<double>();
+    launch_md_save(sp, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, s);
+    MDCHK(hipMemsetAsync(d2_.p, 0, 8 * (size_t)k, s));
+    unsigned long long *d2 = d2_.as<unsigned long long>();
+    for (int i = 0; i < k; ++i) {
+        launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
+        std::swap(F_, Fold_);
+        launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, d2 + i, s);
+        launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
+        eng_.finish_launch(F_.as<double>(), s);
+        launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
+        if (i + 1 < k) launch_md_save(sp + (i + 1) * slot, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, s);
+    }
+    MDCHK(hipGetLastError());
+    MDCHK(hipMemcpyAsync(d2_host_, d2_.p, 8 * (size_t)k, hipMemcpyDeviceToHost, s));
+    const bool ok = eng_.finish_collect(s, epot_.data(), nullptr);   // waits for the stream
+    const double lim = 0.25 * skin_ * skin_;
+    int j = ok ? k : 0;              // first step whose lists were stale
+    for (int i = 0; i < k && ok; ++i) {
+        double d;
+        std::memcpy(&d, &d2_host_[i], 8);
+        if (!(d <= lim)) {
+            j = i;
+            break;
+        }
+    }
+    if (j < k) {   // back to the state after step j-1 (the chunk's start for j = 0)
+        launch_md_load(sp + j * slot, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, s);
+        MDCHK(hipGetLastError());
+        ++rollbacks_;
+    }
+    steps_ += j;
+    since_ += j;
+    return j;
+}
+
 void MdSystems::step(int nsteps, double *epot, double *ekin)
 {
     if (nsteps < 0) throw Error(1, "nsteps < 0");
     hipStream_t s = eng_.stream();
-    for (int t = 0; t < nsteps; ++t) {
-        launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
-        std::swap(F_, Fold_);
-        forces(nullptr);
-        launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
-        MDCHK(hipGetLastError());
-        ++steps_;
+    for (int t = 0; t < nsteps;) {
+        const bool reuse = skin_ > 0 && built_ && !careful_ && eng_.build_generation() == gen_;
+        if (!reuse) {
+            careful_step();
+            ++t;
+            continue;
+        }
+        // through the step after the one the last interval predicts for the
+        // next build, so that the build step usually falls inside the chunk
+        const int want = interval_ > 0 ? interval_ - since_ + 1 : 4;
+        const int k = std::min(nsteps - t, std::max(2, std::min(want, maxk_)));
+        const int done = chunk(k);
+        t += done;
+        if (done < k) careful_ = true;   // one step at a time up to the list build
     }
     launch_md_ekin(mom_.as<double>(), mass_.as<double>(), off_.as<int>(), (int)st_.size(), ekin_.as<double>(), s);
     MDCHK(hipMemcpyAsync(ekin_h_.data(), ekin_.p, 8 * st_.size(), hipMemcpyDeviceToHost, s));

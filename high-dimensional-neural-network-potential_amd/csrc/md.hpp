@@ -5,6 +5,15 @@
 // since they were built (skin = 0 builds them at every step, as the
 // reference does). Systems and evaluations may share one engine: a system
 // whose lists another call has replaced since its build builds them again.
+//
+// Steps that reuse the lists run in chunks without waiting for the device:
+// each step's largest displacement is recorded on the device and its state
+// (positions, momenta, forces) saved, and the chunk is checked once at its
+// end. From the first step that moved an atom beyond skin/2 on, the chunk is
+// undone (the state after the step before it restored) and stepping goes on
+// one step at a time, checked before each force evaluation, up to the list
+// build. The trajectory is the one of checking every step. Chunks are sized
+// from the steps between the last two builds.
 #pragma once
 
 #include <vector>
@@ -33,14 +42,24 @@ class MdSystems {
     size_t device_bytes() const;   // the systems' own device state (not the engine's)
     int nat() const { return nat_; }
 
+    long long rollbacks() const { return rollbacks_; }
+
   private:
+    static constexpr int kMaxChunk = 64;
     void forces(double *epot);
+    void careful_step();          // one step, the skin checked before its forces
+    int chunk(int k);             // up to k steps on reused lists; the steps kept
 
     Engine &eng_;
     std::vector<StructIn> st_;
     int nat_ = 0;
     double dt_, skin_;
-    DevMem Z_, pos_, mom_, mass_, F_, Fold_, posb_, wposb_, off_, ekin_, maxd2_;
+    DevMem Z_, pos_, mom_, mass_, F_, Fold_, posb_, wposb_, off_, ekin_, maxd2_, snap_, d2_;
+    unsigned long long *d2_host_ = nullptr;   // pinned, kMaxChunk entries
+    int maxk_ = kMaxChunk;         // chunk limit (the saved states stay within 256 MiB)
+    int since_ = 0, interval_ = 0;  // steps since the last build, and between the two before
+    bool careful_ = true;          // step one at a time until the next list build
+    long long rollbacks_ = 0;
     std::vector<double> epot_, ekin_h_;
     bool built_ = false;
     unsigned long long gen_ = 0;   // engine build generation of this system's lists

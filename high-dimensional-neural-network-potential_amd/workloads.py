@@ -119,6 +119,71 @@ def cluster(nat: int, seed: int, box: float = 20.0, dmin: float = 1.8,
     return Structure(pos, Z, np.zeros((3, 3)), False)
 
 
+def cluster_scaled(nat: int, seed: int, dmin: float = 1.8, probs=(0.5, 0.25, 0.25),
+                   ref_nat: int = 36, ref_box: float = 20.0) -> Structure:
+    """Nonperiodic C/N/H cluster of any size: the rejection recipe of
+    ``cluster`` in a cube of side ref_box * max(1, nat / ref_nat)^(1/3), so a
+    large cluster keeps the density of a ref_nat-atom one in the 20-bohr box.
+    Candidates are checked against a hash grid of cell size dmin."""
+    rng = np.random.default_rng(seed)
+    box = ref_box * max(1.0, nat / ref_nat) ** (1.0 / 3.0)
+    ncell = max(1, int(box / dmin))
+    cell = box / ncell
+    grid: dict = {}
+    pos = np.zeros((nat, 3))
+    n = 0
+    d2 = dmin * dmin
+    while n < nat:
+        cand = rng.random((256, 3)) * box
+        for p in cand:
+            c = np.minimum((p / cell).astype(int), ncell - 1)
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        for q in grid.get((c[0] + dx, c[1] + dy, c[2] + dz), ()):
+                            if np.sum((pos[q] - p) ** 2) < d2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pos[n] = p
+                grid.setdefault((c[0], c[1], c[2]), []).append(n)
+                n += 1
+                if n == nat:
+                    break
+    Z = rng.choice(np.array([1, 6, 7], dtype=np.int32), size=nat, p=probs).astype(np.int32)
+    return Structure(pos, Z, np.zeros((3, 3)), False)
+
+
+def perovskite_near(nat: int, seed: int) -> Structure:
+    """Perovskite supercell n1 x n2 x n3 of about nat atoms (12 per formula
+    unit, at least one), as near a cube as the count allows."""
+    nfu = max(1, int(round(nat / 12)))
+    n1 = max(1, int(round(nfu ** (1.0 / 3.0))))
+    n2 = max(1, int(round((nfu / n1) ** 0.5)))
+    n3 = max(1, int(round(nfu / (n1 * n2))))
+    return perovskite(n1, n2, n3, seed=seed)
+
+
+def serving_requests(nreq: int, seed: int = 77, nmin: int = 8, nmax: int = 20000):
+    """Request structures of a serving stream: atom counts log-uniform in
+    nmin..nmax, alternately periodic perovskite supercells (perovskite_near)
+    and nonperiodic C/N/H clusters (cluster_scaled)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        sub = int(rng.integers(1 << 30))
+        out.append(perovskite_near(n, sub) if k % 2 == 0 else cluster_scaled(n, sub))
+    return out
+
+
 def batch_mix(nstruct: int, seed: int = 2024, nmax_cell: int = 4, cl_min: int = 8, cl_max: int = 64):
     """Batch of independent structures: half perovskite supercells
     n1 x n2 x n3 with each n uniform in 1..nmax_cell (12 to 12*nmax_cell^3
@@ -232,6 +297,32 @@ def write_weights(dirpath: str, nsf: int, seed: int = 4321, elements=ELEMENTS) -
         for l in range(len(sizes) - 1):
             nin, nout = sizes[l], sizes[l + 1]
             w = rng.normal(0.0, 1.0 / np.sqrt(nin), size=(nout, nin))
+            vals.extend(w.T.reshape(-1).tolist())     # input-major
+            vals.extend([0.0] * nout)
+        with open(os.path.join(dirpath, f"weights.{z:03d}.data"), "w") as f:
+            for v in vals:
+                f.write(f"{float(v)!r}\n")
+
+
+def write_weights_repulsive(dirpath: str, nsf: int, ng2: int, out_scale: float = 1.0, seed: int = 4322,
+                            elements=ELEMENTS) -> None:
+    """weights.ZZZ.data of a purely repulsive network of the same shape as
+    write_weights: the first layer reads only the ng2 radial inputs (the G2
+    come first in the sorted order), every weight is positive (uniform in
+    [0.5, 1.5] times 0.06, 1/fan_in and out_scale/fan_in per layer) and the
+    biases are 0, so each atomic energy increases with every G2 -- that is,
+    as any neighbour comes closer. A soft-sphere model on which velocity
+    Verlet conserves energy; its cost per atom is that of the random one."""
+    rng = np.random.default_rng(seed)
+    sizes = [nsf, *HIDDEN, 1]
+    gains = [0.06, 1.0 / sizes[1], out_scale / sizes[2]]
+    for z in elements:
+        vals = []
+        for l in range(len(sizes) - 1):
+            nin, nout = sizes[l], sizes[l + 1]
+            w = gains[l] * rng.uniform(0.5, 1.5, size=(nout, nin))
+            if l == 0:
+                w[:, ng2:] = 0.0
             vals.extend(w.T.reshape(-1).tolist())     # input-major
             vals.extend([0.0] * nout)
         with open(os.path.join(dirpath, f"weights.{z:03d}.data"), "w") as f:

@@ -15,6 +15,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 GOLD = os.path.join(ROOT, "tests", "golden")
 MA_PARAMS = os.path.join(GOLD, "methylammonium")
 PEROV_PARAMS = os.path.join(PKG_DIR, "data", "perovskite")
+PEROV_MD_PARAMS = os.path.join(PKG_DIR, "data", "perovskite_md")
 POTENTIALS = {"methylammonium": MA_PARAMS, "perovskite": PEROV_PARAMS}
 
 

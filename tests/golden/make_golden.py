@@ -8,6 +8,7 @@ fixtures (inputs and expected outputs). The reference's example parameter
 files are copied as data fixtures into tests/golden/methylammonium/.
 
     python tests/golden/make_golden.py
+    python tests/golden/make_golden.py md-potential   # only data/perovskite_md
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ REFDUMP = os.path.join(ROOT, "oracle", "_ref", "refdump")
 GOLD = os.path.join(ROOT, "tests", "golden")
 MA_DIR = os.path.join(GOLD, "methylammonium")
 PEROV_DIR = os.path.join(PKG, "data", "perovskite")
+MD_DIR = os.path.join(PKG, "data", "perovskite_md")
 KB = 3.166811965e-6           # constants.f90:10
 
 
@@ -165,6 +167,22 @@ def make_perovskite_potential():
     wl.write_input_nn(os.path.join(PEROV_DIR, "input.nn"), scale=True)
 
 
+def make_md_potential():
+    """data/perovskite_md: the perovskite potential's symmetry functions and
+    scaling with repulsive network weights (workloads.write_weights_repulsive),
+    the output layer scaled so that the largest force on a jittered 3x3x3
+    cell is 0.02 Ha/bohr (forces are linear in the output weights)."""
+    os.makedirs(MD_DIR, exist_ok=True)
+    for fn in ("input.nn", "scaling.data"):
+        shutil.copy(os.path.join(PEROV_DIR, fn), os.path.join(MD_DIR, fn))
+    wl.write_weights_repulsive(MD_DIR, 80, 40, out_scale=1.0)
+    s = wl.perovskite(3, 3, 3, seed=17)
+    fmax = float(np.abs(eval_case(MD_DIR, s, keep_sf=False)["force"]).max())
+    wl.write_weights_repulsive(MD_DIR, 80, 40, out_scale=0.02 / fmax)
+    f2 = float(np.abs(eval_case(MD_DIR, s, keep_sf=False)["force"]).max())
+    print(f"perovskite_md: output scale {0.02 / fmax:.6g}, max force {f2:.4g} Ha/bohr")
+
+
 def md_momenta(s, seed=99, T=140.0):
     rng = np.random.default_rng(seed)
     m = np.array([wl.MASS[int(z)] for z in s.Z]) * wl.MASS_UNIT_TO_ELECTRON
@@ -178,6 +196,7 @@ def main():
         shutil.copy(os.path.join(REF, "example", "hdnnp-params", fn), MA_DIR)
     shutil.copy(os.path.join(REF, "example", "input.xyz"), os.path.join(GOLD, "methylammonium.xyz"))
     make_perovskite_potential()
+    make_md_potential()
 
     with tempfile.TemporaryDirectory() as td:
         for name, pdir in (("table_methylammonium", MA_DIR), ("table_perovskite", PEROV_DIR)):
@@ -249,4 +268,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["md-potential"]:      # only data/perovskite_md
+        make_md_potential()
+    else:
+        main()

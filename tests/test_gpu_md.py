@@ -122,3 +122,34 @@ def test_md_shared_context_rebuilds(pkg, wl):
     assert a1.stats[1] < 5
     np.testing.assert_allclose(ea, e1, rtol=0, atol=1e-9)
     np.testing.assert_allclose(ea, g["steps"][:30, 0], rtol=0, atol=1e-9)
+
+
+def test_md_chunks_equal_single_steps(pkg, wl):
+    """Steps on reused lists run in chunks that are checked at their end and
+    undone (then stepped one at a time) when an atom left the skin: 300 steps
+    in one call equal 300 calls of one step bit for bit, with the same list
+    builds, on the repulsive perovskite potential at 300 K, and the total
+    energy is conserved."""
+    from conftest import PEROV_MD_PARAMS
+    s = wl.perovskite(4, 2, 2, seed=71)
+    m = pkg.atomic_masses(s.Z)
+    rng = np.random.default_rng(8)
+    p = np.sqrt(m * 300.0 * 3.166811563e-6)[:, None] * rng.standard_normal((s.nat, 3))
+    out = []
+    for single in (True, False):
+        h = pkg.parseRuNNerInput(PEROV_MD_PARAMS, "mixed")
+        md = pkg.MdSystems(h, [s], [p], DT, skin=1.0)
+        e0 = sum(x[0] for x in md.step(0))
+        if single:
+            for _ in range(300):
+                ep, ek = md.step(1)
+        else:
+            ep, ek = md.step(300)
+        out.append((ep[0], ek[0], *md.get(), md.stats))
+        print(f"single {single}: builds {md.stats[1]}, drift {ep[0] + ek[0] - e0:.3e}", flush=True)
+        assert abs(ep[0] + ek[0] - e0) < 1e-3
+    a, b = out
+    assert a[0] == b[0] and a[1] == b[1] and a[5] == b[5]
+    for x, y in zip(a[2:5], b[2:5]):
+        assert np.array_equal(x, y)
+    assert 5 < a[5][1] < 150          # lists were reused, and rebuilt
