@@ -144,89 +144,108 @@ void Server::finish_job(Job &j, int code, const std::string &err)
     cv_done_.notify_all();
 }
 
-// The oldest runnable work: an MD request, or a batch of evaluations in
-// arrival order. Called with mu_ held by an idle lane.
+// The next work of a lane. Evaluations (latency-bound requests) go first:
+// every queued one that fits, in arrival order, as one batch. MD requests run
+// when no evaluation waits, oldest first and each session's in order; one
+// that has waited longer than kMdAging goes before the evaluations, so that
+// sessions never starve. Called with mu_ held by an idle lane.
 bool Server::pick(int l, std::vector<std::shared_ptr<Job>> &batch, std::shared_ptr<Job> &one)
 {
     Lane &L = *lanes_[l];
     const size_t others = reserved_total() - L.reserved;
-    std::vector<const Session *> seen;
     std::vector<Job *> failed;
+
+    // the oldest runnable MD request
+    std::shared_ptr<Job> md;
+    {
+        std::vector<const Session *> seen;
+        bool open_blocked = false;
+        for (auto &jp : queue_) {
+            Job &j = *jp;
+            if (j.kind == Job::Eval) continue;
+            const Session *S = j.sess.get();
+            if (std::find(seen.begin(), seen.end(), S) != seen.end()) continue;   // only its oldest request
+            seen.push_back(S);
+            if (j.kind == Job::Step) {
+                if (S->failed) {
+                    j.fcode = S->code;
+                    j.ferr = S->err;
+                    failed.push_back(&j);
+                    continue;
+                }
+                if (!S->opened || S->busy) continue;
+                md = jp;
+                break;
+            }
+            if (open_blocked) continue;                 // sessions open in arrival order
+            if (reserved_total() + j.est > budget_) {
+                request_trims(l, reserved_total() + j.est - budget_);
+                if (L.held > 0) L.trim = true;
+                open_blocked = true;
+                continue;
+            }
+            md = jp;
+            break;
+        }
+    }
+    const bool md_overdue = md && Clock::now() - md->t_submit > kMdAging;
+
+    // a batch of evaluations
     Engine::Footprint fp;
     long atoms = 0;
     size_t need_batch = 0;
     bool mem_blocked = false;
     for (auto &jp : queue_) {
+        if (md_overdue) break;
         Job &j = *jp;
-        if (j.kind == Job::Eval) {
-            if (mem_blocked) continue;
-            if (!batch.empty() && atoms + j.st.nat > max_batch_atoms_) continue;
-            Engine::Footprint f2 = fp;
-            size_t est = 0;
-            try {
-                L.eng->footprint_add(f2, j.st, rc_);
-                est = L.eng->footprint_bytes(f2);
-            } catch (const Error &e) {
-                j.fcode = e.code;
-                j.ferr = e.what();
-                failed.push_back(&j);
-                continue;
-            }
-            const size_t need = std::max(L.held, est);
-            if (others + need > budget_) {
-                if (batch.empty()) {
-                    if (est > budget_) {         // does not fit even on an empty server
-                        j.fcode = 6;
-                        j.ferr = "the request needs about " + std::to_string(est / kMiB) +
-                                 " MiB of device memory; the server's budget is " + std::to_string(budget_ / kMiB) +
-                                 " MiB";
-                        refused_ += 1;
-                        failed.push_back(&j);
-                        continue;
-                    }
-                    // idle lanes give back their buffers, then this lane its own excess
-                    request_trims(l, others + need - budget_);
-                    if (L.held > est && others + est <= budget_) L.trim = true;
-                }
-                mem_blocked = true;
-                continue;
-            }
-            fp = f2;
-            atoms += j.st.nat;
-            need_batch = need;
-            batch.push_back(jp);
-            continue;
-        }
-        const Session *S = j.sess.get();
-        if (std::find(seen.begin(), seen.end(), S) != seen.end()) continue;   // only its oldest request
-        seen.push_back(S);
-        if (!batch.empty()) continue;
-        if (j.kind == Job::Open) {
-            if (mem_blocked) continue;
-            if (reserved_total() + j.est > budget_) {
-                request_trims(l, reserved_total() + j.est - budget_);
-                if (!L.running && L.held > 0) L.trim = true;
-                mem_blocked = true;
-                continue;
-            }
-            j.sess->bytes = j.est;
-            sessions_bytes_ += j.est;
-            j.sess->busy = true;
-            one = jp;
-            break;
-        }
-        if (S->failed) {
-            j.fcode = S->code;
-            j.ferr = S->err;
+        if (j.kind != Job::Eval || mem_blocked) continue;
+        if (!batch.empty() && atoms + j.st.nat > max_batch_atoms_) continue;
+        Engine::Footprint f2 = fp;
+        size_t est = 0;
+        try {
+            L.eng->footprint_add(f2, j.st, rc_);
+            est = L.eng->footprint_bytes(f2);
+        } catch (const Error &e) {
+            j.fcode = e.code;
+            j.ferr = e.what();
             failed.push_back(&j);
             continue;
         }
-        if (!S->opened || S->busy) continue;
-        j.sess->busy = true;
-        one = jp;
-        break;
+        const size_t need = std::max(L.held, est);
+        if (others + need > budget_) {
+            if (batch.empty()) {
+                if (est > budget_) {         // does not fit even on an empty server
+                    j.fcode = 6;
+                    j.ferr = "the request needs about " + std::to_string(est / kMiB) +
+                             " MiB of device memory; the server's budget is " + std::to_string(budget_ / kMiB) +
+                             " MiB";
+                    refused_ += 1;
+                    failed.push_back(&j);
+                    continue;
+                }
+                // idle lanes give back their buffers, then this lane its own excess
+                request_trims(l, others + need - budget_);
+                if (L.held > est && others + est <= budget_) L.trim = true;
+            }
+            mem_blocked = true;          // later evaluations do not overtake it
+            continue;
+        }
+        fp = f2;
+        atoms += j.st.nat;
+        need_batch = need;
+        batch.push_back(jp);
     }
-    if (!batch.empty()) L.reserved = need_batch;
+    if (!batch.empty()) {
+        L.reserved = need_batch;
+    } else if (md) {
+        Job &j = *md;
+        if (j.kind == Job::Open) {
+            j.sess->bytes = j.est;
+            sessions_bytes_ += j.est;
+        }
+        j.sess->busy = true;
+        one = md;
+    }
     if (failed.empty() && batch.empty() && !one) return false;
     std::vector<const Job *> gone(failed.begin(), failed.end());
     for (auto &jp : batch) gone.push_back(jp.get());

@@ -2,12 +2,13 @@
 // long-lived molecular-dynamics sessions on one GPU.
 //
 // Lanes are worker threads, each with an engine of its own (its own HIP
-// stream and work buffers). A free lane takes the oldest runnable work in the
-// queue: either one request of an MD session, or every queued evaluation that
+// stream and work buffers). A free lane takes every queued evaluation that
 // fits (in arrival order, up to max_batch_atoms atoms and within the device
-// memory budget), evaluated together as one batch. Requests that arrive while
-// the lanes are busy wait for the next free lane; lanes run at the same time
-// on their own streams, so their work overlaps on the device.
+// memory budget) and evaluates them together as one batch; when none waits,
+// it takes the oldest request of an MD session (an MD request that has waited
+// longer than kMdAging goes first). Requests that arrive while the lanes are
+// busy wait for the next free lane; lanes run at the same time on their own
+// streams, so their work overlaps on the device.
 //
 // Admission by device memory: the work buffers each lane holds (or the
 // estimate of the batch it runs, when larger) and the device state of every
@@ -79,6 +80,8 @@ class Server {
     // MD steps
     static constexpr int kStats = 16;
     void stats(double *out, bool reset);
+    // an MD request waiting longer than this goes before evaluations
+    static constexpr std::chrono::milliseconds kMdAging{100};
 
   private:
     using Clock = std::chrono::steady_clock;

@@ -173,7 +173,8 @@ void hdnnp_md_destroy(hdnnp_md *md);
  *    evaluates them as one batch. Lanes are worker threads with their own
  *    stream and buffers, so their work overlaps on the device.
  *  - MD sessions (hdnnp_server_md_*) keep positions, momenta and neighbour
- *    lists (with a Verlet skin) resident between requests.
+ *    lists (with a Verlet skin) resident between requests. Their requests
+ *    run when no evaluation waits, or once they have waited 100 ms.
  *  - Admission is by device memory. The buffers of the lanes and the state
  *    of the sessions stay within hbm_budget_bytes (0: 80% of the free
  *    device memory). Work waits until it fits; work larger than the whole
