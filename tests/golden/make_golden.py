@@ -153,15 +153,27 @@ def make_perovskite_potential():
     wl.write_weights(PEROV_DIR, nsf)
     dummy = [np.tile([0.0, 1.0, 0.0], (nsf, 1)) for _ in wl.ELEMENTS]
     wl.write_scaling(os.path.join(PEROV_DIR, "scaling.data"), dummy)
+    # scaling set: the 6^3 cell and 64 C/N/H clusters of 8-64 atoms (the
+    # structure kinds the potential evaluates), so that no symmetry function
+    # is scaled by the near-zero range it has on the cell alone
     s = wl.perovskite(6, 6, 6)
     r = eval_case(PEROV_DIR, s, keep_sf=True)
+    rows = {z: [r["G"][s.Z == z][:, :nsf]] for z in wl.ELEMENTS}
+    ei = [r["Ei"]]
+    for k in range(64):
+        c = wl.cluster(8 + (k * 7) % 57, seed=5000 + k)
+        rc_ = eval_case(PEROV_DIR, c, keep_sf=True)
+        for z in wl.ELEMENTS:
+            if np.any(c.Z == z):
+                rows[z].append(rc_["G"][c.Z == z][:, :nsf])
+        ei.append(rc_["Ei"])
     stats = []
     for z in wl.ELEMENTS:
-        G = r["G"][s.Z == z][:, :nsf]
+        G = np.concatenate(rows[z])
         mn, mx, me = G.min(0), G.max(0), G.mean(0)
         mx = np.where(mx - mn > 0, mx, mn + 1.0)
         stats.append(np.stack([mn, mx, me], axis=1))
-    e = r["Ei"]
+    e = np.concatenate(ei)
     wl.write_scaling(os.path.join(PEROV_DIR, "scaling.data"), stats,
                      (float(e.min()), float(e.max())))
     wl.write_input_nn(os.path.join(PEROV_DIR, "input.nn"), scale=True)

@@ -78,3 +78,19 @@ def test_device_batch_equals_host_batch(pkg, wl):
     assert np.array_equal(E, Ed)
     assert np.array_equal(np.concatenate(F), Fd.cpu().numpy())
     assert np.array_equal(L, Ld)
+
+
+def test_mixed_batch_force_error_bound(pkg, wl):
+    """Mixed precision over a batch of perovskite cells and C/N/H clusters:
+    force RMSE against FP64 within 1e-5 Ha/bohr for either kind (the bound of
+    the headline metric), and energies within 1e-5 Ha per atom."""
+    ss = wl.batch_mix(400, seed=2024)
+    E64, F64, _ = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64").energy_and_forces_batch(ss)
+    E32, F32, _ = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed").energy_and_forces_batch(ss)
+    for periodic in (True, False):
+        ks = [k for k, s in enumerate(ss) if bool(s.periodic) == periodic]
+        d = np.concatenate([(F32[k] - F64[k]).ravel() for k in ks])
+        rmse = float(np.sqrt(np.mean(d ** 2)))
+        de = max(abs(E32[k] - E64[k]) / ss[k].nat for k in ks)
+        print(f"periodic {periodic}: {len(ks)} structures, F rmse {rmse:.3e}, max dE/nat {de:.3e}", flush=True)
+        assert rmse <= 1e-5 and de <= 1e-5
