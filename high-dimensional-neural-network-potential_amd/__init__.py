@@ -20,6 +20,7 @@ import os
 import numpy as np
 
 from .workloads import Structure  # noqa: F401  (host mirror of atStruct)
+from . import readers  # noqa: F401  (structure file formats of atomicStructure.f90)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libhdnnp_amd.so")

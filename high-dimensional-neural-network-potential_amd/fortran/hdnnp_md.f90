@@ -8,6 +8,12 @@
 !       one hdnnpEnergyAndForces call with dEdLat, plus dEdLat2Stress
 !   hdnnp_md committee <nnps.txt> <structure> <out>
 !       one globalHdnnpEnergyAndForces call (committee mean, energy std)
+!   hdnnp_md read <format> <file> <out>
+!       the structure(s) of a file read by as_readXYZ / as_readFhiaims /
+!       as_readAscii ('ascii', or 'asciiang' with isAngstrom) /
+!       as_readPOSCAR / as_readData ('xyz', 'fhiaims', 'poscar', 'data'),
+!       written as 'structure nat periodic', lattice rows, 'atom Z x y z'
+!       and for data 'energy E Q' and 'aux q fx fy fz' per atom
 !
 ! Structure file: "nat periodic" / three lattice-vector lines / "Z x y z" rows.
 program hdnnp_md
@@ -29,6 +35,10 @@ program hdnnp_md
     call get_command_argument(6, a6)
     call get_command_argument(7, a7)
     prec = HDNNP_FP64
+    if (trim(mode) == 'read') then
+        call runRead(trim(pdir), trim(a3), trim(a4))
+        stop
+    end if
     select case (trim(mode))
     case ('md')
         if (trim(a7) == 'mixed') prec = HDNNP_MIXED
@@ -48,6 +58,57 @@ program hdnnp_md
     call hdnnpDestroy(h)
 
 contains
+
+    subroutine writeStruct(u, ats, aux)
+        integer, intent(in) :: u
+        type(atStruct), intent(in) :: ats
+        logical, intent(in) :: aux
+        integer :: i
+        write(u, '(A,I8,I2)') 'structure ', ats%nat, merge(1, 0, ats%periodic)
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') merge(ats%lat(:, i), 0._dp, ats%periodic)
+        end do
+        do i = 1, ats%nat
+            write(u, '(A,I4,3ES25.16E3)') 'atom ', ats%el(i), ats%ats(:, i)
+        end do
+        if (aux) then
+            write(u, '(A,2ES25.16E3)') 'energy ', ats%energy, ats%qtot
+            do i = 1, ats%nat
+                write(u, '(A,4ES25.16E3)') 'aux ', ats%q(i), ats%f(:, i)
+            end do
+        end if
+    end subroutine writeStruct
+
+    subroutine runRead(fmt, fname, oname)
+        character(len=*), intent(in) :: fmt, fname, oname
+        type(atStruct) :: ats
+        type(atStruct), allocatable :: many(:)
+        integer :: u, n, i
+        open(newunit=u, file=oname, action='write', status='replace')
+        select case (fmt)
+        case ('xyz')
+            call as_readXYZ(fname, ats)
+        case ('fhiaims')
+            call as_readFhiaims(fname, ats)
+        case ('ascii')
+            call as_readAscii(fname, ats)
+        case ('asciiang')
+            call as_readAscii(fname, ats, .true.)
+        case ('poscar')
+            call as_readPOSCAR(fname, ats)
+        case ('data')
+            call as_readData(fname, n, many)
+            do i = 1, n
+                call writeStruct(u, many(i), .true.)
+            end do
+            close(u)
+            return
+        case default
+            error stop 'unknown format'
+        end select
+        call writeStruct(u, ats, .false.)
+        close(u)
+    end subroutine runRead
 
     subroutine runMD(h, xyz, pfile, nstr, fname)
         type(hdnnp), intent(inout) :: h

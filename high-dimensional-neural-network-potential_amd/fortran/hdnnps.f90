@@ -6,8 +6,9 @@
 !   precision        -- dp kind                         (src/precision.f90)
 !   constants        -- units, masses, element symbols  (src/constants/constants.f90)
 !   atomicStructure  -- type atStruct, as_readXYZ, as_writeXYZ,
-!                       as_repeatUnitCell, dEdLat2Stress, stress2dEdLat,
-!                       center                         (src/atoms/atomicStructure.f90)
+!                       as_readFhiaims, as_readAscii, as_readPOSCAR,
+!                       as_readData, as_repeatUnitCell, dEdLat2Stress,
+!                       stress2dEdLat, center          (src/atoms/atomicStructure.f90)
 !   hdnnps           -- type hdnnp, parseRuNNerInput, hdnnpEnergyAndForces
 !                                                       (src/potentials/hdnnps.f90)
 ! The evaluation runs on the GPU through the C ABI of libhdnnp_amd.so
@@ -211,6 +212,267 @@ contains
         end do
         close(u)
     end subroutine as_writeXYZ
+
+    ! file cannot be opened: the reference's message, then stop
+    subroutine openOrStop(filename, u)
+        character(len=*), intent(in) :: filename
+        integer, intent(out) :: u
+        integer :: stat
+        open(newunit=u, file=filename, iostat=stat, status='old', action='read')
+        if (stat /= 0) then
+            print*, 'Error: Could not open file (' // filename // ')'
+            error stop 1
+        end if
+    end subroutine openOrStop
+
+    ! copies of a (rows x cols) and of a vector with room for cap columns / entries
+    subroutine growCols(a, cap)
+        real(dp), allocatable, intent(inout) :: a(:,:)
+        integer, intent(in) :: cap
+        real(dp), allocatable :: t(:,:)
+        allocate(t(size(a, 1), cap))
+        t(:, 1:size(a, 2)) = a
+        call move_alloc(t, a)
+    end subroutine growCols
+
+    subroutine growReal(a, cap)
+        real(dp), allocatable, intent(inout) :: a(:)
+        integer, intent(in) :: cap
+        real(dp), allocatable :: t(:)
+        allocate(t(cap))
+        t(1:size(a)) = a
+        call move_alloc(t, a)
+    end subroutine growReal
+
+    subroutine growInt(a, cap)
+        integer, allocatable, intent(inout) :: a(:)
+        integer, intent(in) :: cap
+        integer, allocatable :: t(:)
+        allocate(t(cap))
+        t(1:size(a)) = a
+        call move_alloc(t, a)
+    end subroutine growInt
+
+    ! room for at least n atoms in positions p(3,:), elements e and, when
+    ! given, charges q and forces f(3,:)
+    subroutine growAtoms(n, p, e, q, f)
+        integer, intent(in) :: n
+        real(dp), allocatable, intent(inout) :: p(:,:)
+        integer, allocatable, intent(inout) :: e(:)
+        real(dp), allocatable, intent(inout), optional :: q(:), f(:,:)
+        integer :: cap
+        if (n <= size(e)) return
+        cap = max(2 * size(e), n, 16)
+        call growCols(p, cap)
+        call growInt(e, cap)
+        if (present(q)) call growReal(q, cap)
+        if (present(f)) call growCols(f, cap)
+    end subroutine growAtoms
+
+    ! as_readFhiaims (atomicStructure.f90:168-219): FHI-aims geometry in
+    ! Angstrom; lines beginning with 'atom' in column 1 and 'lattice_vector'
+    ! lines; periodic when a lattice vector is given
+    subroutine as_readFhiaims(filename, ats)
+        character(len=*), intent(in) :: filename
+        type(atStruct), intent(out) :: ats
+        character(len=200) :: line, tag
+        character(len=2) :: el
+        real(dp) :: lat(3,3), v(3)
+        real(dp), allocatable :: p(:,:)
+        integer, allocatable :: e(:)
+        integer :: u, stat, n, nlat
+        call openOrStop(filename, u)
+        allocate(p(3, 0), e(0))
+        n = 0
+        nlat = 0
+        lat = 0._dp
+        do
+            read(u, '(A)', iostat=stat) line
+            if (stat /= 0) exit
+            if (line(1:4) == 'atom') then
+                read(line, *) tag, v, el
+                n = n + 1
+                call growAtoms(n, p, e)
+                p(:, n) = v
+                e(n) = elemSymToNum(el)
+            else if (line(1:14) == 'lattice_vector') then
+                if (nlat == 3) error stop 'ERROR: more than 3 lattice vectors'
+                nlat = nlat + 1
+                read(line, *) tag, lat(:, nlat)
+            end if
+        end do
+        close(u)
+        ats%nat = n
+        allocate(ats%ats(3, n), ats%el(n))
+        ats%ats = p(:, 1:n) * ang2bohr
+        ats%el = e(1:n)
+        ats%periodic = nlat > 0
+        ats%lat = 0._dp
+        if (ats%periodic) ats%lat = lat * ang2bohr
+    end subroutine as_readFhiaims
+
+    ! as_readAscii (atomicStructure.f90:245-285): v_sim ascii, periodic; the
+    ! lattice is lower triangular (dxx / dyx dyy on line 2, dzx dzy dzz on
+    ! line 3); Angstrom when line 1 says 'angstroem' or isAngstrom is set
+    subroutine as_readAscii(filename, ats, isAngstrom)
+        character(len=*), intent(in) :: filename
+        type(atStruct), intent(out) :: ats
+        logical, intent(in), optional :: isAngstrom
+        character(len=200) :: line, unit
+        character(len=2) :: el
+        integer :: u, stat, i
+        logical :: angs
+        call openOrStop(filename, u)
+        read(u, '(A)') line
+        unit = ''
+        read(line, *, iostat=stat) ats%nat, unit
+        if (stat /= 0) read(line, *) ats%nat
+        angs = unit(1:9) == 'angstroem'
+        if (present(isAngstrom)) angs = angs .or. isAngstrom
+        ats%periodic = .true.
+        allocate(ats%ats(3, ats%nat), ats%el(ats%nat))
+        ats%lat = 0._dp
+        read(u, *) ats%lat(1:1, 1), ats%lat(1:2, 2)
+        read(u, *) ats%lat(1:3, 3)
+        do i = 1, ats%nat
+            read(u, *) ats%ats(:, i), el
+            ats%el(i) = elemSymToNum(el)
+        end do
+        close(u)
+        if (angs) then
+            ats%ats = ats%ats * ang2bohr
+            ats%lat = ats%lat * ang2bohr
+        end if
+    end subroutine as_readAscii
+
+    ! as_readPOSCAR (atomicStructure.f90:427-475): VASP 5 with element names
+    ! and Direct coordinates; the scale factor is read but not applied, as in
+    ! the reference
+    subroutine as_readPOSCAR(fname, ats)
+        character(len=*), intent(in) :: fname
+        type(atStruct), intent(out) :: ats
+        character(len=256) :: line
+        character(len=2), allocatable :: syms(:)
+        integer, allocatable :: cnt(:)
+        real(dp) :: scale, s(3)
+        integer :: u, i, j, c, n, a
+        call openOrStop(fname, u)
+        read(u, *)
+        read(u, *) scale
+        do i = 1, 3
+            read(u, *) ats%lat(:, i)
+        end do
+        ats%lat = ats%lat * ang2bohr
+        ats%periodic = .true.
+        read(u, '(A)') line
+        n = 0                                  ! blank-separated words
+        do i = 1, len_trim(line)
+            if (line(i:i) /= ' ' .and. (i == 1 .or. line(max(i - 1, 1):max(i - 1, 1)) == ' ')) n = n + 1
+        end do
+        allocate(syms(n), cnt(n))
+        read(line, *) syms
+        read(u, *) cnt
+        read(u, '(A)') line
+        do i = 1, len(line)
+            c = ichar(line(i:i))
+            if (c >= 65 .and. c <= 90) line(i:i) = char(c + 32)
+        end do
+        if (trim(adjustl(line)) /= 'direct') then
+            print*, line
+            error stop 'ERROR: POSCAR without DIRECT not implemented'
+        end if
+        ats%nat = sum(cnt)
+        allocate(ats%el(ats%nat), ats%ats(3, ats%nat))
+        c = 0
+        do i = 1, n
+            do j = 1, cnt(i)
+                c = c + 1
+                ats%el(c) = elemSymToNum(syms(i))
+            end do
+        end do
+        do i = 1, ats%nat                      ! lat . s (matMulVec3, linalg.f90:192-201)
+            read(u, *) s
+            do a = 1, 3
+                ats%ats(a, i) = (ats%lat(a, 1) * s(1) + ats%lat(a, 2) * s(2)) + ats%lat(a, 3) * s(3)
+            end do
+        end do
+        close(u)
+    end subroutine as_readPOSCAR
+
+    ! as_readData (atomicStructure.f90:354-425): RuNNer input.data blocks
+    ! (begin / lattice / atom x y z el q e fx fy fz / energy / charge / end),
+    ! bohr and Hartree; periodic when exactly three lattice lines are given
+    subroutine as_readData(fname, n, ats)
+        character(len=*), intent(in) :: fname
+        integer, intent(out) :: n
+        type(atStruct), allocatable, intent(out) :: ats(:)
+        type(atStruct), allocatable :: tmp(:)
+        character(len=512) :: line
+        character(len=64) :: tag
+        character(len=2) :: el
+        real(dp) :: lat(3,3), v(3), f(3), q, e, energy, qtot
+        real(dp), allocatable :: p(:,:), fs(:,:), qs(:)
+        integer, allocatable :: es(:)
+        integer :: u, stat, na, nlat, k
+        logical :: inside
+        call openOrStop(fname, u)
+        allocate(ats(0), p(3, 0), es(0), qs(0), fs(3, 0))
+        n = 0
+        inside = .false.
+        do
+            read(u, '(A)', iostat=stat) line
+            if (stat /= 0) exit
+            line = adjustl(line)
+            if (line(1:5) == 'begin') then
+                inside = .true.
+                na = 0
+                nlat = 0
+                lat = 0._dp
+                energy = 0._dp
+                qtot = 0._dp
+            end if
+            if (.not. inside) cycle
+            if (line(1:7) == 'lattice') then
+                if (nlat == 3) error stop 'ERROR: more than 3 lattice lines in a structure'
+                nlat = nlat + 1
+                read(line, *) tag, lat(:, nlat)
+            end if
+            if (line(1:4) == 'atom') then
+                read(line, *) tag, v, el, q, e, f
+                na = na + 1
+                call growAtoms(na, p, es, qs, fs)
+                p(:, na) = v
+                es(na) = elemSymToNum(el)
+                qs(na) = q
+                fs(:, na) = f
+            end if
+            if (line(1:6) == 'energy') read(line, *) tag, energy
+            if (line(1:6) == 'charge') read(line, *) tag, qtot
+            if (line(1:3) == 'end') then
+                allocate(tmp(n + 1))
+                do k = 1, n
+                    tmp(k) = ats(k)
+                end do
+                associate (s => tmp(n + 1))
+                    s%nat = na
+                    s%periodic = nlat == 3
+                    s%lat = 0._dp
+                    if (s%periodic) s%lat = lat
+                    allocate(s%ats(3, na), s%el(na), s%q(na), s%f(3, na))
+                    s%ats = p(:, 1:na)
+                    s%el = es(1:na)
+                    s%q = qs(1:na)
+                    s%f = fs(:, 1:na)
+                    s%energy = energy
+                    s%qtot = qtot
+                end associate
+                call move_alloc(tmp, ats)
+                n = n + 1
+                inside = .false.
+            end if
+        end do
+        close(u)
+    end subroutine as_readData
 
     ! as_repeatUnitCell (atomicStructure.f90:636-671): x, y, z loops, atoms innermost
     subroutine as_repeatUnitCell(ats, n)

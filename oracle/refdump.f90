@@ -22,6 +22,11 @@
 !       velocity-Verlet loop of hdnnpExample.f90:58-72 with given momenta.
 !   refdump time  <paramdir/> <structure> <nrep> <out>
 !       wall time of hdnnpEnergyAndForces (E+F+dE/dh), warm-up 1, nrep timed.
+!   refdump read  <format> <file> <out>
+!       the structure(s) one of the reference's readers makes of a file:
+!       format xyz (as_readXYZ), fhiaims (as_readFhiaims), ascii / asciiang
+!       (as_readAscii without / with isAngstrom), poscar (as_readPOSCAR),
+!       data (as_readData, atomicStructure.f90:143-475).
 !
 ! Structure file (text): "nat periodic" / three lines, line k = lattice
 ! vector k / nat lines "Z x y z" (bohr).
@@ -42,6 +47,12 @@ program refdump
 
     call get_command_argument(1, mode)
     call get_command_argument(2, pdir)
+    if (trim(mode) == 'read') then
+        call get_command_argument(3, sfile)
+        call get_command_argument(4, ofile)
+        call dumpRead(trim(pdir), trim(sfile), trim(ofile))
+        stop
+    end if
     call parseRuNNerInput(h, trim(pdir))
 
     select case (trim(mode))
@@ -86,6 +97,64 @@ contains
         close(u)
         if (.not. ats%periodic) ats%lat = 0._dp
     end subroutine readStruct
+
+    ! one structure: 'structure nat periodic', the lattice (line k = vector
+    ! k), 'atom Z x y z' lines; with aux, then 'energy E Q' and per atom
+    ! 'aux q fx fy fz'
+    subroutine writeStruct(u, ats, aux)
+        integer, intent(in) :: u
+        type(atStruct), intent(in) :: ats
+        logical, intent(in) :: aux
+        integer :: i
+        write(u, '(A,I8,I2)') 'structure ', ats%nat, merge(1, 0, ats%periodic)
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') merge(ats%lat(:, i), 0._dp, ats%periodic)
+        end do
+        do i = 1, ats%nat
+            write(u, '(A,I4,3ES25.16E3)') 'atom ', ats%el(i), ats%ats(:, i)
+        end do
+        if (aux) then
+            write(u, '(A,2ES25.16E3)') 'energy ', ats%energy, ats%qtot
+            do i = 1, ats%nat
+                write(u, '(A,4ES25.16E3)') 'aux ', ats%q(i), ats%f(:, i)
+            end do
+        end if
+    end subroutine writeStruct
+
+    subroutine dumpRead(fmt, fname, oname)
+        character(len=*), intent(in) :: fmt, fname, oname
+        type(atStruct) :: ats
+        type(atStruct), allocatable :: many(:)
+        integer :: u, n, i
+
+        open(newunit=u, file=oname, action='write', status='replace')
+        select case (fmt)
+        case ('xyz')
+            call as_readXYZ(fname, ats)
+            call writeStruct(u, ats, .false.)
+        case ('fhiaims')
+            call as_readFhiaims(fname, ats)
+            call writeStruct(u, ats, .false.)
+        case ('ascii')
+            call as_readAscii(fname, ats)
+            call writeStruct(u, ats, .false.)
+        case ('asciiang')
+            call as_readAscii(fname, ats, .true.)
+            call writeStruct(u, ats, .false.)
+        case ('poscar')
+            call as_readPOSCAR(fname, ats)
+            call writeStruct(u, ats, .false.)
+        case ('data')
+            call as_readData(fname, n, many)
+            do i = 1, n
+                call writeStruct(u, many(i), .true.)
+            end do
+        case default
+            print*, 'unknown format'
+            stop 2
+        end select
+        close(u)
+    end subroutine dumpRead
 
     subroutine dumpTable(h, fname)
         type(hdnnp), intent(in) :: h

@@ -9,6 +9,7 @@ files are copied as data fixtures into tests/golden/methylammonium/.
 
     python tests/golden/make_golden.py
     python tests/golden/make_golden.py md-potential   # only data/perovskite_md
+    python tests/golden/make_golden.py readers        # only tests/golden/readers.npz
 """
 from __future__ import annotations
 
@@ -195,6 +196,57 @@ def make_md_potential():
     print(f"perovskite_md: output scale {0.02 / fmax:.6g}, max force {f2:.4g} Ha/bohr")
 
 
+READER_CASES = (("fhiaims", "geometry.in"), ("fhiaims", "molecule.in"), ("ascii", "cell_ang.ascii"),
+                ("ascii", "cell_bohr.ascii"), ("asciiang", "cell_bohr.ascii"), ("poscar", "POSCAR"),
+                ("data", "input.data"), ("data", "big.data"))
+
+
+def parse_read(path):
+    """Structures written by `refdump read`."""
+    out, cur = [], None
+    for ln in open(path).read().splitlines():
+        p = ln.split()
+        if not p:
+            continue
+        if p[0] == "structure":
+            cur = {"nat": int(p[1]), "periodic": int(p[2]), "lat": [], "Z": [], "pos": [], "aux": []}
+            out.append(cur)
+        elif p[0] == "atom":
+            cur["Z"].append(int(p[1]))
+            cur["pos"].append([float(t) for t in p[2:5]])
+        elif p[0] == "energy":
+            cur["energy"], cur["qtot"] = float(p[1]), float(p[2])
+        elif p[0] == "aux":
+            cur["aux"].append([float(t) for t in p[1:5]])
+        else:
+            cur["lat"].append([float(t) for t in p[:3]])
+    return out
+
+
+def make_reader_goldens():
+    """tests/golden/readers.npz: what the reference's structure readers make
+    of the sample files in tests/golden/readers/ (key '<format>:<file>:<k>:<field>')."""
+    d = {}
+    with tempfile.TemporaryDirectory() as td:
+        for fmt, fn in READER_CASES:
+            of = os.path.join(td, "r.txt")
+            subprocess.run([REFDUMP, "read", fmt, os.path.join(GOLD, "readers", fn), of], check=True,
+                           stdout=subprocess.DEVNULL)
+            for k, st in enumerate(parse_read(of)):
+                key = f"{fmt}:{fn}:{k}:"
+                d[key + "periodic"] = np.array(st["periodic"])
+                d[key + "lat"] = np.array(st["lat"])
+                d[key + "Z"] = np.array(st["Z"], dtype=np.int32)
+                d[key + "pos"] = np.array(st["pos"]).reshape(-1, 3)
+                if st["aux"]:
+                    a = np.array(st["aux"])
+                    d[key + "energy"] = np.array(st["energy"])
+                    d[key + "qtot"] = np.array(st["qtot"])
+                    d[key + "charges"] = a[:, 0]
+                    d[key + "forces"] = a[:, 1:4]
+    np.savez_compressed(os.path.join(GOLD, "readers.npz"), **d)
+
+
 def md_momenta(s, seed=99, T=140.0):
     rng = np.random.default_rng(seed)
     m = np.array([wl.MASS[int(z)] for z in s.Z]) * wl.MASS_UNIT_TO_ELECTRON
@@ -209,6 +261,7 @@ def main():
     shutil.copy(os.path.join(REF, "example", "input.xyz"), os.path.join(GOLD, "methylammonium.xyz"))
     make_perovskite_potential()
     make_md_potential()
+    make_reader_goldens()
 
     with tempfile.TemporaryDirectory() as td:
         for name, pdir in (("table_methylammonium", MA_DIR), ("table_perovskite", PEROV_DIR)):
@@ -282,5 +335,7 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["md-potential"]:      # only data/perovskite_md
         make_md_potential()
+    elif sys.argv[1:] == ["readers"]:         # only tests/golden/readers.npz
+        make_reader_goldens()
     else:
         main()

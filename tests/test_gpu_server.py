@@ -151,3 +151,21 @@ def test_server_committee(pkg, wl, tmp_path):
         E, F, D, sd, _ = sv.result(t, timeout=120)
         assert E == E0 and np.array_equal(F, F0) and np.array_equal(D, D0) and sd == sd0 and sd > 0
     sv.close()
+
+
+def test_server_ingests_structure_files(pkg):
+    """Structures read from the reference's file formats (input.data,
+    POSCAR, FHI-aims, v_sim ascii) are evaluated by the server as by a
+    context of their own."""
+    import os
+    from conftest import GOLD
+    rd, d = pkg.readers, os.path.join(GOLD, "readers")
+    ss = (rd.read_structures(os.path.join(d, "input.data"), "data") + [rd.read_poscar(os.path.join(d, "POSCAR"))]
+          + [rd.read_fhiaims(os.path.join(d, "geometry.in")), rd.read_ascii(os.path.join(d, "cell_ang.ascii"))])
+    want = _alone(pkg, "fp64", ss)
+    sv = pkg.Server(PEROV_PARAMS, "fp64", lanes=2)
+    ts = [sv.submit(s) for s in ss]
+    for k, t in enumerate(ts):
+        E, F, D, _, _ = sv.result(t, timeout=120)
+        assert E == want[k][0] and np.array_equal(F, want[k][1]) and np.array_equal(D, want[k][2])
+    sv.close()
