@@ -32,7 +32,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device",
            "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee",
-           "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_destroy", "hdnnp_neighbor_count",
+           "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy",
@@ -86,6 +86,7 @@ def load_library():
         lib.hdnnp_md_step.argtypes = [vp, ctypes.c_int32, _dp, _dp]
         lib.hdnnp_md_get.argtypes = [vp, _dp, _dp, _dp]
         lib.hdnnp_md_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        lib.hdnnp_md_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_md_destroy.argtypes = [vp]
         lib.hdnnp_md_destroy.restype = None
         lib.hdnnp_create_committee.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
@@ -366,6 +367,13 @@ class MdSystems:
         st, nb = ctypes.c_int64(), ctypes.c_int64()
         self.h._check(self.lib.hdnnp_md_stats(self.md, ctypes.byref(st), ctypes.byref(nb)))
         return st.value, nb.value
+
+    @property
+    def counters(self) -> dict:
+        """steps, list builds, chunks cut back, graph captures, graph replays"""
+        out = (ctypes.c_int64 * 5)()
+        self.h._check(self.lib.hdnnp_md_counters(self.md, out))
+        return dict(zip(("steps", "builds", "rollbacks", "graph_captures", "graph_replays"), list(out)))
 
     def close(self):
         if getattr(self, "md", None) and self.md.value:

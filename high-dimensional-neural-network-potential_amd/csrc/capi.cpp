@@ -154,6 +154,17 @@ int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds)
     return HDNNP_OK;
 }
 
+int hdnnp_md_counters(const hdnnp_md *md, int64_t *out)
+{
+    if (!md || !out) return HDNNP_ERR_ARG;
+    out[0] = md->sys->steps();
+    out[1] = md->sys->builds();
+    out[2] = md->sys->rollbacks();
+    out[3] = md->sys->graph_captures();
+    out[4] = md->sys->graph_replays();
+    return HDNNP_OK;
+}
+
 void hdnnp_md_destroy(hdnnp_md *md)
 {
     if (!md) return;

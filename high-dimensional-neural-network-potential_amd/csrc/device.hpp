@@ -178,13 +178,16 @@ void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int n
                    hipStream_t s);
 
 // ---- kernels_md.hip: velocity Verlet and Verlet-skin bookkeeping ----
-void launch_md_drift(double *pos, const double *mom, const double *F, const double *mass, double dt, int nat,
-                     hipStream_t s);
+void launch_md_drift(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,
+                     int nat, hipStream_t s);
 void launch_md_kick(double *mom, const double *F, const double *Fold, double dt, int nat, hipStream_t s);
-void launch_md_disp(const double *pos, const double *posb, int nat, unsigned long long *maxd2, hipStream_t s);
+void launch_md_disp(const double *pos, const double *posb, int nat, unsigned long long *maxd2, const int *step,
+                    hipStream_t s);
 void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, const double *posb, int nat,
                        hipStream_t s);
-void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, hipStream_t s);
+void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, const int *step,
+                    hipStream_t s);
+void launch_md_tick(int *step, hipStream_t s);
 void launch_md_load(const double *slot, double *pos, double *mom, double *F, int nat, hipStream_t s);
 void launch_md_ekin(const double *mom, const double *mass, const int *off, int nsys, double *ekin, hipStream_t s);
 

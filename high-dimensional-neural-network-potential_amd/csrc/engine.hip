@@ -394,7 +394,7 @@ static constexpr size_t kPosStartBytes = size_t(1) << 30;
 static constexpr int kReduceChunk = 512;
 
 template <typename R>
-void Engine::compute(Work &w, const Geo &g, hipStream_t s)
+void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
 {
     const int nat = w.nat;
     G_.ensure(sizeof(R) * (size_t)nat * dm_.nsfmax);
@@ -404,7 +404,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s)
 
     launch_sf_values<R>(dm_, g, w, w.cap, G, s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[3], s));
+    if (timed) HIPCHK(hipEventRecord(ev_[3], s));
 
     // one block per 16-atom tile of one element; the element counts stay on
     // the device, so the grid covers the most tiles any split can need and
@@ -418,11 +418,11 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s)
                         R(1) / R(nm), k > 0, s);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[4], s));
+    if (timed) HIPCHK(hipEventRecord(ev_[4], s));
 
     launch_sf_grad<R>(dm_, g, w, w.cap, dEdG, gent, s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[5], s));
+    if (timed) HIPCHK(hipEventRecord(ev_[5], s));
 
     launch_force_gather_t<R>(w, gent, s);
     HIPCHK(hipGetLastError());
@@ -743,7 +743,23 @@ bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dE
     return finish_collect(s, energy, dEdLat);
 }
 
-void Engine::finish_launch(double *force_dev, hipStream_t s)
+std::vector<char> Engine::launch_key() const
+{
+    const void *ptrs[] = {G_.p, dEdG_.p, gent_.p, chunks_.p, sc_.p, partial_.p, res_.p, misc_.p};
+    const int ints[] = {prep_.nat, prep_.nchunk, (int)prep_.st.size(), precision_, prep_.valid ? 1 : 0};
+    std::vector<char> k(sizeof(Work) + sizeof(Geo) + sizeof ptrs + sizeof ints);
+    char *p = k.data();
+    std::memcpy(p, &prep_.w, sizeof(Work));
+    p += sizeof(Work);
+    std::memcpy(p, &prep_.g, sizeof(Geo));
+    p += sizeof(Geo);
+    std::memcpy(p, ptrs, sizeof ptrs);
+    p += sizeof ptrs;
+    std::memcpy(p, ints, sizeof ints);
+    return k;
+}
+
+void Engine::finish_launch(double *force_dev, hipStream_t s, bool timed)
 {
     if (prep_.nat == 0) return;
     if (!prep_.valid) throw Error(5, "no prepared evaluation");
@@ -755,9 +771,9 @@ void Engine::finish_launch(double *force_dev, hipStream_t s)
     const int nat = prep_.nat, nchunk = prep_.nchunk, nm = (int)dmm_.size();
     const int nstruct = (int)prep_.st.size();
     if (fp64)
-        compute<double>(w, g, s);
+        compute<double>(w, g, s, timed);
     else
-        compute<float>(w, g, s);
+        compute<float>(w, g, s, timed);
     for (int k = 0; k < nm; ++k) {       // per member: sum of its E_i (and the common virial)
         Work wk = w;
         wk.Ei = w.Ei + (size_t)k * nat;
@@ -765,7 +781,7 @@ void Engine::finish_launch(double *force_dev, hipStream_t s)
         launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), s);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[6], s));
+    if (timed) HIPCHK(hipEventRecord(ev_[6], s));
 }
 
 bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)

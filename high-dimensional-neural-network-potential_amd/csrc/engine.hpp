@@ -86,7 +86,12 @@ class Engine {
     bool finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat);
     // finish in two parts: the kernels (queued on s, nothing waited for),
     // then the read-back, checks and sums (waits for s)
-    void finish_launch(double *force_dev, hipStream_t s);
+    // timed = false: no timing events (the form a captured graph can hold)
+    void finish_launch(double *force_dev, hipStream_t s, bool timed = true);
+    // Everything finish_launch's kernels take as arguments (launch
+    // parameters and buffer addresses): a graph captured from finish_launch
+    // is valid while this stays the same.
+    std::vector<char> launch_key() const;
     bool finish_collect(hipStream_t s, double *energy, double *dEdLat);
     double4 *wpos() { return wpos_.as<double4>(); }
     // bounding boxes of the nonperiodic structures (for build), from device positions
@@ -116,7 +121,7 @@ class Engine {
     void *staging(size_t bytes);
     std::vector<DevMem *> work_buffers();
     template <typename R>
-    void compute(Work &w, const Geo &g, hipStream_t s);
+    void compute(Work &w, const Geo &g, hipStream_t s, bool timed);
 
     HostModel model_;
     std::vector<HostModel> members_;          // committee members 2..N

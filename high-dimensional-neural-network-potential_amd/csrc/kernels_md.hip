@@ -11,13 +11,16 @@
 
 namespace hdnnp {
 
-__global__ __launch_bounds__(256) void k_md_drift(double *pos, const double *mom, const double *F, const double *mass,
-                                                  double dt, int n3)
+// drift, keeping the forces of the step before in Fold for the kick
+__global__ __launch_bounds__(256) void k_md_drift(double *pos, const double *mom, const double *F, double *Fold,
+                                                  const double *mass, double dt, int n3)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n3) return;
     const double m = mass[x / 3];
-    pos[x] = (pos[x] + mom[x] / m * dt) + 0.5 * F[x] / m * (dt * dt);
+    const double f = F[x];
+    pos[x] = (pos[x] + mom[x] / m * dt) + 0.5 * f / m * (dt * dt);
+    Fold[x] = f;
 }
 
 __global__ __launch_bounds__(256) void k_md_kick(double *mom, const double *F, const double *Fold, double dt, int n3)
@@ -27,10 +30,11 @@ __global__ __launch_bounds__(256) void k_md_kick(double *mom, const double *F, c
     mom[x] = mom[x] + dt / 2.0 * (F[x] + Fold[x]);
 }
 
-// largest squared displacement since the last list build; maxd2 holds the
-// bits of a non-negative double, which order like unsigned integers
+// largest squared displacement since the last list build, into maxd2[*step]
+// (maxd2[0] without a step counter); maxd2 holds the bits of non-negative
+// doubles, which order like unsigned integers
 __global__ __launch_bounds__(256) void k_md_disp(const double *pos, const double *posb, int nat,
-                                                 unsigned long long *maxd2)
+                                                 unsigned long long *maxd2, const int *step)
 {
     __shared__ double red[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,7 +50,7 @@ __global__ __launch_bounds__(256) void k_md_disp(const double *pos, const double
         if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMax(maxd2, (unsigned long long)__double_as_longlong(red[0]));
+    if (threadIdx.x == 0) atomicMax(maxd2 + (step ? *step : 0), (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // wrapped positions for a reused list: those of the build plus the
@@ -61,12 +65,14 @@ __global__ __launch_bounds__(256) void k_md_refresh(double4 *wpos, const double4
                            b.z + (pos[3 * i + 2] - posb[3 * i + 2]), 0.0);
 }
 
-// state of a step (positions, momenta, forces) to or from one saved slot
+// state of a step (positions, momenta, forces) to or from one saved slot;
+// with a step counter the slot after step *step (slots of 3 n3 doubles)
 __global__ __launch_bounds__(256) void k_md_save(double *slot, const double *pos, const double *mom, const double *F,
-                                                 int n3)
+                                                 int n3, const int *step)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n3) return;
+    if (step) slot += (size_t)(*step + 1) * 3 * n3;
     slot[x] = pos[x];
     slot[n3 + x] = mom[x];
     slot[2 * n3 + x] = F[x];
@@ -79,6 +85,12 @@ __global__ __launch_bounds__(256) void k_md_load(const double *slot, double *pos
     pos[x] = slot[x];
     mom[x] = slot[n3 + x];
     F[x] = slot[2 * n3 + x];
+}
+
+// the step counter of a chunk, advanced at the end of each step
+__global__ void k_md_tick(int *step)
+{
+    if (threadIdx.x == 0) *step += 1;
 }
 
 // kinetic energy of each system: one block per system, fixed-order tree
@@ -102,10 +114,11 @@ __global__ __launch_bounds__(256) void k_md_ekin(const double *mom, const double
     if (threadIdx.x == 0) ekin[s] = red[0] / 2.0;
 }
 
-void launch_md_drift(double *pos, const double *mom, const double *F, const double *mass, double dt, int nat,
-                     hipStream_t s)
+void launch_md_drift(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,
+                     int nat, hipStream_t s)
 {
-    if (nat > 0) hipLaunchKernelGGL(k_md_drift, dim3((3 * nat + 255) / 256), dim3(256), 0, s, pos, mom, F, mass, dt, 3 * nat);
+    if (nat > 0)
+        hipLaunchKernelGGL(k_md_drift, dim3((3 * nat + 255) / 256), dim3(256), 0, s, pos, mom, F, Fold, mass, dt, 3 * nat);
 }
 
 void launch_md_kick(double *mom, const double *F, const double *Fold, double dt, int nat, hipStream_t s)
@@ -113,9 +126,10 @@ void launch_md_kick(double *mom, const double *F, const double *Fold, double dt,
     if (nat > 0) hipLaunchKernelGGL(k_md_kick, dim3((3 * nat + 255) / 256), dim3(256), 0, s, mom, F, Fold, dt, 3 * nat);
 }
 
-void launch_md_disp(const double *pos, const double *posb, int nat, unsigned long long *maxd2, hipStream_t s)
+void launch_md_disp(const double *pos, const double *posb, int nat, unsigned long long *maxd2, const int *step,
+                    hipStream_t s)
 {
-    if (nat > 0) hipLaunchKernelGGL(k_md_disp, dim3((nat + 255) / 256), dim3(256), 0, s, pos, posb, nat, maxd2);
+    if (nat > 0) hipLaunchKernelGGL(k_md_disp, dim3((nat + 255) / 256), dim3(256), 0, s, pos, posb, nat, maxd2, step);
 }
 
 void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, const double *posb, int nat,
@@ -124,10 +138,14 @@ void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, c
     if (nat > 0) hipLaunchKernelGGL(k_md_refresh, dim3((nat + 255) / 256), dim3(256), 0, s, wpos, wposb, pos, posb, nat);
 }
 
-void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, hipStream_t s)
+void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, const int *step,
+                    hipStream_t s)
 {
-    if (nat > 0) hipLaunchKernelGGL(k_md_save, dim3((3 * nat + 255) / 256), dim3(256), 0, s, slot, pos, mom, F, 3 * nat);
+    if (nat > 0)
+        hipLaunchKernelGGL(k_md_save, dim3((3 * nat + 255) / 256), dim3(256), 0, s, slot, pos, mom, F, 3 * nat, step);
 }
+
+void launch_md_tick(int *step, hipStream_t s) { hipLaunchKernelGGL(k_md_tick, dim3(1), dim3(64), 0, s, step); }
 
 void launch_md_load(const double *slot, double *pos, double *mom, double *F, int nat, hipStream_t s)
 {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace hdnnp {
@@ -55,6 +56,8 @@ MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const 
     maxk_ = (int)std::min<size_t>(kMaxChunk, q > 2 ? q - 1 : 1);
     snap_.ensure(72 * n * (size_t)(maxk_ + 1));
     d2_.ensure(8 * kMaxChunk);
+    ctr_.ensure(256);
+    if (const char *g = std::getenv("HDNNP_MD_GRAPHS")) graphs_ = std::strcmp(g, "0") != 0;
     MDCHK(hipHostMalloc((void **)&d2_host_, 8 * kMaxChunk, hipHostMallocDefault));
     MDCHK(hipMemcpy(Z_.p, Z, 4 * (size_t)nat_, hipMemcpyHostToDevice));
     MDCHK(hipMemcpy(pos_.p, pos, 24 * (size_t)nat_, hipMemcpyHostToDevice));
@@ -71,6 +74,9 @@ MdSystems::~MdSystems()
     for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_, &snap_, &d2_})
         d->release();
     if (d2_host_) (void)hipHostFree(d2_host_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    ctr_.release();
 }
 
 size_t MdSystems::device_bytes() const
@@ -90,7 +96,7 @@ void MdSystems::forces(double *epot)
     bool rebuild = !built_ || skin_ == 0.0 || eng_.build_generation() != gen_;
     if (!rebuild) {
         MDCHK(hipMemsetAsync(maxd2_.p, 0, 8, s));
-        launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, maxd2_.as<unsigned long long>(), s);
+        launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, maxd2_.as<unsigned long long>(), nullptr, s);
         unsigned long long bits = 0;
         MDCHK(hipMemcpyAsync(&bits, maxd2_.p, 8, hipMemcpyDeviceToHost, s));
         MDCHK(hipStreamSynchronize(s));
@@ -124,8 +130,8 @@ void MdSystems::forces(double *epot)
 void MdSystems::careful_step()
 {
     hipStream_t s = eng_.stream();
-    launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
-    std::swap(F_, Fold_);
+    launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), mass_.as<double>(), dt_,
+                    nat_, s);
     forces(nullptr);
     launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
     MDCHK(hipGetLastError());
@@ -133,23 +139,72 @@ void MdSystems::careful_step()
     ++since_;
 }
 
+void MdSystems::step_launch(bool timed)
+{
+    hipStream_t s = eng_.stream();
+    const int *ctr = ctr_.as<int>();
+    launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), mass_.as<double>(), dt_,
+                    nat_, s);
+    launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, d2_.as<unsigned long long>(), ctr, s);
+    launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
+    eng_.finish_launch(F_.as<double>(), s, timed);
+    launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
+    launch_md_save(snap_.as<double>(), pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, ctr, s);
+    launch_md_tick(ctr_.as<int>(), s);
+}
+
+bool MdSystems::replay(int n)
+{
+    if (!graphs_) return false;
+    hipStream_t s = eng_.stream();
+    std::vector<char> key = eng_.launch_key();
+    if (!exec_ || key != key_) {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            graphs_ = false;
+            return false;
+        }
+        bool ok = true;
+        try {
+            step_launch(false);
+        } catch (const std::exception &) {
+            ok = false;
+        }
+        hipGraph_t g = nullptr;
+        ok = hipStreamEndCapture(s, &g) == hipSuccess && ok && g;
+        if (ok) ok = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) == hipSuccess;
+        graph_ = g;
+        if (!ok) {   // launch directly from now on
+            if (exec_) (void)hipGraphExecDestroy(exec_);
+            if (graph_) (void)hipGraphDestroy(graph_);
+            exec_ = nullptr;
+            graph_ = nullptr;
+            graphs_ = false;
+            (void)hipGetLastError();
+            return false;
+        }
+        key_ = std::move(key);
+        ++captures_;
+    }
+    for (int i = 0; i < n; ++i) MDCHK(hipGraphLaunch(exec_, s));
+    replays_ += n;
+    return true;
+}
+
 int MdSystems::chunk(int k)
 {
     hipStream_t s = eng_.stream();
     const size_t slot = 9 * (size_t)nat_;    // doubles per saved state
     double *sp = snap_.as<double>();
-    launch_md_save(sp, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, s);
+    launch_md_save(sp, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, nullptr, s);
     MDCHK(hipMemsetAsync(d2_.p, 0, 8 * (size_t)k, s));
-    unsigned long long *d2 = d2_.as<unsigned long long>();
-    for (int i = 0; i < k; ++i) {
-        launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), mass_.as<double>(), dt_, nat_, s);
-        std::swap(F_, Fold_);
-        launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, d2 + i, s);
-        launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
-        eng_.finish_launch(F_.as<double>(), s);
-        launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
-        if (i + 1 < k) launch_md_save(sp + (i + 1) * slot, pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, s);
-    }
+    MDCHK(hipMemsetAsync(ctr_.p, 0, 4, s));
+    step_launch(true);                                   // sizes every buffer the step uses
+    if (k > 1 && !replay(k - 1))
+        for (int i = 1; i < k; ++i) step_launch(true);
     MDCHK(hipGetLastError());
     MDCHK(hipMemcpyAsync(d2_host_, d2_.p, 8 * (size_t)k, hipMemcpyDeviceToHost, s));
     const bool ok = eng_.finish_collect(s, epot_.data(), nullptr);   // waits for the stream

@@ -14,6 +14,13 @@
 // one step at a time, checked before each force evaluation, up to the list
 // build. The trajectory is the one of checking every step. Chunks are sized
 // from the steps between the last two builds.
+//
+// Within a chunk every step launches the same kernels on the same buffers
+// (a step counter on the device selects the displacement and state slots),
+// so after the chunk's first step the rest are replays of one captured HIP
+// graph of a step. The graph is captured again when the engine's launch
+// arguments change (a list build that moved or resized buffers). Setting
+// HDNNP_MD_GRAPHS=0 in the environment launches every step directly.
 #pragma once
 
 #include <vector>
@@ -43,12 +50,16 @@ class MdSystems {
     int nat() const { return nat_; }
 
     long long rollbacks() const { return rollbacks_; }
+    long long graph_captures() const { return captures_; }
+    long long graph_replays() const { return replays_; }
 
   private:
     static constexpr int kMaxChunk = 64;
     void forces(double *epot);
     void careful_step();          // one step, the skin checked before its forces
     int chunk(int k);             // up to k steps on reused lists; the steps kept
+    void step_launch(bool timed); // one chunk step's kernels (the captured sequence)
+    bool replay(int n);           // n more chunk steps by the captured graph; false: not available
 
     Engine &eng_;
     std::vector<StructIn> st_;
@@ -60,6 +71,12 @@ class MdSystems {
     int since_ = 0, interval_ = 0;  // steps since the last build, and between the two before
     bool careful_ = true;          // step one at a time until the next list build
     long long rollbacks_ = 0;
+    DevMem ctr_;                   // step counter of the running chunk
+    bool graphs_ = true;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    std::vector<char> key_;        // engine launch key the graph was captured with
+    long long captures_ = 0, replays_ = 0;
     std::vector<double> epot_, ekin_h_;
     bool built_ = false;
     unsigned long long gen_ = 0;   // engine build generation of this system's lists

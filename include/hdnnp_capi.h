@@ -162,6 +162,13 @@ int hdnnp_md_create(hdnnp_ctx *ctx, int32_t nsys, const int32_t *nat, const int3
 int hdnnp_md_step(hdnnp_md *md, int32_t nsteps, double *epot, double *ekin);
 int hdnnp_md_get(hdnnp_md *md, double *pos, double *mom, double *force);
 int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds);
+/* out[5]: steps, list builds, chunks cut back at a needed build, captures
+ * of the reused-list step graph, replays of it. Steps on reused lists run in
+ * chunks checked once at their end; all but a chunk's first step are
+ * replays of one captured HIP graph of a step (HDNNP_MD_GRAPHS=0 in the
+ * environment launches them directly). The trajectory is the same either
+ * way, bit for bit. No counterpart in the reference. */
+int hdnnp_md_counters(const hdnnp_md *md, int64_t *out);
 void hdnnp_md_destroy(hdnnp_md *md);
 
 /* Request server. The reference has no counterpart: its potential is called

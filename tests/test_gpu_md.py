@@ -153,3 +153,31 @@ def test_md_chunks_equal_single_steps(pkg, wl):
     for x, y in zip(a[2:5], b[2:5]):
         assert np.array_equal(x, y)
     assert 5 < a[5][1] < 150          # lists were reused, and rebuilt
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_graph_replays_equal_direct_launches(pkg, wl, precision, monkeypatch):
+    """All but the first step of a chunk are replays of a captured HIP graph
+    of one step. 60 calls of 5 steps with graphs equal the same calls with
+    every step launched directly, bit for bit after every call (each replay
+    feeds the next, with the program's own synchronisations between)."""
+    from conftest import PEROV_MD_PARAMS
+    s = wl.perovskite(4, 2, 2, seed=72)
+    m = pkg.atomic_masses(s.Z)
+    p = np.sqrt(m * 300.0 * 3.166811563e-6)[:, None] * np.random.default_rng(9).standard_normal((s.nat, 3))
+    runs = []
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("HDNNP_MD_GRAPHS", graphs)
+        h = pkg.parseRuNNerInput(PEROV_MD_PARAMS, precision)
+        md = pkg.MdSystems(h, [s], [p], DT, skin=1.0)
+        trace = []
+        for _ in range(60):
+            ep, ek = md.step(5)
+            trace.append((ep[0], ek[0], md.get()[0].copy()))
+        runs.append((trace, md.counters))
+    (ta, ca), (tb, cb) = runs
+    print(f"{precision}: with graphs {ca}; without {cb}", flush=True)
+    assert ca["graph_replays"] > 100 and ca["graph_captures"] >= 1 and cb["graph_replays"] == 0
+    assert ca["builds"] == cb["builds"] and ca["rollbacks"] == cb["rollbacks"]
+    for x, y in zip(ta, tb):
+        assert x[0] == y[0] and x[1] == y[1] and np.array_equal(x[2], y[2])
