@@ -57,7 +57,8 @@ class ModelInfo(ctypes.Structure):
                 ("nlayers", ctypes.c_int32), ("nodes", ctypes.c_int32 * 8), ("act", ctypes.c_char * 8),
                 ("scalesf", ctypes.c_int32), ("centersf", ctypes.c_int32), ("maxrc", ctypes.c_double),
                 ("atom_energy", ctypes.c_double * 16), ("emin", ctypes.c_double),
-                ("emax", ctypes.c_double)]
+                ("emax", ctypes.c_double), ("cutoff_type", ctypes.c_int32), ("scale_range", ctypes.c_int32),
+                ("scale_min", ctypes.c_double), ("scale_max", ctypes.c_double)]
 
 
 _LIB = None

@@ -273,6 +273,10 @@ int hdnnp_get_model_info(hdnnp_ctx *ctx, hdnnp_model_info *info)
     info->maxrc = m.maxrc;
     info->emin = m.emin;
     info->emax = m.emax;
+    info->cutoff_type = m.cutoff_type;
+    info->scale_range = m.scale_range;
+    info->scale_min = m.scale_min;
+    info->scale_max = m.scale_max;
     return HDNNP_OK;
 }
 

@@ -51,8 +51,8 @@ struct DevModel {
     int g4_max;              // largest g4_count
     int powbits;             // bits of the largest |g4_ipow| of integral zeta
     const ElemInfo *einfo;   // [nel]
-    const double *sf_mean;   // [sf_off[e] + s]; 0 when not centring
-    const double *sf_den;    // max - min, 1 when not scaling
+    const double *sf_mean;   // [sf_off[e] + s]: scaled G = (G - sf_mean) / sf_den
+    const double *sf_den;
     int nlayers;
     int nodes[kMaxLay];
     char act[kMaxLay];
@@ -61,6 +61,7 @@ struct DevModel {
     double atom_energy[kMaxEl];
     const int *zlut;             // Z (0..127) -> element rank; -1 absent, -2-rank unusable
     int fallback_el;             // usable element given to flagged atoms (results discarded)
+    int cutoff_type;             // RuNNer cutoff_type 0..8 (2: tanh^3, the reference's)
 };
 
 // Cell grid of one structure of an evaluation (host-computed, reference

@@ -198,6 +198,7 @@ void Engine::init_device()
     DevModel d{};
     d.nel = (int)m.el.size();
     d.fallback_el = -1;
+    d.cutoff_type = m.cutoff_type;
     std::vector<int> zlut(128, -1);
     std::vector<double> mean, den;
     std::vector<int2> g2tab(kMaxEl * kMaxEl, make_int2(0, 0)), g4tab(kMaxEl * kMaxEl * kMaxEl, make_int2(0, 0));
@@ -215,8 +216,16 @@ void Engine::init_device()
         d.rc[e] = em.sfs.empty() ? 0.0 : em.sfs[0].rcut;
         d.atom_energy[e] = em.atom_energy;
         for (auto &s : em.sfs) {
-            mean.push_back(m.centersf ? s.sfmean : 0.0);
-            den.push_back(m.scalesf ? s.sfmax - s.sfmin : 1.0);
+            // scaled G = (G - c) / den: the reference's centring and scaling
+            // (hdnnps.f90:471-481), or RuNNer's min-max form when
+            // scale_min_short / scale_max_short are given
+            double c = m.centersf ? s.sfmean : 0.0, dn = m.scalesf ? s.sfmax - s.sfmin : 1.0;
+            if (m.scale_range && m.scalesf) {
+                dn = (s.sfmax - s.sfmin) / (m.scale_max - m.scale_min);
+                c = (m.centersf ? s.sfmean : s.sfmin) - m.scale_min * dn;
+            }
+            mean.push_back(c);
+            den.push_back(dn);
         }
         for (int f = 0; f < d.nel; ++f) {          // G2 of (e, f)
             const int start = (int)g2s.size();

@@ -49,6 +49,9 @@ template <> struct M<double> {
     static __device__ double pow(double x, double y) { return ::pow(x, y); }
     // exp(-eta x) from the table value a = -eta
     static __device__ double expsc(double a, double x) { return ::exp(a * x); }
+    static __device__ double exp(double x) { return ::exp(x); }
+    static __device__ double cospi(double x) { return ::cospi(x); }
+    static __device__ double sinpi(double x) { return ::sinpi(x); }
 };
 template <> struct M<float> {
     static __device__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -62,6 +65,9 @@ template <> struct M<float> {
     static __device__ float pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
     // exp(-eta x) from the table value a = -eta log2(e)
     static __device__ float expsc(float a, float x) { return __builtin_amdgcn_exp2f(a * x); }
+    static __device__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+    static __device__ float cospi(float x) { return ::cospif(x); }
+    static __device__ float sinpi(float x) { return ::sinpif(x); }
 };
 
 // The model tables are not written while a kernel runs: reading them through
@@ -117,36 +123,117 @@ __device__ __forceinline__ double uni(double x)
     return __builtin_bit_cast(double, ((long long)hi << 32) | lo);
 }
 
-// tanh^3 cutoff, symmetryFunctions.f90:310-327; dfc = fc'(r)/r
+// Cutoff functions of RuNNer's cutoff_type (x = r/rc < 1; dfc = fc'(r)/r):
+//   0 hard 1, 1 cosine (cos(pi x) + 1)/2, 2 tanh^3(1 - x) (the reference's,
+//   symmetryFunctions.f90:310-327), 3 tanh^3(1 - x) / tanh^3(1), 4 exp(1 -
+//   1/(1 - x^2)), 5-8 the polynomials 1 + x^2 (2x - 3), 1 + x^3 ((15 - 6x) x
+//   - 10), 1 + x^4 (x (x (20x - 70) + 84) - 35), 1 + x^5 (x (x ((315 - 70x)
+//   x - 540) + 420) - 126) (RuNNer's / n2p2's numbering). The type is one per
+//   model (wave-uniform).
+constexpr double kTanhNorm = 2.2637537952253513;   // 1 / tanh^3(1)
+
+// The cutoff is evaluated without contracting products into FMAs: its
+// rounding is then the same in every kernel instance that inlines it (each
+// staging capacity), so that a structure gets the same result in any batch.
 template <typename R>
-__device__ __forceinline__ void cutoff_fn(R r, R rc, R rrc, R &fc, R &dfc)
+__device__ __forceinline__ void cutoff_other(int ct, R r, R rrc, R &fc, R &dfc)
 {
-    if (r < rc) {
-        const R t = M<R>::tanh(R(1) - r * rrc);
+#pragma clang fp contract(off)
+    const R x = r * rrc;
+    R f, d;   // f(x) and df/dx
+    switch (ct) {
+    case 0:
+        f = 1;
+        d = 0;
+        break;
+    case 1:
+        f = R(0.5) * (M<R>::cospi(x) + R(1));
+        d = R(-0.5 * M_PI) * M<R>::sinpi(x);
+        break;
+    case 3: {
+        const R t = M<R>::tanh(R(1) - x), tt = t * t;
+        f = R(kTanhNorm) * (tt * t);
+        d = R(-3.0 * kTanhNorm) * (tt - tt * tt);
+        break;
+    }
+    case 4: {
+        const R iu = M<R>::rcp(R(1) - x * x);
+        f = M<R>::exp(R(1) - iu);
+        d = f * (R(-2) * x * iu * iu);
+        break;
+    }
+    default: {
+        // the polynomials are smooth steps, f = 1 - S(x) = S(1 - x) with
+        // S(u) = u^k p(u); S is evaluated at the smaller of x and 1 - x, so
+        // that no large terms cancel; f'(x) = -c x^(k-1) (1 - x)^(k-1)
+        const R y = R(1) - x;
+        const bool lo = x < R(0.5);
+        const R u = lo ? x : y, u2 = u * u;
+        R S;
+        if (ct == 5) {
+            S = u2 * (R(3) - R(2) * u);
+            d = R(-6) * x * y;
+        } else if (ct == 6) {
+            S = u2 * u * (R(10) + u * (R(-15) + R(6) * u));
+            d = R(-30) * (x * x) * (y * y);
+        } else if (ct == 7) {
+            S = u2 * u2 * (R(35) + u * (R(-84) + u * (R(70) - R(20) * u)));
+            d = R(-140) * (x * x * x) * (y * y * y);
+        } else {
+            S = u2 * u2 * u * (R(126) + u * (R(-420) + u * (R(540) + u * (R(-315) + R(70) * u))));
+            const R xx = x * x, yy = y * y;
+            d = R(-630) * (xx * xx) * (yy * yy);
+        }
+        f = lo ? R(1) - S : S;
+        break;
+    }
+    }
+    fc = f;
+    dfc = d * rrc * M<R>::rcp(r);
+}
+
+// the cutoff within rc (the callers test r < rc)
+// tanh^3 with its fused multiply-adds written out (explicit fma calls are
+// kept as written in every instance)
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename R>
+__device__ __forceinline__ void cutoff_in(int ct, R r, R rrc, R &fc, R &dfc)
+{
+#pragma clang fp contract(off)
+    if (ct == 2) {
+        const R t = M<R>::tanh(fmaR(-r, rrc, R(1)));
         const R tt = t * t;
         fc = tt * t;
-        dfc = -(R(3) * rrc * M<R>::rcp(r)) * (tt - tt * tt);
+        dfc = -(R(3) * rrc * M<R>::rcp(r)) * fmaR(-tt, tt, tt);
+    } else {
+        cutoff_other<R>(ct, r, rrc, fc, dfc);
+    }
+}
+template <typename R>
+__device__ __forceinline__ R cutoff_in(int ct, R r, R rrc)
+{
+#pragma clang fp contract(off)
+    if (ct == 2) {
+        const R t = M<R>::tanh(fmaR(-r, rrc, R(1)));
+        return t * t * t;
+    }
+    R fc, dfc;
+    cutoff_other<R>(ct, r, rrc, fc, dfc);
+    return fc;
+}
+
+// with the range test: fc = dfc = 0 from rc on
+template <typename R>
+__device__ __forceinline__ void cutoff_fn(int ct, R r, R rc, R rrc, R &fc, R &dfc)
+{
+    if (r < rc) {
+        cutoff_in<R>(ct, r, rrc, fc, dfc);
     } else {
         fc = 0;
         dfc = 0;
     }
-}
-
-// the same without the range test, for r_jk of a compacted triplet: r_jk^2 < rc^2
-// was tested already, and at r = rc both expressions vanish
-template <typename R>
-__device__ __forceinline__ void cutoff_in(R r, R rrc, R &fc, R &dfc)
-{
-    const R t = M<R>::tanh(R(1) - r * rrc);
-    const R tt = t * t;
-    fc = tt * t;
-    dfc = -(R(3) * rrc * M<R>::rcp(r)) * (tt - tt * tt);
-}
-template <typename R>
-__device__ __forceinline__ R cutoff_in(R r, R rrc)
-{
-    const R t = M<R>::tanh(R(1) - r * rrc);
-    return t * t * t;
 }
 
 // full-wave sum with DPP (the result is read from lane 63)
@@ -462,7 +549,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
         if (f >= 0) {
             const R rr = (R)r;
             R fc, dfc;
-            cutoff_fn<R>(rr, (R)rce, rrc, fc, dfc);
+            cutoff_fn<R>(m.cutoff_type, rr, (R)rce, rrc, fc, dfc);
             typename Lds<R>::V va, vb;
             va.x = (R)d0;
             va.y = (R)d1;
@@ -620,7 +707,7 @@ __device__ __forceinline__ void g4_values(const DevModel &m, const Lds<R> &L, in
         const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
         const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk = d_ik - d_ij
         const R rjk2 = ex * ex + ey * ey + ez * ez;
-        const R fcjk = cutoff_in<R>(M<R>::sqrt(rjk2), rrc);
+        const R fcjk = cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc);
         const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * aj.w * ak.w;
         const R F = bj.y * bk.y * fcjk;
         const R r2 = bj.w + bk.w + rjk2;
@@ -664,7 +751,7 @@ __device__ __forceinline__ void g4_grad(const DevModel &m, const Lds<R> &L, int 
         const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
         const R rjk2 = ex * ex + ey * ey + ez * ez;
         R fcjk, dfcjk;
-        cutoff_in<R>(M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+        cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
         const R irj = aj.w, irk = ak.w;
         const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
         const R inv = irj * irk;
@@ -856,7 +943,7 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
         const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
         const R rjk2 = ex * ex + ey * ey + ez * ez;
         R fcjk, dfcjk;
-        cutoff_in<R>(M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+        cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
         const R irj = aj.w, irk = ak.w;
         const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
         const R inv = irj * irk;

@@ -190,6 +190,21 @@ HostModel read_runner(const std::string &dir_in)
             m.scalesf = true;
         } else if (key == "center_symmetry_functions") {
             m.centersf = true;
+        } else if (key == "cutoff_type") {                // not read by the reference
+            need(2);
+            m.cutoff_type = read_int(w[1], "cutoff_type");
+            if (m.cutoff_type < 0 || m.cutoff_type > 8)
+                throw Error(3, "cutoff_type " + w[1] + " is not one of RuNNer's cutoff functions 0..8");
+            if (w.size() > 2 && read_real(w[2], "cutoff_type") != 0.0)
+                throw Error(3, "cutoff_type with an inner cutoff (alpha != 0) is not supported");
+        } else if (key == "scale_min_short") {            // not read by the reference
+            need(2);
+            m.scale_min = read_real(w[1], "scale_min_short");
+            m.scale_range = true;
+        } else if (key == "scale_max_short") {            // not read by the reference
+            need(2);
+            m.scale_max = read_real(w[1], "scale_max_short");
+            m.scale_range = true;
         } else if (key == "global_hidden_layers_short") {
             need(2);
             nlayers = read_int(w[1], "global_hidden_layers_short");
@@ -235,7 +250,9 @@ HostModel read_runner(const std::string &dir_in)
                 s.zeta = read_real(w[7], "symfunction_short");
                 s.rcut = read_real(w[8], "symfunction_short");
             } else {
-                continue;   // other types are not read by the reference
+                // the reference skips other types silently, and its weights
+                // would then be read with the wrong layout: refused here
+                throw Error(3, "symmetry-function type " + w[2] + " is not supported (types 2 and 3 are)");
             }
             if (s.rcut > maxrc) maxrc = s.rcut;
             if (centre < 0 || !allocated[centre])
@@ -251,6 +268,8 @@ HostModel read_runner(const std::string &dir_in)
     if (nlayers < 0 || nnodes.empty() || acts.empty())
         throw Error(2, "input.nn lacks global_hidden_layers_short/global_nodes_short/global_activation_short");
     m.maxrc = maxrc;
+    if (m.scale_range && !(m.scale_max > m.scale_min))
+        throw Error(3, "scale_max_short must be greater than scale_min_short");
 
     for (auto &e : elements) {                        // :204-212, sortSymFunctions :235-252
         ElementModel em;

@@ -44,6 +44,12 @@ struct HostModel {
     std::vector<ElementModel> el;   // ascending Z
     bool scalesf = false, centersf = false;
     double maxrc = 0, emin = 0, emax = 0;
+    // RuNNer keywords beyond the reference's: cutoff_type (0..8; 2, the
+    // reference's tanh^3, when absent) and scale_min_short / scale_max_short
+    // (RuNNer's min-max scaling of the SFs to [scale_min, scale_max])
+    int cutoff_type = 2;
+    bool scale_range = false;
+    double scale_min = 0, scale_max = 1;
     int rank_of_Z(int z) const {
         for (size_t e = 0; e < el.size(); ++e)
             if (el[e].Z == z) return (int)e;

@@ -127,6 +127,13 @@ typedef struct {
     double maxrc;
     double atom_energy[16];
     double emin, emax;      /* scaling.data energy range (parsed, unused) */
+    /* RuNNer keywords the reference does not read (it always uses
+     * cutoff_type 2 and centre/scale as below): cutoff_type 0..8 (default
+     * 2); with scale_min_short / scale_max_short present (scale_range = 1)
+     * a scaled SF is scale_min + (scale_max - scale_min) (G - c) / (max -
+     * min), c = mean when centring, else min */
+    int32_t cutoff_type, scale_range;
+    double scale_min, scale_max;
 } hdnnp_model_info;
 
 int hdnnp_get_model_info(hdnnp_ctx *ctx, hdnnp_model_info *info);

@@ -77,19 +77,79 @@ void orc_wrap(int nat, const double lat[9], double *pos)
     }
 }
 
-/* cutoff, symmetryFunctions.f90:310-327: fc = tanh^3(1 - r/rc),
- * dfc = fc'(r)/r = -3/(rc r) (t^2 - t^4) */
-void orc_cutoff(double r, double rc, double *fc, double *dfc)
+/* Cutoff functions, value fc and dfc = fc'(r)/r, by RuNNer's cutoff_type
+ * (x = r/rc): 2 is the reference's, symmetryFunctions.f90:310-327,
+ * fc = tanh^3(1 - x), dfc = -3/(rc r) (t^2 - t^4). The others are written
+ * here from their definitions, as polynomials in x:
+ *   0 hard: 1
+ *   1 cosine: (cos(pi x) + 1) / 2
+ *   3 tanh^3(1 - x) / tanh^3(1)
+ *   4 exp(1 - 1/(1 - x^2))
+ *   5 1 - 3x^2 + 2x^3
+ *   6 1 - 10x^3 + 15x^4 - 6x^5
+ *   7 1 - 35x^4 + 84x^5 - 70x^6 + 20x^7
+ *   8 1 - 126x^5 + 420x^6 - 540x^7 + 315x^8 - 70x^9
+ * fc = dfc = 0 from rc on. */
+void orc_cutoff(int type, double r, double rc, double *fc, double *dfc)
 {
-    if (r < rc) {
-        const double t = tanh(1.0 - r / rc);
-        const double tt = t * t;
-        *fc = tt * t;
-        *dfc = -(3.0 / (rc * r) * (tt - tt * tt));
-    } else {
+    if (!(r < rc)) {
         *fc = 0.0;
         *dfc = 0.0;
+        return;
     }
+    const double x = r / rc;
+    double f, d;   /* f(x), f'(x) */
+    switch (type) {
+    case 0:
+        f = 1.0;
+        d = 0.0;
+        break;
+    case 1:
+        f = 0.5 * (cos(M_PI * x) + 1.0);
+        d = -0.5 * M_PI * sin(M_PI * x);
+        break;
+    case 2:
+    case 3: {
+        const double t = tanh(1.0 - r / rc);
+        const double tt = t * t;
+        const double nrm = type == 3 ? 1.0 / pow(tanh(1.0), 3) : 1.0;
+        *fc = nrm * (tt * t);
+        *dfc = nrm * -(3.0 / (rc * r) * (tt - tt * tt));
+        return;
+    }
+    case 4: {
+        const double u = 1.0 - x * x;
+        f = exp(1.0 - 1.0 / u);
+        d = f * (-2.0 * x / (u * u));
+        break;
+    }
+    default: {
+        /* the expanded monomials cancel terms of up to ~500 near x = 1:
+         * summed in extended precision, the result is exact to double */
+        const long double X = (long double)r / (long double)rc;
+        long double F, D;
+        if (type == 5) {
+            F = 1.0L - 3.0L * powl(X, 2) + 2.0L * powl(X, 3);
+            D = -6.0L * X + 6.0L * powl(X, 2);
+        } else if (type == 6) {
+            F = 1.0L - 10.0L * powl(X, 3) + 15.0L * powl(X, 4) - 6.0L * powl(X, 5);
+            D = -30.0L * powl(X, 2) + 60.0L * powl(X, 3) - 30.0L * powl(X, 4);
+        } else if (type == 7) {
+            F = 1.0L - 35.0L * powl(X, 4) + 84.0L * powl(X, 5) - 70.0L * powl(X, 6) + 20.0L * powl(X, 7);
+            D = -140.0L * powl(X, 3) + 420.0L * powl(X, 4) - 420.0L * powl(X, 5) + 140.0L * powl(X, 6);
+        } else {
+            F = 1.0L - 126.0L * powl(X, 5) + 420.0L * powl(X, 6) - 540.0L * powl(X, 7) + 315.0L * powl(X, 8) -
+                70.0L * powl(X, 9);
+            D = -630.0L * powl(X, 4) + 2520.0L * powl(X, 5) - 3780.0L * powl(X, 6) + 2520.0L * powl(X, 7) -
+                630.0L * powl(X, 8);
+        }
+        f = (double)F;
+        d = (double)D;
+        break;
+    }
+    }
+    *fc = f;
+    *dfc = d / (rc * r);
 }
 
 static void cross3(const double *a, const double *b, double *c)
@@ -365,7 +425,7 @@ static int calc_all(const orc_model *m, int e, int nn, const double *P, const in
         for (int k = 0; k < 3; ++k) dijs[3 * j + k] = P[k] - P[3 * j + k];
         const double *d = dijs + 3 * j;
         rijs[j] = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        orc_cutoff(rijs[j], rcut, &fcs[j], &dfcs[j]);
+        orc_cutoff(m->cutoff_type, rijs[j], rcut, &fcs[j], &dfcs[j]);
     }
     for (int j = 1; j < nn; ++j) {
         const double *dij = dijs + 3 * j;
@@ -388,7 +448,7 @@ static int calc_all(const orc_model *m, int e, int nn, const double *P, const in
             for (int k = 0; k < 3; ++k) djk[k] = P[3 * j + k] - P[3 * kk + k];
             const double rjk = sqrt(djk[0] * djk[0] + djk[1] * djk[1] + djk[2] * djk[2]);
             double fcjk, dfcjk;
-            orc_cutoff(rjk, rcut, &fcjk, &dfcjk);
+            orc_cutoff(m->cutoff_type, rjk, rcut, &fcjk, &dfcjk);
             const double t1 = -1.0 / (rij * rik);
             const double cost = 0.5 * (rjk * rjk - rij * rij - rik * rik) * t1;
             const double t2 = dij[0] * dik[0] + dij[1] * dik[1] + dij[2] * dik[2];
@@ -570,6 +630,14 @@ static int atom_work(const orc_model *m, const int *Z, const nlist *nl, const do
     int rc = calc_all(m, e, nn, w->P, w->els, w->sf, w->dsf, w->scratch);
     if (rc) return rc;
     for (int s = 0; s < nsf; ++s) {                    /* hdnnps.f90:471-481 */
+        if (m->scale_range && m->scalesf) {            /* RuNNer: smin + (smax-smin)(G-shift)/(max-min) */
+            const double shift = m->centersf ? m->sf_mean[o + s] : m->sf_min[o + s];
+            const double k = (m->scale_max - m->scale_min) / (m->sf_max[o + s] - m->sf_min[o + s]);
+            w->sf[s] = m->scale_min + k * (w->sf[s] - shift);
+            double *d = w->dsf + (size_t)s * nn * 3;
+            for (int q = 0; q < nn * 3; ++q) d[q] = k * d[q];
+            continue;
+        }
         if (m->centersf) w->sf[s] = w->sf[s] - m->sf_mean[o + s];
         if (m->scalesf) {
             const double den = m->sf_max[o + s] - m->sf_min[o + s];

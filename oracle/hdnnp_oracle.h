@@ -35,14 +35,18 @@ typedef struct {
     int nodes[ORC_MAXLAYERS];    /* nOut of each layer                     */
     char act[ORC_MAXLAYERS];     /* 'l' 't' 'p' 'r'                        */
     const double *weights[ORC_MAXEL]; /* per layer W(i,j) at i+nOut*j, then bias */
+    /* RuNNer keywords the reference does not read */
+    int cutoff_type;             /* 0..8; 2 = the reference's tanh^3        */
+    int scale_range;             /* scale_min_short / scale_max_short given */
+    double scale_min, scale_max;
 } orc_model;
 
 /* linalg.f90:264-289 */
 void orc_inv3(const double A[9], double B[9]);
 /* pointsets.f90:201-217 */
 void orc_wrap(int nat, const double lat[9], double *pos);
-/* symmetryFunctions.f90:310-327 */
-void orc_cutoff(double r, double rc, double *fc, double *dfc);
+/* cutoff of RuNNer's cutoff_type; type 2 is symmetryFunctions.f90:310-327 */
+void orc_cutoff(int type, double r, double rc, double *fc, double *dfc);
 
 /* Neighbour entries of buildNeighborListGrid (neighborLists.f90:104-230),
  * preceded by the wrap of hdnnpCalcNeighborLists (hdnnps.f90:373-388),

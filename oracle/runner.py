@@ -98,6 +98,10 @@ class Potential:
     maxrc: float = 0.0
     emin: float = 0.0
     emax: float = 0.0
+    cutoff_type: int = 2                 # RuNNer keywords the reference does not read
+    scale_range: bool = False
+    scale_min: float = 0.0
+    scale_max: float = 1.0
 
 
 def _stable_sort_index(keys):
@@ -117,6 +121,7 @@ def read_potential(path: str) -> Potential:
     sflists = {}
     atom_energy = {}
     scalesf = centersf = False
+    extra = {}
     nlayers = 0
     nnodes, acts = [], []
     maxrc = 0.0
@@ -141,6 +146,16 @@ def read_potential(path: str) -> Potential:
             scalesf = True
         elif key == "center_symmetry_functions":
             centersf = True
+        elif key == "cutoff_type":
+            extra["cutoff_type"] = int(w[1])
+            if not 0 <= extra["cutoff_type"] <= 8:
+                raise RunnerError(f"cutoff_type {w[1]} is not one of RuNNer's 0..8")
+            if len(w) > 2 and _freal(w[2]) != 0.0:
+                raise RunnerError("cutoff_type with an inner cutoff is not supported")
+        elif key == "scale_min_short":
+            extra["scale_min"], extra["scale_range"] = _freal(w[1]), True
+        elif key == "scale_max_short":
+            extra["scale_max"], extra["scale_range"] = _freal(w[1]), True
         elif key == "global_hidden_layers_short":
             nlayers = int(w[1])
         elif key == "global_nodes_short":
@@ -217,8 +232,10 @@ def read_potential(path: str) -> Potential:
             ls.append((W, b))
             nin = nout
         layers[z] = ls
+    if extra.get("scale_range") and not extra.get("scale_max", 1.0) > extra.get("scale_min", 0.0):
+        raise RunnerError("scale_max_short must be greater than scale_min_short")
     return Potential(zs, {z: sflists[z] for z in zs}, layers, acts, nnodes, atom_energy,
-                     scalesf, centersf, maxrc, emin, emax)
+                     scalesf, centersf, maxrc, emin, emax, **extra)
 
 
 # ---------------------------------------------------------------------------
@@ -241,6 +258,8 @@ class _Model(ctypes.Structure):
         ("atom_energy", ctypes.c_double * MAXEL), ("maxrc", ctypes.c_double),
         ("nlayers", ctypes.c_int), ("nodes", ctypes.c_int * MAXLAYERS),
         ("act", ctypes.c_char * MAXLAYERS), ("weights", _dp * MAXEL),
+        ("cutoff_type", ctypes.c_int), ("scale_range", ctypes.c_int),
+        ("scale_min", ctypes.c_double), ("scale_max", ctypes.c_double),
     ]
 
 
@@ -259,7 +278,7 @@ def load_library():
         lib.orc_neighbor_entries.restype = ctypes.c_long
         lib.orc_neighbor_entries.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double,
                                              ctypes.c_long, _ip]
-        lib.orc_cutoff.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.orc_cutoff.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         _LIB = lib
     return _LIB
 
@@ -303,6 +322,10 @@ class Oracle:
             setattr(m, name, _ptr(arr([getattr(s, attr) for s in allsf], np.float64), _dp))
         m.scalesf = int(pot.scalesf)
         m.centersf = int(pot.centersf)
+        m.cutoff_type = int(pot.cutoff_type)
+        m.scale_range = int(pot.scale_range)
+        m.scale_min = float(pot.scale_min)
+        m.scale_max = float(pot.scale_max)
         m.maxrc = pot.maxrc
         m.nlayers = len(pot.nodes)
         for l, n in enumerate(pot.nodes):
@@ -400,8 +423,8 @@ def canonical(ent: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(ent[order])
 
 
-def cutoff(r: float, rc: float):
+def cutoff(r: float, rc: float, cutoff_type: int = 2):
     lib = load_library()
     fc, dfc = ctypes.c_double(), ctypes.c_double()
-    lib.orc_cutoff(r, rc, ctypes.byref(fc), ctypes.byref(dfc))
+    lib.orc_cutoff(int(cutoff_type), r, rc, ctypes.byref(fc), ctypes.byref(dfc))
     return fc.value, dfc.value
