@@ -204,6 +204,9 @@ def bench_batch(args):
     res = {}
     for mode in args.modes.split(","):
         h = pkg.parseRuNNerInput(PARAMS, mode)
+        tile, tile_ms = h.autotune(ss)               # the context's tile, timed on this batch
+        print(f"{mode}: autotuner keeps {tile} atom(s) per wavefront; SF-pass ms per tile {tile_ms}",
+              file=sys.stderr, flush=True)
         for _ in range(args.warmup):
             h.evaluate_batch_torch(nat_s, Z, pos, per, lats, force)
         torch.cuda.synchronize()
@@ -216,6 +219,7 @@ def bench_batch(args):
         t = float(np.median(lat))
         res[mode] = {"atoms_per_s": nat / t, "structures_per_s": len(ss) / t, "ms_per_batch": t * 1e3,
                      "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+                     "autotune": {"atoms_per_wave": tile, "sf_ms_per_tile": tile_ms},
                      "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy()}
         print(f"{mode}: {t * 1e3:.3f} ms per batch -> {nat / t:.4g} atoms/s, {len(ss) / t:.4g} structures/s; "
               f"stage ms {res[mode]['stage_ms']}", file=sys.stderr, flush=True)

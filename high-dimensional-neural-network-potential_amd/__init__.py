@@ -35,7 +35,7 @@ EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_eval
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
-           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy",
+           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy", "hdnnp_set_tile", "hdnnp_get_tile", "hdnnp_autotune",
            "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_md_open",
            "hdnnp_server_md_step", "hdnnp_server_md_wait", "hdnnp_server_md_get", "hdnnp_server_md_close",
            "hdnnp_server_stats", "hdnnp_server_last_error", "hdnnp_server_destroy")
@@ -107,6 +107,9 @@ def load_library():
         lib.hdnnp_last_timings.argtypes = [vp, _dp]
         lib.hdnnp_destroy.argtypes = [vp]
         lib.hdnnp_destroy.restype = None
+        lib.hdnnp_set_tile.argtypes = [vp, ctypes.c_int32]
+        lib.hdnnp_get_tile.argtypes = [vp, _ip]
+        lib.hdnnp_autotune.argtypes = [vp, ctypes.c_int32, _ip, _ip, _dp, _ip, _dp, _ip, _dp]
         i64p = ctypes.POINTER(ctypes.c_int64)
         lib.hdnnp_server_create.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(vp)]
@@ -218,6 +221,31 @@ class Hdnnp:
         self._check(self.lib.hdnnp_evaluate(self.ctx, nat, _p(Z, _ip), _p(pos, _dp), int(bool(s.periodic)),
                                             _p(lat, _dp), ctypes.byref(E), _p(F, _dp), _p(L, _dp)))
         return E.value, F, L
+
+    @property
+    def tile(self) -> int:
+        """atoms per wavefront of the symmetry-function kernels (1, 2 or 4)"""
+        t = ctypes.c_int32()
+        self._check(self.lib.hdnnp_get_tile(self.ctx, ctypes.byref(t)))
+        return t.value
+
+    @tile.setter
+    def tile(self, atoms_per_wave: int):
+        self._check(self.lib.hdnnp_set_tile(self.ctx, int(atoms_per_wave)))
+
+    def autotune(self, structures):
+        """hdnnp_autotune on a sample: (chosen tile, {tile: ms of the SF passes})."""
+        nat = np.array([int(np.asarray(s.Z).shape[0]) for s in structures], dtype=np.int32)
+        Z = np.ascontiguousarray(np.concatenate([np.asarray(s.Z, dtype=np.int32) for s in structures]))
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(s.pos, dtype=np.float64).reshape(-1, 3)
+                                                   for s in structures]))
+        per = np.array([int(bool(s.periodic)) for s in structures], dtype=np.int32)
+        lat = np.ascontiguousarray(np.stack([np.asarray(s.lat, dtype=np.float64).reshape(3, 3) for s in structures]))
+        t = ctypes.c_int32()
+        ms = np.zeros(3)
+        self._check(self.lib.hdnnp_autotune(self.ctx, len(structures), _p(nat, _ip), _p(Z, _ip), _p(pos, _dp),
+                                            _p(per, _ip), _p(lat, _dp), ctypes.byref(t), _p(ms, _dp)))
+        return t.value, dict(zip((1, 2, 4), ms.tolist()))
 
     def energy_and_forces_batch(self, structures, want_dEdLat: bool = True):
         """hdnnp_evaluate_batch: many structures in one evaluation. Returns

@@ -370,6 +370,29 @@ void hdnnp_destroy(hdnnp_ctx *ctx)
     delete ctx;
 }
 
+int hdnnp_set_tile(hdnnp_ctx *ctx, int32_t atoms_per_wave)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->set_tile(atoms_per_wave); });
+}
+
+int hdnnp_get_tile(const hdnnp_ctx *ctx, int32_t *atoms_per_wave)
+{
+    if (!ctx || !atoms_per_wave) return HDNNP_ERR_ARG;
+    *atoms_per_wave = ctx->eng->tile();
+    return HDNNP_OK;
+}
+
+int hdnnp_autotune(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z, const double *pos,
+                   const int32_t *periodic, const double *lat, int32_t *atoms_per_wave, double *ms)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        const int t = ctx->eng->autotune(nstruct, nat, Z, pos, periodic, lat, ms);
+        if (atoms_per_wave) *atoms_per_wave = t;
+    });
+}
+
 int hdnnp_server_create(int32_t n, const char *const *param_dirs, int32_t precision, int32_t lanes,
                         int64_t hbm_budget_bytes, int64_t max_batch_atoms, hdnnp_server **out)
 {

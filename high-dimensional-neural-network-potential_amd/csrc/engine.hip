@@ -411,7 +411,8 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     gent_.ensure(sizeof(R) * 3 * (size_t)nat * w.cap);
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
-    launch_sf_values<R>(dm_, g, w, w.cap, G, s);
+    const int ga = sf_tile(sizeof(R) == 8, w.cap, tile_, dm_.nsfmax, dm_.g4_max);
+    launch_sf_values<R>(dm_, g, w, w.cap, ga, G, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
 
@@ -429,7 +430,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[4], s));
 
-    launch_sf_grad<R>(dm_, g, w, w.cap, dEdG, gent, s);
+    launch_sf_grad<R>(dm_, g, w, w.cap, ga, dEdG, gent, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[5], s));
 
@@ -755,7 +756,7 @@ bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dE
 std::vector<char> Engine::launch_key() const
 {
     const void *ptrs[] = {G_.p, dEdG_.p, gent_.p, chunks_.p, sc_.p, partial_.p, res_.p, misc_.p};
-    const int ints[] = {prep_.nat, prep_.nchunk, (int)prep_.st.size(), precision_, prep_.valid ? 1 : 0};
+    const int ints[] = {prep_.nat, prep_.nchunk, (int)prep_.st.size(), precision_, prep_.valid ? 1 : 0, tile_};
     std::vector<char> k(sizeof(Work) + sizeof(Geo) + sizeof ptrs + sizeof ints);
     char *p = k.data();
     std::memcpy(p, &prep_.w, sizeof(Work));
@@ -920,6 +921,50 @@ void Engine::evaluate_batch_host(int nstruct, const int *nat, const int *Z, cons
         HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)total, hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
     }
+}
+
+void Engine::set_tile(int atoms_per_wave)
+{
+    if (atoms_per_wave != 1 && atoms_per_wave != 2 && atoms_per_wave != 4)
+        throw Error(1, "the tile (atoms per wavefront) must be 1, 2 or 4");
+    tile_ = atoms_per_wave;
+}
+
+int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
+                     const double *lat, double *ms)
+{
+    if (nstruct < 1 || !nat) throw Error(1, "autotune needs a sample of at least one structure");
+    long total = 0;
+    for (int k = 0; k < nstruct; ++k) total += std::max(0, nat[k]);
+    std::vector<double> E(nstruct), F(3 * (size_t)std::max<long>(total, 1));
+    const int cand[3] = {1, 2, 4};
+    const int keep = tile_;
+    int best = 1;
+    double best_ms = 0;
+    try {
+        for (int c = 0; c < 3; ++c) {
+            tile_ = cand[c];
+            evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);   // warm-up
+            double t = 0;
+            if (sf_tile(precision_ == 0, last_cap_, cand[c], dm_.nsfmax, dm_.g4_max) == cand[c]) {
+                for (int rep = 0; rep < 3; ++rep) {
+                    evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);
+                    const double v = ms_[2] + ms_[4];
+                    t = rep == 0 ? v : std::min(t, v);
+                }
+                if (c == 0 || t < best_ms) {
+                    best = cand[c];
+                    best_ms = t;
+                }
+            }
+            if (ms) ms[c] = t;
+        }
+    } catch (...) {
+        tile_ = keep;
+        throw;
+    }
+    tile_ = best;
+    return best;
 }
 
 void Engine::evaluate_host(int nat, const int *Z, const double *pos, int periodic, const double *lat,

@@ -103,6 +103,20 @@ class Engine {
     void ensure_device() { if (!dev_ready_) init_device(); }
     hipStream_t stream() const { return stream_; }
 
+    // Tile of the symmetry-function kernels: atoms per wavefront, 1 (the
+    // default), 2 or 4. It is one per context, so every evaluation on the
+    // context uses it; a staging capacity or LDS size that does not take it
+    // falls back to a smaller one.
+    void set_tile(int atoms_per_wave);
+    int tile() const { return tile_; }
+    // Times the tiles on a sample (nstruct structures, host arrays as for
+    // evaluate_batch_host): the two symmetry-function passes by HIP events,
+    // best of 3 after a warm-up. Keeps the fastest as the context's tile and
+    // returns it; ms[3] (may be null) gets the time of tiles 1, 2, 4 (0 for
+    // one the sample's staging capacity does not take).
+    int autotune(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
+                 const double *lat, double *ms);
+
     // Device memory of the work buffers: bytes held, and their release (the
     // next evaluation allocates again; the device copy of the potential stays)
     size_t device_bytes() const;
@@ -156,6 +170,7 @@ class Engine {
         bool valid = false;
     } prep_;
     unsigned long long build_gen_ = 0;
+    int tile_ = 1;
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
     size_t pinned_bytes_ = 0;

@@ -265,14 +265,12 @@ __device__ __forceinline__ int mbcnt(unsigned long long m)
 template <typename R> struct Vec4 { typedef R T __attribute__((ext_vector_type(4))); };
 typedef int I2 __attribute__((ext_vector_type(2)));
 
-// Atoms per wavefront for a staging capacity. Several atoms of one element
-// can share a wavefront (64/GA lanes each; the SF parameters stay
-// wave-uniform), which fills the lanes of the small element-pair blocks, but
-// the LDS per atom then limits the waves in flight: one atom per wavefront
-// measured fastest on MI355X for 50-100 neighbours.
-template <int NMAX> struct AtomsPerWave {
-    static constexpr int value = 1;
-};
+// Atoms per wavefront (GA, the tile): several atoms of one element can share
+// a wavefront (64/GA lanes each; the SF parameters stay wave-uniform), which
+// fills the lanes when atoms have few neighbours, but the LDS per atom then
+// limits the waves in flight. One atom per wavefront is the default and
+// measured fastest for 50-100 neighbours; 2 and 4 are built for staging
+// capacities up to 128 and chosen per context (Engine::set_tile, autotune).
 
 // One atom's LDS region.
 template <typename R>
@@ -994,10 +992,10 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
 }
 
 // ---------------------------------------------------------------- pass A
-template <typename R, int NMAX>
+template <typename R, int NMAX, int GA>
 __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *G)
 {
-    constexpr int GA = AtomsPerWave<NMAX>::value, GL = 64 / GA;
+    constexpr int GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
     int e, t0, tn;
     elem_tile(w, m.nel, blockIdx.x, GA, e, t0, tn);
@@ -1038,10 +1036,10 @@ __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *
 }
 
 // ---------------------------------------------------------------- pass B
-template <typename R, int NMAX>
+template <typename R, int NMAX, int GA>
 __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const R *dEdG, R *gent)
 {
-    constexpr int GA = AtomsPerWave<NMAX>::value, GL = 64 / GA;
+    constexpr int GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
     int e, t0, tn;
     elem_tile(w, m.nel, blockIdx.x, GA, e, t0, tn);
@@ -1125,39 +1123,51 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
 static int sf_blocks(const DevModel &m, const Work &w, int ga) { return ga == 1 ? w.nat : (w.nat + ga - 1) / ga + m.nel; }
 
 template <typename R>
-void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, R *G, hipStream_t s)
+void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, R *G, hipStream_t s)
 {
-#define HDNNP_SFV(N)                                                                               \
-    hipLaunchKernelGGL((k_sf_values<R, N>), dim3(sf_blocks(m, w, AtomsPerWave<N>::value)), dim3(64), \
-                       lds_bytes<R>(N, AtomsPerWave<N>::value, m.nsfmax, 0, false), s, m, g, w, G)
-    if (nmax <= 64) HDNNP_SFV(64);
-    else if (nmax <= 128) HDNNP_SFV(128);
-    else if (nmax <= 256) HDNNP_SFV(256);
-    else if (nmax <= 512) HDNNP_SFV(512);
-    else HDNNP_SFV(1024);
+#define HDNNP_SFV(N, A)                                                                            \
+    hipLaunchKernelGGL((k_sf_values<R, N, A>), dim3(sf_blocks(m, w, A)), dim3(64),                  \
+                       lds_bytes<R>(N, A, m.nsfmax, 0, false), s, m, g, w, G)
+    if (nmax <= 64) {
+        if (ga == 4) HDNNP_SFV(64, 4);
+        else if (ga == 2) HDNNP_SFV(64, 2);
+        else HDNNP_SFV(64, 1);
+    } else if (nmax <= 128) {
+        if (ga == 4) HDNNP_SFV(128, 4);
+        else if (ga == 2) HDNNP_SFV(128, 2);
+        else HDNNP_SFV(128, 1);
+    } else if (nmax <= 256) HDNNP_SFV(256, 1);
+    else if (nmax <= 512) HDNNP_SFV(512, 1);
+    else HDNNP_SFV(1024, 1);
 #undef HDNNP_SFV
 }
 
 template <typename R>
-void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, const R *dEdG, R *gent,
+void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, const R *dEdG, R *gent,
                     hipStream_t s)
 {
-#define HDNNP_SFG(N)                                                                               \
-    hipLaunchKernelGGL((k_sf_grad<R, N>), dim3(sf_blocks(m, w, AtomsPerWave<N>::value)), dim3(64),   \
-                       lds_bytes<R>(N, AtomsPerWave<N>::value, m.nsfmax, m.g4_max, true), s, m, g, w, dEdG, gent)
-    if (nmax <= 64) HDNNP_SFG(64);
-    else if (nmax <= 128) HDNNP_SFG(128);
-    else if (nmax <= 256) HDNNP_SFG(256);
-    else if (nmax <= 512) HDNNP_SFG(512);
-    else HDNNP_SFG(1024);
+#define HDNNP_SFG(N, A)                                                                            \
+    hipLaunchKernelGGL((k_sf_grad<R, N, A>), dim3(sf_blocks(m, w, A)), dim3(64),                    \
+                       lds_bytes<R>(N, A, m.nsfmax, m.g4_max, true), s, m, g, w, dEdG, gent)
+    if (nmax <= 64) {
+        if (ga == 4) HDNNP_SFG(64, 4);
+        else if (ga == 2) HDNNP_SFG(64, 2);
+        else HDNNP_SFG(64, 1);
+    } else if (nmax <= 128) {
+        if (ga == 4) HDNNP_SFG(128, 4);
+        else if (ga == 2) HDNNP_SFG(128, 2);
+        else HDNNP_SFG(128, 1);
+    } else if (nmax <= 256) HDNNP_SFG(256, 1);
+    else if (nmax <= 512) HDNNP_SFG(512, 1);
+    else HDNNP_SFG(1024, 1);
 #undef HDNNP_SFG
 }
 
-template void launch_sf_values<double>(const DevModel &, const Geo &, Work &, int, double *, hipStream_t);
-template void launch_sf_values<float>(const DevModel &, const Geo &, Work &, int, float *, hipStream_t);
-template void launch_sf_grad<double>(const DevModel &, const Geo &, Work &, int, const double *, double *,
+template void launch_sf_values<double>(const DevModel &, const Geo &, Work &, int, int, double *, hipStream_t);
+template void launch_sf_values<float>(const DevModel &, const Geo &, Work &, int, int, float *, hipStream_t);
+template void launch_sf_grad<double>(const DevModel &, const Geo &, Work &, int, int, const double *, double *,
                                      hipStream_t);
-template void launch_sf_grad<float>(const DevModel &, const Geo &, Work &, int, const float *, float *,
+template void launch_sf_grad<float>(const DevModel &, const Geo &, Work &, int, int, const float *, float *,
                                     hipStream_t);
 
 // staging capacity of the kernel instance that serves nmax neighbours
@@ -1168,16 +1178,27 @@ int sf_bucket(int nmax)
     return b;
 }
 
-static int atoms_per_wave(int nmax)
-{
-    return nmax <= 128 ? AtomsPerWave<128>::value : (nmax <= 256 ? AtomsPerWave<256>::value : AtomsPerWave<512>::value);
-}
-
+// LDS of one wavefront of the kernels serving nmax neighbours with one atom
+// per wavefront (this decides the staging capacity limit)
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad)
 {
     nmax = sf_bucket(nmax);
-    const int ga = atoms_per_wave(nmax);
-    return fp64 ? lds_bytes<double>(nmax, ga, nsfmax, g4max, grad) : lds_bytes<float>(nmax, ga, nsfmax, g4max, grad);
+    return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, grad) : lds_bytes<float>(nmax, 1, nsfmax, g4max, grad);
+}
+
+// the tile the kernels use for a requested one: ga when that variant is built
+// for the staging capacity and its wavefront fits 64 KiB of LDS in both
+// passes, else the largest smaller one that does (1 always does)
+int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max)
+{
+    const int b = sf_bucket(nmax);
+    for (int a = ga >= 4 ? 4 : (ga >= 2 ? 2 : 1); a > 1; a /= 2) {
+        if (b > 128) continue;
+        const size_t v = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, false) : lds_bytes<float>(b, a, nsfmax, g4max, false);
+        const size_t gr = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, true) : lds_bytes<float>(b, a, nsfmax, g4max, true);
+        if (v <= 65536 && gr <= 65536) return a;
+    }
+    return 1;
 }
 
 }  // namespace hdnnp

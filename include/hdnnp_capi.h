@@ -178,6 +178,25 @@ int hdnnp_md_stats(const hdnnp_md *md, int64_t *steps, int64_t *list_builds);
 int hdnnp_md_counters(const hdnnp_md *md, int64_t *out);
 void hdnnp_md_destroy(hdnnp_md *md);
 
+/* Tile of the symmetry-function kernels, one per context (no counterpart
+ * in the reference): atoms per wavefront, 1 (default), 2 or 4. Several atoms
+ * per wavefront fill the lanes when atoms have few neighbours (molecules,
+ * small clusters); one is fastest for condensed matter. Kernels serving more
+ * than 128 staged neighbours, or whose LDS would not fit, use a smaller tile.
+ * The tile sets the order in which partial G4 sums are added, so results of
+ * different tiles agree to rounding. Batched results equal single ones bit
+ * for bit when both run the same tile: always for tiles 1 and 2; tile 4
+ * falls back to 2 in FP64 beyond 64 staged neighbours (LDS). */
+int hdnnp_set_tile(hdnnp_ctx *ctx, int32_t atoms_per_wave);
+int hdnnp_get_tile(const hdnnp_ctx *ctx, int32_t *atoms_per_wave);
+/* Autotuner: evaluates a sample (as for hdnnp_evaluate_batch) with each tile,
+ * times the two symmetry-function passes by HIP events (best of 3 after a
+ * warm-up), and keeps the fastest as the context's tile. atoms_per_wave gets
+ * the choice, ms[3] (may be NULL) the time of tiles 1, 2, 4 (0: not taken by
+ * the sample's staging capacity). */
+int hdnnp_autotune(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z, const double *pos,
+                   const int32_t *periodic, const double *lat, int32_t *atoms_per_wave, double *ms);
+
 /* Request server. The reference has no counterpart: its potential is called
  * by one serial caller at a time (hdnnps.f90:333, hdnnpExample.f90:58-72).
  * The server keeps the potential resident on the device for many callers:
