@@ -147,8 +147,10 @@ void Server::finish_job(Job &j, int code, const std::string &err)
 // The next work of a lane. Evaluations (latency-bound requests) go first:
 // every queued one that fits, in arrival order, as one batch. MD requests run
 // when no evaluation waits, oldest first and each session's in order; one
-// that has waited longer than kMdAging goes before the evaluations, so that
-// sessions never starve. Called with mu_ held by an idle lane.
+// that has waited longer than kMdAging goes before the evaluations while no
+// other lane runs an MD request, so that sessions never starve and, when they
+// alone would fill every lane, evaluations still keep all lanes but one.
+// Called with mu_ held by an idle lane.
 bool Server::pick(int l, std::vector<std::shared_ptr<Job>> &batch, std::shared_ptr<Job> &one)
 {
     Lane &L = *lanes_[l];
@@ -188,7 +190,7 @@ bool Server::pick(int l, std::vector<std::shared_ptr<Job>> &batch, std::shared_p
             break;
         }
     }
-    const bool md_overdue = md && Clock::now() - md->t_submit > kMdAging;
+    const bool md_overdue = md && md_running_ == 0 && Clock::now() - md->t_submit > kMdAging;
 
     // a batch of evaluations
     Engine::Footprint fp;
@@ -244,6 +246,7 @@ bool Server::pick(int l, std::vector<std::shared_ptr<Job>> &batch, std::shared_p
             sessions_bytes_ += j.est;
         }
         j.sess->busy = true;
+        ++md_running_;
         one = md;
     }
     if (failed.empty() && batch.empty() && !one) return false;
@@ -414,6 +417,7 @@ void Server::run_md(int l, const std::shared_ptr<Job> &jp)
     Lane &L = *lanes_[l];
     L.running = false;
     --running_;
+    --md_running_;
     sessions_bytes_ -= S.bytes;
     S.bytes = held;
     sessions_bytes_ += held;

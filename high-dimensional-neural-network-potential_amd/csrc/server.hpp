@@ -5,8 +5,10 @@
 // stream and work buffers). A free lane takes every queued evaluation that
 // fits (in arrival order, up to max_batch_atoms atoms and within the device
 // memory budget) and evaluates them together as one batch; when none waits,
-// it takes the oldest request of an MD session (an MD request that has waited
-// longer than kMdAging goes first). Requests that arrive while the lanes are
+// it takes the oldest request of an MD session. An MD request that has waited
+// longer than kMdAging goes first while no other lane runs one, so under
+// evaluation load the sessions keep one lane and never starve, and the
+// evaluations keep the others. Requests that arrive while the lanes are
 // busy wait for the next free lane; lanes run at the same time on their own
 // streams, so their work overlaps on the device.
 //
@@ -80,7 +82,8 @@ class Server {
     // MD steps
     static constexpr int kStats = 16;
     void stats(double *out, bool reset);
-    // an MD request waiting longer than this goes before evaluations
+    // an MD request waiting longer than this goes before evaluations (while
+    // no other lane runs an MD request)
     static constexpr std::chrono::milliseconds kMdAging{100};
 
   private:
@@ -151,6 +154,7 @@ class Server {
     std::vector<double> lat_eval_, lat_md_;
     double atoms_eval_ = 0, atom_steps_ = 0, md_steps_ = 0, batches_ = 0, batch_structs_ = 0, refused_ = 0;
     int running_ = 0, max_running_ = 0;
+    int md_running_ = 0;                 // lanes running an MD request
     Clock::time_point t0_;
 };
 

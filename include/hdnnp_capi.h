@@ -207,7 +207,8 @@ int hdnnp_autotune(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const in
  *    stream and buffers, so their work overlaps on the device.
  *  - MD sessions (hdnnp_server_md_*) keep positions, momenta and neighbour
  *    lists (with a Verlet skin) resident between requests. Their requests
- *    run when no evaluation waits, or once they have waited 100 ms.
+ *    run when no evaluation waits, or once they have waited 100 ms and no
+ *    other lane runs an MD request.
  *  - Admission is by device memory. The buffers of the lanes and the state
  *    of the sessions stay within hbm_budget_bytes (0: 80% of the free
  *    device memory). Work waits until it fits; work larger than the whole
