@@ -97,6 +97,7 @@ class Engine {
     // bounding boxes of the nonperiodic structures (for build), from device positions
     void device_bboxes(std::vector<StructIn> &st, const double *pos_dev, hipStream_t s);
     int prepared_nat() const { return prep_.nat; }
+    bool prepared() const { return prep_.valid; }   // a built list that finish can use
     // count of builds so far: a caller that reuses lists checks that no other
     // evaluation on this engine has built since its own build
     unsigned long long build_generation() const { return build_gen_; }

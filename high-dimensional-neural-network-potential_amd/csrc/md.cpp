@@ -19,8 +19,19 @@ MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const 
                      const double *mass, const int *periodic, const double *lat, double dt, double skin)
     : eng_(eng), dt_(dt), skin_(skin)
 {
+    try {
+        init(nsys, nat, Z, pos, mom, mass, periodic, lat);
+    } catch (...) {
+        release();   // a destructor does not run for a constructor that throws
+        throw;
+    }
+}
+
+void MdSystems::init(int nsys, const int *nat, const int *Z, const double *pos, const double *mom, const double *mass,
+                     const int *periodic, const double *lat)
+{
     if (nsys < 1 || !nat || !Z || !pos || !mom || !mass || !periodic) throw Error(1, "null argument or nsys < 1");
-    if (!(dt > 0) || !(skin >= 0)) throw Error(1, "dt must be > 0 and skin >= 0");
+    if (!(dt_ > 0) || !(skin_ >= 0)) throw Error(1, "dt must be > 0 and skin >= 0");
     eng_.ensure_device();
     st_.resize(nsys);
     std::vector<int> off(nsys + 1, 0);
@@ -69,14 +80,19 @@ MdSystems::MdSystems(Engine &eng, int nsys, const int *nat, const int *Z, const 
     forces(nullptr);     // initial forces (the reference's call before the loop)
 }
 
-MdSystems::~MdSystems()
+MdSystems::~MdSystems() { release(); }
+
+void MdSystems::release()
 {
-    for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_, &snap_, &d2_})
+    for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_, &snap_, &d2_,
+                      &ctr_})
         d->release();
     if (d2_host_) (void)hipHostFree(d2_host_);
+    d2_host_ = nullptr;
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    ctr_.release();
+    exec_ = nullptr;
+    graph_ = nullptr;
 }
 
 size_t MdSystems::device_bytes() const
@@ -93,7 +109,7 @@ size_t MdSystems::device_bytes() const
 void MdSystems::forces(double *epot)
 {
     hipStream_t s = eng_.stream();
-    bool rebuild = !built_ || skin_ == 0.0 || eng_.build_generation() != gen_;
+    bool rebuild = !built_ || skin_ == 0.0 || eng_.build_generation() != gen_ || !eng_.prepared();
     if (!rebuild) {
         MDCHK(hipMemsetAsync(maxd2_.p, 0, 8, s));
         launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, maxd2_.as<unsigned long long>(), nullptr, s);
@@ -233,7 +249,7 @@ void MdSystems::step(int nsteps, double *epot, double *ekin)
     if (nsteps < 0) throw Error(1, "nsteps < 0");
     hipStream_t s = eng_.stream();
     for (int t = 0; t < nsteps;) {
-        const bool reuse = skin_ > 0 && built_ && !careful_ && eng_.build_generation() == gen_;
+        const bool reuse = skin_ > 0 && built_ && !careful_ && eng_.build_generation() == gen_ && eng_.prepared();
         if (!reuse) {
             careful_step();
             ++t;

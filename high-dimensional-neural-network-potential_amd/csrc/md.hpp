@@ -56,6 +56,9 @@ class MdSystems {
   private:
     static constexpr int kMaxChunk = 64;
     void forces(double *epot);
+    void init(int nsys, const int *nat, const int *Z, const double *pos, const double *mom, const double *mass,
+              const int *periodic, const double *lat);
+    void release();               // every device and pinned buffer, the graph
     void careful_step();          // one step, the skin checked before its forces
     int chunk(int k);             // up to k steps on reused lists; the steps kept
     void step_launch(bool timed); // one chunk step's kernels (the captured sequence)

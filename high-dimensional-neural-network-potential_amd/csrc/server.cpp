@@ -92,6 +92,7 @@ Server::~Server()
         queue_.clear();
         jobs_.clear();
     }
+    (void)hipSetDevice(dev_);
     sessions_.clear();   // each session's MD state, then its engine
     lanes_.clear();
 }
@@ -591,6 +592,7 @@ void Server::md_get(long long id, double *pos, double *mom, double *force)
     int code = 0;
     std::string err;
     try {
+        SVCHK(hipSetDevice(dev_));   // the session's device, from any client thread
         S->md->get(pos, mom, force);
     } catch (const Error &e) {
         code = e.code;
@@ -614,6 +616,7 @@ void Server::md_close(long long id)
         sessions_.erase(id);
         S->busy = true;
     }
+    (void)hipSetDevice(dev_);
     S->md.reset();
     S->eng.reset();
     std::lock_guard<std::mutex> lk(mu_);

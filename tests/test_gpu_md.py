@@ -181,3 +181,30 @@ def test_md_graph_replays_equal_direct_launches(pkg, wl, precision, monkeypatch)
     assert ca["builds"] == cb["builds"] and ca["rollbacks"] == cb["rollbacks"]
     for x, y in zip(ta, tb):
         assert x[0] == y[0] and x[1] == y[1] and np.array_equal(x[2], y[2])
+
+
+def test_md_failed_create_frees_its_buffers(pkg, wl):
+    """An MD object whose first forces fail (an element the potential lacks)
+    raises and gives its device buffers back: the per-step state saves of a
+    24,576-atom cell are ~115 MB, so four leaked creations would show."""
+    import torch
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed")
+    s = wl.perovskite(16, 16, 8, seed=3)
+    s.Z = s.Z.copy()
+    s.Z[5] = 79
+    p = np.zeros((s.nat, 3))
+    m = np.ones(s.nat)
+    with pytest.raises(pkg.HdnnpError):            # sizes the context's own buffers, which it keeps
+        pkg.MdSystems(h, [s], [p], DT, skin=1.0, masses=m)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(4):
+        with pytest.raises(pkg.HdnnpError) as ei:
+            pkg.MdSystems(h, [s], [p], DT, skin=1.0, masses=m)
+        assert ei.value.code == 4 and "Z=79" in str(ei.value)
+    torch.cuda.synchronize()
+    assert free0 - torch.cuda.mem_get_info()[0] < (64 << 20)
+    # the context still evaluates
+    good = wl.perovskite(2, 2, 2, seed=4)
+    E, F, _ = h.energy_and_forces(good)
+    assert np.isfinite(E) and np.all(np.isfinite(F))

@@ -136,6 +136,38 @@ def test_server_error_stays_with_its_request(pkg, wl):
     sv.close()
 
 
+def test_server_failed_session_stays_with_itself(pkg, wl):
+    """A session whose structure has an element the potential lacks fails
+    its own requests only; a sound session and evaluations beside it run as
+    alone, and the failed session closes."""
+    bad = wl.cluster(20, seed=6)
+    bad.Z = bad.Z.copy()
+    bad.Z[2] = 79
+    good = wl.cluster(24, seed=7)
+    ev = _mix(wl, 3, seed=17)
+    want = _alone(pkg, "fp64", ev)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    p = np.zeros((good.nat, 3))
+    md = pkg.MdSystems(h, [good], [p], DT, skin=1.0)
+    want_md = md.step(3)
+    md.close()
+    sv = pkg.Server(PEROV_PARAMS, "fp64", lanes=2)
+    sb = sv.md_open(bad, np.zeros((bad.nat, 3)), DT, skin=1.0, masses=np.ones(bad.nat))
+    sg = sv.md_open(good, p, DT, skin=1.0)
+    ts = [sv.submit(s) for s in ev]
+    with pytest.raises(pkg.HdnnpError) as ei:
+        sv.md_result(sv.md_step(sb, 2), timeout=120)
+    assert ei.value.code == 4 and "Z=79" in str(ei.value)
+    ep, ek, _ = sv.md_result(sv.md_step(sg, 3), timeout=120)
+    assert ep == want_md[0][0] and ek == want_md[1][0]
+    for k, t in enumerate(ts):
+        E, F, D, _, _ = sv.result(t, timeout=120)
+        assert E == want[k][0] and np.array_equal(F, want[k][1]) and np.array_equal(D, want[k][2])
+    sv.md_close(sb)
+    sv.md_close(sg)
+    sv.close()
+
+
 def test_server_committee(pkg, wl, tmp_path):
     """A server of two potentials returns the committee mean and spread of a
     committee context."""
