@@ -129,6 +129,33 @@ def test_large_cell_against_oracle(pkg, wl, oracle, oracles):
     assert rmse <= 1e-5
 
 
+@pytest.mark.slow
+def test_large_cell_strain_derivative(pkg, wl):
+    """49,152-atom cell, FP64: dEdLat is the derivative of E with respect to
+    the lattice at fixed fractional coordinates, so for a homogeneous strain
+    (lat, R) -> (lat, R)(1 + eps M^T) the central difference of E equals
+    sum dEdLat * (lat M^T); and the stress of a rotation-invariant energy is
+    symmetric."""
+    s = wl.perovskite(16, 16, 16, seed=21)
+    h = handle(pkg, "perovskite")
+    E, F, L = h.energy_and_forces(s)
+    M = np.random.default_rng(5).normal(size=(3, 3))
+    eps = 1e-5
+
+    def strained(e):
+        t = s.copy()
+        A = np.eye(3) + e * M.T
+        t.lat, t.pos = s.lat @ A, s.pos @ A
+        return h.energy_and_forces(t, want_dEdLat=False)[0]
+
+    fd = (strained(eps) - strained(-eps)) / (2 * eps)
+    an = float(np.sum(L * (s.lat @ M.T)))
+    print(f"strain derivative: analytic {an:.10e} central difference {fd:.10e}", flush=True)
+    assert abs(fd - an) <= 1e-7 * abs(an)
+    S = pkg.dEdLat2Stress(s.lat, L)
+    np.testing.assert_allclose(S, S.T, rtol=0, atol=1e-9 * np.abs(S).max())
+
+
 def test_error_paths(pkg, wl):
     h = handle(pkg, "methylammonium")
     s = wl.methylammonium()
