@@ -250,6 +250,27 @@ __device__ __forceinline__ R wave_total(R v)
     return rocprim::warp_shuffle(v, 63, 64);
 }
 
+// FP32: the same steps with lanes that have no source reading 0 (the
+// identity of the add), so that each step is one v_add_f32 with a DPP
+// operand; the adds of lane 63 and their order are those above
+template <int CTRL>
+__device__ __forceinline__ float dpp_or_zero(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <>
+__device__ __forceinline__ float wave_total<float>(float v)
+{
+#pragma clang fp contract(off)
+    v += dpp_or_zero<0xb1>(v);
+    v += dpp_or_zero<0x4e>(v);
+    v += dpp_or_zero<0x124>(v);
+    v += dpp_or_zero<0x128>(v);
+    v += dpp_or_zero<0x142>(v);
+    v += dpp_or_zero<0x143>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 __device__ __forceinline__ int mbcnt(unsigned long long m)
