@@ -635,6 +635,13 @@ __device__ __forceinline__ R pow2z(R b, int sq)
     return b;
 }
 
+// floor(p / n) of a candidate index, for n <= 1024 and p < 1024 n: the float
+// quotient (p + 1/2) * (1/n) errs by less than 2e-7 (p/n) <= 2e-4, while
+// (p + 1/2)/n lies at least 1/(2n) >= 4.9e-4 from an integer, so truncation
+// gives the row exactly (checked for every such n and p, also with 1/n off
+// by one ulp); p - q n then takes the full-rate 24-bit multiply
+__device__ __forceinline__ int row_of(int p, float inv_n) { return (int)(((float)p + 0.5f) * inv_n); }
+
 // The candidate triplets [cb, ce) of one element-pair block of each atom,
 // compacted to those with r_jk < rc (the others give fc(r_jk) = 0). Same
 // element (segment sa, na): circulant rows, p = q*na + a is
@@ -654,10 +661,8 @@ __device__ __forceinline__ int compact(const Lds<R> &L, int cb, int ce, int cew,
         bool ok = false;
         int pk = 0;
         if (p < ce) {
-            int q = (int)(((float)p + 0.5f) * inv_na);
-            if (q * na > p) --q;
-            if ((q + 1) * na <= p) ++q;
-            const int a = p - q * na;
+            const int q = row_of(p, inv_na);
+            const int a = p - __mul24(q, na);
             int j, k;
             if (same) {
                 int b = a + q + 1;
@@ -877,11 +882,9 @@ __device__ __forceinline__ void circulant_pair(int p, int n, float inv_n, int &j
     const int rows = (n - 1) / 2;
     int d, jj;
     if (p < rows * n) {
-        int q = (int)(((float)p + 0.5f) * inv_n);
-        if (q * n > p) --q;
-        if ((q + 1) * n <= p) ++q;
+        const int q = row_of(p, inv_n);
         d = q + 1;
-        jj = p - q * n;
+        jj = p - __mul24(q, n);
     } else {
         d = n / 2;
         jj = p - rows * n;
