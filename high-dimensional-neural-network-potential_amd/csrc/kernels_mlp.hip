@@ -90,6 +90,8 @@ struct MlpShape {
     int total;              // LDS elements
 };
 
+constexpr int kKU = 8;   // matrix-core steps whose operands are loaded together
+
 // Ei_out[i] = E_i of this network; dEdG = scale * dE_i/dG, or added to dEdG
 // when accumulate is set (committee mean over several networks).
 template <typename R>
@@ -125,10 +127,18 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
     R *P1 = P0 + T * ld;
     R *Cb = P1 + T * ld;
     const R *W = weights<R>(m, e);
+    // the atoms of the tile rows, in LDS past the activations: the loops
+    // below read them there, so that their global loads do not wait on one
+    // another
+    int *rows = (int *)(smem + sizeof(R) * (size_t)sh.total);
+    const int atom = lane < tn ? w.perm[t0 + lane] : 0;
+    if (lane < T) rows[lane] = atom;
+    __syncthreads();
 
+#pragma unroll 8
     for (int t = lane; t < T * ld; t += 64) {
         const int a = t / ld, s = t - a * ld;
-        P0[t] = (a < tn && s < nsf) ? G[(size_t)w.perm[t0 + a] * ldg + s] : R(0);
+        P0[t] = (a < tn && s < nsf) ? G[(size_t)rows[a] * ldg + s] : R(0);
     }
     __syncthreads();
 
@@ -149,11 +159,18 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         for (int j0 = 0; j0 < nout; j0 += 16) {
             V acc = {0, 0, 0, 0};
             const int j = j0 + ri;
-            for (int k0 = 0; k0 < nin; k0 += 4) {
-                const int k = k0 + kq;
-                const R a = k < nin ? in[ri * ld + k] : R(0);
-                const R b = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
-                acc = Mma<R>::op(a, b, acc);
+            // kKU steps at a time, operands loaded first; steps past nin
+            // multiply zeros, which leaves the accumulator as it is
+            for (int k0 = 0; k0 < nin; k0 += 4 * kKU) {
+                R a[kKU], b[kKU];
+#pragma unroll
+                for (int u = 0; u < kKU; ++u) {
+                    const int k = k0 + 4 * u + kq;
+                    a[u] = k < nin ? in[ri * ld + k] : R(0);
+                    b[u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
             }
             if (j < nout) {
                 const R bj = bias[j];
@@ -174,7 +191,7 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         out = tmp;
     }
     // atomic energy (hdnnps.f90:526); the output layer has one node
-    if (lane < tn) Ei_out[w.perm[t0 + lane]] = (double)in[lane * ld] + m.einfo[e].atom_energy;
+    if (lane < tn) Ei_out[atom] = (double)in[lane * ld] + m.einfo[e].atom_energy;
 
     // backward, layer_backward (:116-158) with gradOut = 1
     R *g = out;
@@ -198,11 +215,16 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         for (int j0 = 0; j0 < ni; j0 += 16) {
             V acc = {0, 0, 0, 0};
             const int j = j0 + ri;
-            for (int k0 = 0; k0 < nout; k0 += 4) {
-                const int k = k0 + kq;
-                const R a = k < nout ? g[ri * ld + k] : R(0);
-                const R b = (k < nout && j < ni) ? wl[(size_t)j * nout + k] : R(0);
-                acc = Mma<R>::op(a, b, acc);
+            for (int k0 = 0; k0 < nout; k0 += 4 * kKU) {
+                R a[kKU], b[kKU];
+#pragma unroll
+                for (int u = 0; u < kKU; ++u) {
+                    const int k = k0 + 4 * u + kq;
+                    a[u] = k < nout ? g[ri * ld + k] : R(0);
+                    b[u] = (k < nout && j < ni) ? wl[(size_t)j * nout + k] : R(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
             }
             if (j < ni) {
 #pragma unroll
@@ -212,10 +234,11 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         __syncthreads();
         g = gn;
     }
+#pragma unroll 8
     for (int t = lane; t < T * ldg; t += 64) {
         const int a = t / ldg, s = t - a * ldg;
         if (a < tn) {
-            R *d = dEdG + (size_t)w.perm[t0 + a] * ldg + s;
+            R *d = dEdG + (size_t)rows[a] * ldg + s;
             const R v = s < nsf ? scale * g[a * ld + s] : R(0);
             *d = accumulate ? *d + v : v;
         }
@@ -228,7 +251,7 @@ size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw)
 {
     size_t tot = 2 * 16 * (size_t)odd(maxw);
     for (int l = 0; l < m.nlayers; ++l) tot += 16 * (size_t)odd(m.nodes[l]);
-    return tot * (fp64 ? 8 : 4);
+    return tot * (fp64 ? 8 : 4) + 16 * sizeof(int);
 }
 
 template <typename R>
@@ -247,8 +270,8 @@ void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G,
         off += 16 * sh.ldc[l];
     }
     sh.total = 2 * 16 * sh.ld + off;
-    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total, s, m, w, sh, G, dEdG, Ei,
-                       scale, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total + 16 * sizeof(int), s, m, w,
+                       sh, G, dEdG, Ei, scale, accumulate ? 1 : 0);
 }
 
 template void launch_mlp_t<double>(const DevModel &, Work &, int, int, const double *, double *, double *, double,
