@@ -134,17 +134,15 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
     const int atom = lane < tn ? w.perm[t0 + lane] : 0;
     if (lane < T) rows[lane] = atom;
     __syncthreads();
+    // lane (ri, kq) takes operand rows from the atom of tile row ri
+    const R *Grow = G + (size_t)rows[ri] * ldg;
+    const bool live = ri < tn;
 
-#pragma unroll 8
-    for (int t = lane; t < T * ld; t += 64) {
-        const int a = t / ld, s = t - a * ld;
-        P0[t] = (a < tn && s < nsf) ? G[(size_t)rows[a] * ldg + s] : R(0);
-    }
-    __syncthreads();
-
-    // forward, layer_forward (neuralNetworks.f90:94-114): c = W x + b, y = act(c)
+    // forward, layer_forward (neuralNetworks.f90:94-114): c = W x + b, y = act(c);
+    // the first layer reads the symmetry functions from global memory, the
+    // others the activations of the layer before from LDS
     const R *Wl[kMaxLay];
-    R *in = P0, *out = P1;
+    R *in = P1, *out = P0;
     int nin = nsf;
     const R *wp = W;
 #pragma unroll
@@ -166,7 +164,7 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
 #pragma unroll
                 for (int u = 0; u < kKU; ++u) {
                     const int k = k0 + 4 * u + kq;
-                    a[u] = k < nin ? in[ri * ld + k] : R(0);
+                    a[u] = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
                     b[u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
                 }
 #pragma unroll
@@ -193,7 +191,8 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
     // atomic energy (hdnnps.f90:526); the output layer has one node
     if (lane < tn) Ei_out[atom] = (double)in[lane * ld] + m.einfo[e].atom_energy;
 
-    // backward, layer_backward (:116-158) with gradOut = 1
+    // backward, layer_backward (:116-158) with gradOut = 1; the gradient with
+    // respect to the symmetry functions goes from the accumulators to dEdG
     R *g = out;
     if (lane < T) g[lane * ld] = 1;
     __syncthreads();
@@ -228,28 +227,45 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
             }
             if (j < ni) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) gn[Mma<R>::row(kq, r) * ld + j] = acc[r];
+                for (int r = 0; r < 4; ++r) {
+                    const int row = Mma<R>::row(kq, r);
+                    if (l > 0) {
+                        gn[row * ld + j] = acc[r];
+                    } else if (row < tn) {
+                        R *d = dEdG + (size_t)rows[row] * ldg + j;
+                        const R v = scale * acc[r];
+                        *d = accumulate ? *d + v : v;
+                    }
+                }
             }
         }
         __syncthreads();
         g = gn;
     }
-#pragma unroll 8
-    for (int t = lane; t < T * ldg; t += 64) {
-        const int a = t / ldg, s = t - a * ldg;
-        if (a < tn) {
-            R *d = dEdG + (size_t)rows[a] * ldg + s;
-            const R v = s < nsf ? scale * g[a * ld + s] : R(0);
-            *d = accumulate ? *d + v : v;
+    // the padding columns nsf..nsfmax-1 of the tile's rows
+    if (!accumulate) {
+        const int pad = ldg - nsf;
+        for (int t = lane; t < T * pad; t += 64) {
+            const int a = t / pad;
+            if (a < tn) dEdG[(size_t)rows[a] * ldg + nsf + (t - a * pad)] = R(0);
         }
     }
 }
 
 static int odd(int x) { return x | 1; }
 
-size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw)
+// the two activation buffers hold the hidden layers only (the symmetry
+// functions and their gradient stay in global memory)
+static int hidden_width(const DevModel &m)
 {
-    size_t tot = 2 * 16 * (size_t)odd(maxw);
+    int w = 1;
+    for (int l = 0; l < m.nlayers; ++l) w = w > m.nodes[l] ? w : m.nodes[l];
+    return w;
+}
+
+size_t mlp_lds_bytes(bool fp64, const DevModel &m, int)
+{
+    size_t tot = 2 * 16 * (size_t)odd(hidden_width(m));
     for (int l = 0; l < m.nlayers; ++l) tot += 16 * (size_t)odd(m.nodes[l]);
     return tot * (fp64 ? 8 : 4) + 16 * sizeof(int);
 }
@@ -260,7 +276,8 @@ void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G,
 {
     MlpShape sh{};
     sh.nlayers = m.nlayers;
-    sh.ld = odd(maxw);
+    sh.ld = odd(hidden_width(m));
+    (void)maxw;
     int off = 0;
     for (int l = 0; l < m.nlayers; ++l) {
         sh.nodes[l] = m.nodes[l];
