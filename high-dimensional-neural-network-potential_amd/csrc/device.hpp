@@ -148,15 +148,19 @@ __host__ __device__ inline void unpack_shift(int p, int &x, int &y, int &z)
 }
 
 // ---- launchers (kernels_nl.hip) ----
-void launch_map_elements(const DevModel &m, Work &w, hipStream_t s);
 void launch_bbox(const double *pos, const int *off, const int *which, int nwhich, double *out, hipStream_t s);
-void launch_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells, hipStream_t s);
-void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s);
-void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_rev(const Geo &g, Work &w, hipStream_t s);
-void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s);
 void launch_fill_int(int *p, int v, long n, hipStream_t s);
+// per-atom set-up of a build: element ids and flags, structure ids of atoms
+// and cells, wrapped positions and cell keys, the identity order
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s);
+// cell_start / cell_end from the key-sorted atoms
+void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
+// perm (atoms by element, index order within one), elem_off, elem_count;
+// chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk
+int elem_sort_chunks(int nat);
+void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s);
 
 // ---- launchers (kernels_sf.hip); R = double or float ----
 template <typename R>

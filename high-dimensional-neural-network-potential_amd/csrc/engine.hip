@@ -102,12 +102,6 @@ static void get_ncells(const double *L, double rc, int n[3])
     n[2] = (int)std::ceil(rc / std::fabs(p2));
 }
 
-__global__ void k_iota(int *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = i;
-}
-
 // ---------------------------------------------------------------------------
 
 Engine::Engine(std::vector<HostModel> members, int precision) : precision_(precision)
@@ -145,8 +139,8 @@ std::vector<DevMem *> Engine::work_buffers()
 {
     return {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
             &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
-            &iota_, &cell_start_, &cell_end_, &count_, &perm_, &eidx_sorted_, &Ei_,
-            &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_};
+            &iota_, &cell_start_, &cell_end_, &count_, &perm_, &Ei_,
+            &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &elem_chunks_};
 }
 
 size_t Engine::device_bytes() const
@@ -599,7 +593,6 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     iota_.ensure(4 * (size_t)nat);
     count_.ensure(4 * (size_t)nat);
     perm_.ensure(4 * (size_t)nat);
-    eidx_sorted_.ensure(4 * (size_t)nat);
     Ei_.ensure(8 * (size_t)nat * dmm_.size());
     vir_.ensure(72 * (size_t)nat);
     sid_.ensure(4 * (size_t)nat);
@@ -669,7 +662,6 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.result = res_.as<double>();
 
     HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
-    launch_map_elements(dm_, w, s);
 
     Geo g{};
     g.sg = geo_.as<StructGeo>();
@@ -677,24 +669,14 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     g.nstruct = nstruct;
     g.ncell = ncell;
     g.nat = nat;
-    if (nstruct == 1) {
-        launch_fill_int(w.sid, 0, nat, s);
-        launch_fill_int(w.csid, 0, ncell, s);
-    } else {
-        launch_segment_ids(w.sid, nat, g.sg, nstruct, false, s);
-        launch_segment_ids(w.csid, ncell, g.sg, nstruct, true, s);
-    }
     int bits = 1;
     while ((1L << bits) < ncell) ++bits;
 
-    launch_wrap_bin(g, w, s);
-    hipLaunchKernelGGL(k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, iota_.as<int>(), nat);
-    size_t b1 = 0, b3 = 0;
+    launch_prep(dm_, g, w, iota_.as<int>(), s);
+    size_t b1 = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
-                                              w.perm, nat, 0, 4, s));
-    cub_tmp_.ensure(std::max(b1, b3));
+    cub_tmp_.ensure(b1);
     size_t bt = cub_tmp_.bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                               w.cell_atoms, nat, 0, bits, s));
@@ -702,12 +684,10 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     cell_end_.ensure(4 * (size_t)ncell);
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
-    launch_cell_range(g, w, ncell, s);
+    launch_cell_bounds(w, ncell, s);
     // atoms sorted by element (MLP tiles); needs no host round trip
-    bt = cub_tmp_.bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.eidx, eidx_sorted_.as<int>(), iota_.as<int>(),
-                                              w.perm, nat, 0, 4, s));
-    launch_elem_bounds(w, eidx_sorted_.as<int>(), s);
+    elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
+    launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[1], s));
 

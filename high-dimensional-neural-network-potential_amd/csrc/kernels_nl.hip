@@ -54,26 +54,6 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
-// Element rank of each atom. Unknown Z and elements the model cannot
-// evaluate are flagged (rare, so the atomics are cheap); such atoms get a
-// usable element so that every kernel stays in bounds, and the host discards
-// the results.
-__global__ __launch_bounds__(256) void k_map_elements(DevModel m, Work w)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.nat) return;
-    const int z = w.Z[i];
-    const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
-    if (e == -1) atomicMax(&w.flags[0], z > 0 ? z : 1);
-    if (e <= -2) atomicMax(&w.flags[3], z);
-    w.eidx[i] = e < 0 ? m.fallback_el : e;
-}
-
-void launch_map_elements(const DevModel &m, Work &w, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_map_elements, dim3((w.nat + 255) / 256), dim3(256), 0, s, m, w);
-}
-
 // Bounding boxes of the raw positions of nonperiodic structures (artificial
 // lattice, neighborLists.f90:131-138): block b covers structure which[b],
 // atoms [off[s], off[s+1]); out[6b..6b+5] = min x, y, z, max x, y, z.
@@ -110,76 +90,190 @@ void launch_bbox(const double *pos, const int *off, const int *which, int nwhich
     if (nwhich > 0) hipLaunchKernelGGL(k_bbox, dim3(nwhich), dim3(256), 0, s, pos, off, which, out);
 }
 
-// Structure of each atom (cells = false) or each cell (cells = true) of a
-// batch: the last structure whose first atom / cell is <= x.
-__global__ __launch_bounds__(256) void k_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells)
+// inclusive prefix sum over the lanes of a wavefront
+__device__ inline int wave_incl_scan_int(int v, int lane)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// The structure of atom x (cells = false) or cell x (cells = true) of a
+// batch: the last structure whose first atom / cell is <= x.
+__device__ inline int segment_of(const StructGeo *sg, int nstruct, int x, bool cells)
+{
     int lo = 0, hi = nstruct - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if ((cells ? sg[mid].cell_off : sg[mid].atom_off) <= x) lo = mid;
         else hi = mid - 1;
     }
-    out[x] = lo;
+    return lo;
 }
 
-void launch_segment_ids(int *out, int n, const StructGeo *sg, int nstruct, bool cells, hipStream_t s)
+// The per-atom set-up of a build in one pass (thread x also covers cell x):
+// - the structure of each cell;
+// - the element rank of each atom. Unknown Z and elements the model cannot
+//   evaluate are flagged (rare, so the atomics are cheap); such atoms get a
+//   usable element so that every kernel stays in bounds, and the host
+//   discards the results;
+// - the structure of each atom and the identity order (the sorts' values);
+// - wrap into the list frame and bin: periodic input is wrapped twice (as
+//   hdnnps.f90:381 and neighborLists.f90:156 do); nonperiodic input is
+//   shifted by the artificial origin and wrapped once in the artificial box.
+//   Cell index int(s*ngrid) (neighborLists.f90:161-166), clamped into the
+//   grid, offset by the structure's first cell.
+__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota)
 {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_segment_ids, dim3((n + 255) / 256), dim3(256), 0, s, out, n, sg, nstruct, cells);
-}
-
-// Wrap into the list frame and bin: periodic input is wrapped twice (as
-// hdnnps.f90:381 and neighborLists.f90:156 do); nonperiodic input is shifted
-// by the artificial origin and wrapped once in the artificial box. Cell index
-// int(s*ngrid) (neighborLists.f90:161-166), clamped into the grid, offset by
-// the structure's first cell.
-__global__ __launch_bounds__(256) void k_wrap_bin(Geo g, Work w)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.nat) return;
-    const StructGeo &G = g.sg[w.sid[i]];
-    double x = w.pos[3 * i + 0], y = w.pos[3 * i + 1], z = w.pos[3 * i + 2];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < g.ncell) w.csid[x] = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, true);
+    if (x >= w.nat) return;
+    const int z = w.Z[x];
+    const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
+    if (e == -1) atomicMax(&w.flags[0], z > 0 ? z : 1);
+    if (e <= -2) atomicMax(&w.flags[3], z);
+    w.eidx[x] = e < 0 ? m.fallback_el : e;
+    const int sid = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, false);
+    w.sid[x] = sid;
+    iota[x] = x;
+    const StructGeo &G = g.sg[sid];
+    double px = w.pos[3 * x + 0], py = w.pos[3 * x + 1], pz = w.pos[3 * x + 2];
     if (G.periodic) {
-        wrap1(G.lat, G.inv, x, y, z);
-        wrap1(G.lat, G.inv, x, y, z);
+        wrap1(G.lat, G.inv, px, py, pz);
+        wrap1(G.lat, G.inv, px, py, pz);
     } else {
-        x = x - G.origin[0];
-        y = y - G.origin[1];
-        z = z - G.origin[2];
-        wrap1(G.lat, G.inv, x, y, z);
+        px = px - G.origin[0];
+        py = py - G.origin[1];
+        pz = pz - G.origin[2];
+        wrap1(G.lat, G.inv, px, py, pz);
     }
-    w.wpos[i] = make_double4(x, y, z, 0.0);
+    w.wpos[x] = make_double4(px, py, pz, 0.0);
     double s0, s1, s2;
-    mv3(G.inv, x, y, z, s0, s1, s2);
+    mv3(G.inv, px, py, pz, s0, s1, s2);
     int c0 = (int)(s0 * G.ng[0]), c1 = (int)(s1 * G.ng[1]), c2 = (int)(s2 * G.ng[2]);
     c0 = min(max(c0, 0), G.ng[0] - 1);
     c1 = min(max(c1, 0), G.ng[1] - 1);
     c2 = min(max(c2, 0), G.ng[2] - 1);
-    w.cellkey[i] = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
+    w.cellkey[x] = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
 }
 
-void launch_wrap_bin(const Geo &g, Work &w, hipStream_t s)
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_wrap_bin, dim3((w.nat + 255) / 256), dim3(256), 0, s, g, w);
+    const int n = w.nat > g.ncell ? w.nat : g.ncell;
+    if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, m, g, w, iota);
 }
 
-__global__ __launch_bounds__(256) void k_cell_range(Work w)
+// first position q in [0, n) with keys[q] >= v
+__device__ inline int lower_bound(const int *keys, int n, int v)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= w.nat) return;
-    const int key = w.cellkey_sorted[q];
-    if (q == 0 || w.cellkey_sorted[q - 1] != key) w.cell_start[key] = q;
-    if (q == w.nat - 1 || w.cellkey_sorted[q + 1] != key) w.cell_end[key] = q + 1;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
 }
 
-void launch_cell_range(const Geo &g, Work &w, int ncell, hipStream_t s)
+// The atoms of cell c are cell_atoms[cell_start[c] .. cell_end[c]) of the
+// key-sorted order (an empty cell has start = end).
+__global__ __launch_bounds__(256) void k_cell_bounds(Work w, int ncell)
 {
-    launch_fill_int(w.cell_start, 0, ncell, s);
-    launch_fill_int(w.cell_end, 0, ncell, s);
-    hipLaunchKernelGGL(k_cell_range, dim3((w.nat + 255) / 256), dim3(256), 0, s, w);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    w.cell_start[c] = lower_bound(w.cellkey_sorted, w.nat, c);
+    w.cell_end[c] = lower_bound(w.cellkey_sorted, w.nat, c + 1);
+}
+
+void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
+{
+    if (ncell > 0) hipLaunchKernelGGL(k_cell_bounds, dim3((ncell + 255) / 256), dim3(256), 0, s, w, ncell);
+}
+
+// Atoms ordered by element, in index order within an element (the tiles of
+// the MLP), by a stable counting sort over chunks of kSortChunk atoms:
+// per-chunk counts, their offsets, then each chunk places its atoms.
+constexpr int kSortChunk = 1024;   // atoms per chunk: 4 rounds of 256 threads
+
+__global__ __launch_bounds__(256) void k_elem_count(Work w, int *chunk_cnt)
+{
+    __shared__ int cnt[kMaxEl];
+    if (threadIdx.x < kMaxEl) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int a0 = blockIdx.x * kSortChunk;
+    for (int r = 0; r < kSortChunk; r += 256) {
+        const int i = a0 + r + (int)threadIdx.x;
+        if (i < w.nat) atomicAdd(&cnt[w.eidx[i]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxEl) chunk_cnt[blockIdx.x * kMaxEl + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// one block of kMaxEl wavefronts: wave e scans element e's chunk counts into
+// chunk offsets; then elem_count[e], and elem_off[e] = first position of
+// element e (elem_off[kMaxEl] = nat)
+__global__ __launch_bounds__(64 * kMaxEl) void k_elem_offsets(Work w, int *chunk_cnt, int nchunk)
+{
+    __shared__ int tot[kMaxEl];
+    const int e = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int base = 0;
+    for (int c0 = 0; c0 < nchunk; c0 += 64) {
+        const int c = c0 + lane;
+        const int v = c < nchunk ? chunk_cnt[c * kMaxEl + e] : 0;
+        const int incl = wave_incl_scan_int(v, lane);
+        if (c < nchunk) chunk_cnt[c * kMaxEl + e] = base + incl - v;
+        base += __shfl(incl, 63);
+    }
+    if (lane == 0) tot[e] = base;
+    __syncthreads();
+    if (threadIdx.x <= kMaxEl) {
+        int off = 0;
+        for (int f = 0; f < (int)threadIdx.x; ++f) off += tot[f];
+        w.elem_off[threadIdx.x] = off;
+        if (threadIdx.x < kMaxEl) w.elem_count[threadIdx.x] = tot[threadIdx.x];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_elem_place(Work w, const int *chunk_off, int nel)
+{
+    __shared__ int wcnt[4][kMaxEl];
+    __shared__ int base[kMaxEl];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    if (threadIdx.x < kMaxEl) base[threadIdx.x] = w.elem_off[threadIdx.x] + chunk_off[blockIdx.x * kMaxEl + threadIdx.x];
+    const unsigned long long below = (1ull << lane) - 1;
+    const int a0 = blockIdx.x * kSortChunk;
+    for (int r = 0; r < kSortChunk; r += 256) {
+        const int i = a0 + r + (int)threadIdx.x;
+        const int e = i < w.nat ? w.eidx[i] : -1;
+        int rank = 0;
+        for (int f = 0; f < nel; ++f) {
+            const unsigned long long mk = __ballot(e == f);
+            if (e == f) rank = __popcll(mk & below);
+            if (lane == 0) wcnt[wave][f] = __popcll(mk);
+        }
+        __syncthreads();                           // wcnt of this round, base of the rounds before
+        if (e >= 0) {
+            int pos = base[e] + rank;
+            for (int v = 0; v < wave; ++v) pos += wcnt[v][e];
+            w.perm[pos] = i;
+        }
+        __syncthreads();                           // every lane has read base and wcnt
+        if (threadIdx.x < nel) base[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] +
+                                                    wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
+        __syncthreads();                           // base updated before the next round's wcnt
+    }
+}
+
+int elem_sort_chunks(int nat) { return (nat + kSortChunk - 1) / kSortChunk; }
+
+void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s)
+{
+    const int nchunk = elem_sort_chunks(w.nat);
+    if (nchunk > 0) hipLaunchKernelGGL(k_elem_count, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt);
+    hipLaunchKernelGGL(k_elem_offsets, dim3(1), dim3(64 * kMaxEl), 0, s, w, chunk_cnt, nchunk);
+    if (nchunk > 0) hipLaunchKernelGGL(k_elem_place, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt, nel);
 }
 
 // One stencil cell of atom i: wrapped cell id (within the structure's grid)
@@ -234,15 +328,6 @@ __device__ inline bool stencil_cell(const StructGeo &g, int key, int s, StencilC
     }
     c.cell = (cx * g.ng[1] + cy) * g.ng[2] + cz;
     return true;
-}
-
-__device__ inline int wave_incl_scan_int(int v, int lane)
-{
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(v, o);
-        if (lane >= o) v += y;
-    }
-    return v;
 }
 
 // Neighbour lists of the atoms of one grid cell per wave (one-wave blocks).
@@ -413,31 +498,6 @@ void launch_nl_rev(const Geo &g, Work &w, hipStream_t s)
 {
     const long slots = (long)w.nat * w.cap;
     hipLaunchKernelGGL(k_nl_rev, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, g, w);
-}
-
-// Element segments of the element-sorted atom order: elem_off[e] = first
-// position whose element is >= e (binary search), elem_count = differences.
-__global__ void k_elem_bounds(Work w, const int *sorted)
-{
-    __shared__ int off[kMaxEl + 1];
-    const int e = threadIdx.x;
-    if (e <= kMaxEl) {
-        int lo = 0, hi = w.nat;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sorted[mid] < e) lo = mid + 1;
-            else hi = mid;
-        }
-        off[e] = lo;
-        w.elem_off[e] = lo;
-    }
-    __syncthreads();
-    if (e < kMaxEl) w.elem_count[e] = off[e + 1] - off[e];
-}
-
-void launch_elem_bounds(Work &w, const int *sorted_elem, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_elem_bounds, dim3(1), dim3(64), 0, s, w, sorted_elem);
 }
 
 }  // namespace hdnnp
