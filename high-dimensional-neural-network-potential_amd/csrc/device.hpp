@@ -168,9 +168,9 @@ void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga
 template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, const R *dEdG, R *gent,
                     hipStream_t s);
-size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad);
+size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad);
 // atoms per wavefront the SF kernels use for a requested tile (1, 2 or 4)
-int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max);
+int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max, int nel);
 // ---- kernels_mlp.hip: 16 atoms per wavefront, maxw = widest layer/input ----
 template <typename R>
 void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, double *Ei, R scale,

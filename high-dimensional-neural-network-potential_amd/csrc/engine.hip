@@ -405,7 +405,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     gent_.ensure(sizeof(R) * 3 * (size_t)nat * w.cap);
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
-    const int ga = sf_tile(sizeof(R) == 8, w.cap, tile_, dm_.nsfmax, dm_.g4_max);
+    const int ga = sf_tile(sizeof(R) == 8, w.cap, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
     launch_sf_values<R>(dm_, g, w, w.cap, ga, G, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
@@ -562,8 +562,8 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     const bool fp64 = precision_ == 0;
     int lds_lim = 0;
     for (int c = 64; c <= 1024; c *= 2)
-        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, true) <= 65536 &&
-            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, false) <= 65536)
+        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, true) <= 65536 &&
+            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, false) <= 65536)
             lds_lim = c;
     if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
     long cell_off = 0, ps_total = 0;
@@ -926,7 +926,7 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
             tile_ = cand[c];
             evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);   // warm-up
             double t = 0;
-            if (sf_tile(precision_ == 0, last_cap_, cand[c], dm_.nsfmax, dm_.g4_max) == cand[c]) {
+            if (sf_tile(precision_ == 0, last_cap_, cand[c], dm_.nsfmax, dm_.g4_max, dm_.nel) == cand[c]) {
                 for (int rep = 0; rep < 3; ++rep) {
                     evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);
                     const double v = ms_[2] + ms_[4];

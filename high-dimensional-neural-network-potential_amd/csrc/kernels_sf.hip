@@ -319,7 +319,7 @@ struct LdsOffsets {
 };
 
 template <typename R>
-__host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, bool grad)
+__host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, int nel, bool grad)
 {
     LdsOffsets o{};
     size_t off = 0;
@@ -340,7 +340,7 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
         o.sp0 = take(sizeof(typename Vec4<R>::T) * (size_t)g4max);
         o.sp1 = take(sizeof(R) * (size_t)g4max);
         o.smeta = take(sizeof(int) * (size_t)g4max);
-        o.btab = take(sizeof(I2) * kMaxEl * kMaxEl);
+        o.btab = take(sizeof(I2) * (size_t)nel * (size_t)nel);   // block f*nel+h, f <= h
     }
     o.total = off;
     return o;
@@ -348,15 +348,15 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
 
 // bytes of dynamic LDS of a wavefront (GA atom regions)
 template <typename R>
-__host__ __device__ inline size_t lds_bytes(int nmax, int ga, int nsfmax, int g4max, bool grad)
+__host__ __device__ inline size_t lds_bytes(int nmax, int ga, int nsfmax, int g4max, int nel, bool grad)
 {
-    return ga * lds_offsets<R>(nmax, nsfmax, g4max, grad).total;
+    return ga * lds_offsets<R>(nmax, nsfmax, g4max, nel, grad).total;
 }
 
 template <typename R>
-__device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, bool grad, LDS char *base)
+__device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int nel, bool grad, LDS char *base)
 {
-    const LdsOffsets o = lds_offsets<R>(nmax, nsfmax, g4max, grad);
+    const LdsOffsets o = lds_offsets<R>(nmax, nsfmax, g4max, nel, grad);
     Lds<R> L{};
     L.a = (LDS typename Lds<R>::V *)(base + o.a);
     L.b = (LDS typename Lds<R>::V *)(base + o.b);
@@ -914,10 +914,10 @@ __device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, 
         const int sq = cld(m.g4_sq, x);
         L.smeta[u] = (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0);
     }
-    for (int t = lane; t < m.nel * kMaxEl; t += 64) {
-        const int f = t / kMaxEl, h = t - f * kMaxEl;
+    for (int t = lane; t < m.nel * m.nel; t += 64) {
+        const int f = t / m.nel, h = t - f * m.nel;
         int sx = 0, cnt = 0;
-        if (h < m.nel && f <= h) {
+        if (f <= h) {
             cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, sx, cnt);
             sx -= g4s;
         }
@@ -928,7 +928,7 @@ __device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, 
 // candidates [cb, ce) of the circulant enumeration, compacted to the
 // triplets whose block has G4 functions and r_jk < rc
 template <typename R>
-__device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int n, float inv_n, R rc2)
+__device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int n, float inv_n, R rc2, int nel)
 {
     const int lane = lane_id();
     int nv = 0;
@@ -940,7 +940,7 @@ __device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int
             int j, k;
             circulant_pair(p, n, inv_n, j, k);
             const int ej = L.el[j], ek = L.el[k];
-            const int blk = min(ej, ek) * kMaxEl + max(ej, ek);
+            const int blk = min(ej, ek) * nel + max(ej, ek);
             const typename Lds<R>::V vj = L.a[j], vk = L.a[k];
             const R ex = vk.x - vj.x, ey = vk.y - vj.y, ez = vk.z - vj.z;
             ok = L.btab[blk].y > 0 && ex * ex + ey * ey + ez * ez < rc2;
@@ -1028,8 +1028,8 @@ __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *
     const int i = tn < 0 ? t0 : (grp < tn ? w.perm[t0 + grp] : -1);
     const ElemInfo ei = cld_einfo(m.einfo, e);
     const int nsf = ei.nsf;
-    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, 0, false,
-                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, 0, false).total);
+    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, 0, m.nel, false,
+                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, 0, m.nel, false).total);
     for (int s = gl; s < nsf; s += GL) L.acc[s] = 0;
     stage<R, GL, NMAX, false>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, (R *)nullptr);
     __syncthreads();
@@ -1073,8 +1073,8 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
     const ElemInfo ei = cld_einfo(m.einfo, e);
     const int nsf = ei.nsf;
     const int so = ei.sf_off;
-    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, m.g4_max, true,
-                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, m.g4_max, true).total);
+    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, m.g4_max, m.nel, true,
+                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, m.g4_max, m.nel, true).total);
     // dE/dG of the unscaled G: the scaled G is (G - mean)/den
     for (int s = gl; s < nsf; s += GL)
         L.acc[s] = i >= 0 ? (R)((double)dEdG[(size_t)i * m.nsfmax + s] / m.sf_den[so + s]) : R(0);
@@ -1112,7 +1112,7 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
         const int np = n * (n - 1) / 2;
         const float inv_n = 1.0f / (float)max(n, 1);
         for (int cb = 0; cb < np; cb += kPairChunk) {
-            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk), n, inv_n, rce2));
+            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk), n, inv_n, rce2, m.nel));
             __syncthreads();
             g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
             __syncthreads();
@@ -1151,7 +1151,7 @@ void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga
 {
 #define HDNNP_SFV(N, A)                                                                            \
     hipLaunchKernelGGL((k_sf_values<R, N, A>), dim3(sf_blocks(m, w, A)), dim3(64),                  \
-                       lds_bytes<R>(N, A, m.nsfmax, 0, false), s, m, g, w, G)
+                       lds_bytes<R>(N, A, m.nsfmax, 0, m.nel, false), s, m, g, w, G)
     if (nmax <= 64) {
         if (ga == 4) HDNNP_SFV(64, 4);
         else if (ga == 2) HDNNP_SFV(64, 2);
@@ -1172,7 +1172,7 @@ void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, 
 {
 #define HDNNP_SFG(N, A)                                                                            \
     hipLaunchKernelGGL((k_sf_grad<R, N, A>), dim3(sf_blocks(m, w, A)), dim3(64),                    \
-                       lds_bytes<R>(N, A, m.nsfmax, m.g4_max, true), s, m, g, w, dEdG, gent)
+                       lds_bytes<R>(N, A, m.nsfmax, m.g4_max, m.nel, true), s, m, g, w, dEdG, gent)
     if (nmax <= 64) {
         if (ga == 4) HDNNP_SFG(64, 4);
         else if (ga == 2) HDNNP_SFG(64, 2);
@@ -1204,22 +1204,25 @@ int sf_bucket(int nmax)
 
 // LDS of one wavefront of the kernels serving nmax neighbours with one atom
 // per wavefront (this decides the staging capacity limit)
-size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, bool grad)
+size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad)
 {
     nmax = sf_bucket(nmax);
-    return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, grad) : lds_bytes<float>(nmax, 1, nsfmax, g4max, grad);
+    return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, nel, grad)
+                : lds_bytes<float>(nmax, 1, nsfmax, g4max, nel, grad);
 }
 
 // the tile the kernels use for a requested one: ga when that variant is built
 // for the staging capacity and its wavefront fits 64 KiB of LDS in both
 // passes, else the largest smaller one that does (1 always does)
-int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max)
+int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max, int nel)
 {
     const int b = sf_bucket(nmax);
     for (int a = ga >= 4 ? 4 : (ga >= 2 ? 2 : 1); a > 1; a /= 2) {
         if (b > 128) continue;
-        const size_t v = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, false) : lds_bytes<float>(b, a, nsfmax, g4max, false);
-        const size_t gr = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, true) : lds_bytes<float>(b, a, nsfmax, g4max, true);
+        const size_t v = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, nel, false)
+                              : lds_bytes<float>(b, a, nsfmax, g4max, nel, false);
+        const size_t gr = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, nel, true)
+                               : lds_bytes<float>(b, a, nsfmax, g4max, nel, true);
         if (v <= 65536 && gr <= 65536) return a;
     }
     return 1;
