@@ -38,7 +38,10 @@
 
 namespace hdnnp {
 
-constexpr int kPairChunk = 512;    // triplet candidates compacted per LDS chunk
+// triplet candidates compacted per LDS chunk: FP32 takes the smaller chunk
+// for one more wavefront per CU; FP64, bound by its staging arrays either
+// way, the larger one for fewer rounds
+template <typename R> constexpr int kPairChunk = sizeof(R) == 4 ? 256 : 512;
 
 // ---------------------------------------------------------------- math
 template <typename R> struct M;
@@ -299,7 +302,8 @@ struct Lds {
     typedef typename Vec4<R>::T V;
     LDS V *a;                     // (dx, dy, dz, 1/r) of each staged neighbour
     LDS V *b;                     // (r, fc, fc'/r, r^2)
-    LDS int *el, *ent;            // element rank and list entry
+    LDS unsigned char *el;        // element rank
+    LDS int *ent;                 // list entry
     LDS int *pairs;               // compacted triplets (j | k << 16)
     LDS int *seg;                 // [kMaxEl+1] element segments of the staged neighbours
     LDS double *gx, *gy, *gz;     // gradients: per-neighbour accumulators (FP64: ds_add_f64
@@ -330,9 +334,9 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
     };
     o.a = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
     o.b = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
-    o.el = take(sizeof(int) * (size_t)nmax);
+    o.el = take((size_t)nmax);
     o.ent = take(sizeof(int) * (size_t)nmax);
-    o.pairs = take(sizeof(int) * kPairChunk);
+    o.pairs = take(sizeof(int) * kPairChunk<R>);
     o.seg = take(sizeof(int) * (kMaxEl + 1));
     o.g = grad ? take(sizeof(double) * 3 * (size_t)nmax) : 0;
     o.acc = take(sizeof(R) * (size_t)nsfmax);
@@ -360,7 +364,7 @@ __device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int 
     Lds<R> L{};
     L.a = (LDS typename Lds<R>::V *)(base + o.a);
     L.b = (LDS typename Lds<R>::V *)(base + o.b);
-    L.el = (LDS int *)(base + o.el);
+    L.el = (LDS unsigned char *)(base + o.el);
     L.ent = (LDS int *)(base + o.ent);
     L.pairs = (LDS int *)(base + o.pairs);
     L.seg = (LDS int *)(base + o.seg);
@@ -581,7 +585,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
             L.a[pos] = va;
             L.b[pos] = vb;
             L.ent[pos] = off + q;
-            L.el[pos] = f;
+            L.el[pos] = (unsigned char)f;
         } else if (q < n && gent) {
             gent[3 * (size_t)(off + q) + 0] = 0;
             gent[3 * (size_t)(off + q) + 1] = 0;
@@ -818,7 +822,7 @@ __device__ __forceinline__ void g4_grad(const DevModel &m, const Lds<R> &L, int 
 }
 
 // All G4 blocks of the centre element e, for every atom of the wavefront:
-// per block, the candidates in chunks of kPairChunk are compacted, then the
+// per block, the candidates in chunks of kPairChunk<R> are compacted, then the
 // block's functions are evaluated kSfChunk at a time. GRAD selects pass B.
 template <typename R, int GL, bool GRAD>
 __device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, int e, R rce)
@@ -844,8 +848,8 @@ __device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, in
             const float inv_na = 1.0f / (float)max(na, 1);
             bool fast = true;    // every zeta a power of two
             for (int t = 0; t < tby; ++t) fast = fast && uni(cld(m.g4_sq, tbx + t)) >= 0;
-            for (int cb = 0; cb < npw; cb += kPairChunk) {
-                const int nv = compact<R, GL>(L, cb, min(np, cb + kPairChunk), min(npw, cb + kPairChunk), same, sa,
+            for (int cb = 0; cb < npw; cb += kPairChunk<R>) {
+                const int nv = compact<R, GL>(L, cb, min(np, cb + kPairChunk<R>), min(npw, cb + kPairChunk<R>), same, sa,
                                               na, inv_na, sb, nb, rce2);
                 const int nvw = GL == 64 ? uni(nv) : wave_max(nv);
                 __syncthreads();
@@ -1111,8 +1115,8 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
         const R rce2 = rce * rce, rrc = R(1) / rce;
         const int np = n * (n - 1) / 2;
         const float inv_n = 1.0f / (float)max(n, 1);
-        for (int cb = 0; cb < np; cb += kPairChunk) {
-            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk), n, inv_n, rce2, m.nel));
+        for (int cb = 0; cb < np; cb += kPairChunk<R>) {
+            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk<R>), n, inv_n, rce2, m.nel));
             __syncthreads();
             g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
             __syncthreads();
