@@ -303,7 +303,7 @@ struct Lds {
     LDS V *a;                     // (dx, dy, dz, 1/r) of each staged neighbour
     LDS V *b;                     // (r, fc, fc'/r, r^2)
     LDS unsigned char *el;        // element rank
-    LDS int *ent;                 // list entry
+    LDS unsigned short *ent;      // list slot within the atom's list (< cap <= 1024)
     LDS int *pairs;               // compacted triplets (j | k << 16)
     LDS int *seg;                 // [kMaxEl+1] element segments of the staged neighbours
     LDS double *gx, *gy, *gz;     // gradients: per-neighbour accumulators (FP64: ds_add_f64
@@ -335,7 +335,7 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
     o.a = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
     o.b = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
     o.el = take((size_t)nmax);
-    o.ent = take(sizeof(int) * (size_t)nmax);
+    o.ent = take(sizeof(unsigned short) * (size_t)nmax);
     o.pairs = take(sizeof(int) * kPairChunk<R>);
     o.seg = take(sizeof(int) * (kMaxEl + 1));
     o.g = grad ? take(sizeof(double) * 3 * (size_t)nmax) : 0;
@@ -365,7 +365,7 @@ __device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int 
     L.a = (LDS typename Lds<R>::V *)(base + o.a);
     L.b = (LDS typename Lds<R>::V *)(base + o.b);
     L.el = (LDS unsigned char *)(base + o.el);
-    L.ent = (LDS int *)(base + o.ent);
+    L.ent = (LDS unsigned short *)(base + o.ent);
     L.pairs = (LDS int *)(base + o.pairs);
     L.seg = (LDS int *)(base + o.seg);
     if (grad) {
@@ -584,7 +584,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
             vb.w = rr * rr;
             L.a[pos] = va;
             L.b[pos] = vb;
-            L.ent[pos] = off + q;
+            L.ent[pos] = (unsigned short)q;
             L.el[pos] = (unsigned char)f;
         } else if (q < n && gent) {
             gent[3 * (size_t)(off + q) + 0] = 0;
@@ -1130,7 +1130,7 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
     for (int t = 0; t < 9; ++t) v[t] = 0;
     for (int q = gl; q < n; q += GL) {
         const double gx = L.gx[q], gy = L.gy[q], gz = L.gz[q];
-        const size_t en = (size_t)L.ent[q];
+        const size_t en = (size_t)i * w.cap + L.ent[q];
         gent[3 * en + 0] = (R)gx;
         gent[3 * en + 1] = (R)gy;
         gent[3 * en + 2] = (R)gz;
