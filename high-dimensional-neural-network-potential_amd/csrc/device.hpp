@@ -104,9 +104,11 @@ struct Work {
     int *cell_start, *cell_end;
     // Neighbour entries of atom i occupy slots [i*cap, i*cap + count[i]) of
     // the entry arrays (cap a power of two, fixed for one evaluation), in the
-    // reference's list order: stencil cell, then ascending atom index.
-    int cap, cap_log2;
-    int *count;          // [nat] neighbours per atom (at most cap)
+    // reference's list order: stencil cell, then ascending atom index. An
+    // atom holds at most lim <= cap entries: lim is the staging size of the
+    // symmetry-function kernels (64, 96, 128, 256, 512 or 1024).
+    int cap, cap_log2, lim;
+    int *count;          // [nat] neighbours per atom (at most lim)
     int *ent_j;          // [nat*cap] neighbour atom
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)
     int *ent_s;          // [nat*cap] stencil-cell index of the entry

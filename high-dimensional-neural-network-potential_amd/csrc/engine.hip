@@ -405,8 +405,8 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     gent_.ensure(sizeof(R) * 3 * (size_t)nat * w.cap);
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
-    const int ga = sf_tile(sizeof(R) == 8, w.cap, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
-    launch_sf_values<R>(dm_, g, w, w.cap, ga, G, s);
+    const int ga = sf_tile(sizeof(R) == 8, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
+    launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
 
@@ -424,7 +424,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[4], s));
 
-    launch_sf_grad<R>(dm_, g, w, w.cap, ga, dEdG, gent, s);
+    launch_sf_grad<R>(dm_, g, w, w.lim, ga, dEdG, gent, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[5], s));
 
@@ -499,12 +499,27 @@ static double neighbour_estimate(const StructGeo &G, double rc)
 
 // entry capacity per atom: a power of two from 64 covering the estimate (or
 // the most neighbours this engine has seen), at most lim
+// staging sizes of the symmetry-function kernels (entries per atom)
+static const int kStage[] = {64, 96, 128, 256, 512, 1024};
+
+// the smallest staging size that holds `want` entries, at most `lim`
 static int entry_capacity(double est, int hint, int lim)
 {
     const double want = std::max((double)hint, std::min(1.2 * est + 8.0, 1e6));
-    int cap = 64;
-    while (cap < want && cap < lim) cap *= 2;
+    int cap = kStage[0];
+    for (int c : kStage) {
+        cap = c;
+        if (cap >= want || cap >= lim) break;
+    }
     return cap;
+}
+
+// the slot stride of a list holding `lim` entries per atom: a power of two
+static int slot_stride(int lim)
+{
+    int c = 64;
+    while (c < lim) c *= 2;
+    return c;
 }
 
 void Engine::footprint_add(Footprint &f, const StructIn &in, double rc_list) const
@@ -523,7 +538,7 @@ size_t Engine::footprint_bytes(const Footprint &f) const
     const size_t R = precision_ == 0 ? 8 : 4, nm = (size_t)committee_size();
     size_t nsfmax = 0;
     for (const ElementModel &em : model_.el) nsfmax = std::max(nsfmax, em.sfs.size());
-    const size_t cap = (size_t)entry_capacity(f.est, cap_hint_, 1024);
+    const size_t cap = (size_t)slot_stride(entry_capacity(f.est, cap_hint_, 1024));
     const size_t nat = (size_t)f.nat;
     size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
     b += nat * cap * (16 + 3 * R);                                                           // list slots
@@ -561,7 +576,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     hgeo_.resize(nstruct);
     const bool fp64 = precision_ == 0;
     int lds_lim = 0;
-    for (int c = 64; c <= 1024; c *= 2)
+    for (int c : kStage)
         if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, true) <= 65536 &&
             sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, false) <= 65536)
             lds_lim = c;
@@ -695,7 +710,8 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     // estimate from the mean density, rounded up to a staging size of the
     // symmetry-function kernels. An atom with more neighbours raises flags[1]
     // and the evaluation is repeated with room for it.
-    w.cap = entry_capacity(est, cap_hint_, lds_lim);
+    w.lim = entry_capacity(est, cap_hint_, lds_lim);
+    w.cap = slot_stride(w.lim);
     w.cap_log2 = 0;
     while ((1 << w.cap_log2) < w.cap) ++w.cap_log2;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
@@ -808,6 +824,7 @@ bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)
     if (hflags[2]) throw Error(6, "periodic image shift beyond +-127 cells");
     last_nat_ = nat;
     last_cap_ = w.cap;
+    last_lim_ = w.lim;
     last_total_ = -1;
     for (int k = 0; k < 6; ++k) {
         float ms = 0;
@@ -926,7 +943,7 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
             tile_ = cand[c];
             evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);   // warm-up
             double t = 0;
-            if (sf_tile(precision_ == 0, last_cap_, cand[c], dm_.nsfmax, dm_.g4_max, dm_.nel) == cand[c]) {
+            if (sf_tile(precision_ == 0, last_lim_, cand[c], dm_.nsfmax, dm_.g4_max, dm_.nel) == cand[c]) {
                 for (int rep = 0; rep < 3; ++rep) {
                     evaluate_batch_host(nstruct, nat, Z, pos, periodic, lat, E.data(), F.data(), nullptr);
                     const double v = ms_[2] + ms_[4];

@@ -175,7 +175,7 @@ class Engine {
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
     size_t pinned_bytes_ = 0;
-    int last_nat_ = 0, last_cap_ = 0;
+    int last_nat_ = 0, last_cap_ = 0, last_lim_ = 0;
     int64_t last_total_ = 0;   // -1: not yet summed
     int cap_hint_ = 0;         // most neighbours of one atom seen so far
 };

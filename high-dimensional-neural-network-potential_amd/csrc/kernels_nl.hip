@@ -417,7 +417,7 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
                         const int pos = na + (int)__builtin_amdgcn_mbcnt_hi(
                                                  (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
                         if (inf >> 25 & 1) atomicMax(&w.flags[2], 1);
-                        if (pos < w.cap) {
+                        if (pos < w.lim) {
                             const int q = ai * w.cap + pos;
                             w.ent_j[q] = j;
                             w.ent_shift[q] = inf & 0xFFFFFF;
@@ -443,8 +443,8 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
                 }
         }
         if (lane < Ag) {
-            w.count[ia[lane]] = min(n, w.cap);
-            if (n > w.cap) atomicMax(&w.flags[1], n);
+            w.count[ia[lane]] = min(n, w.lim);
+            if (n > w.lim) atomicMax(&w.flags[1], n);
         }
     }
 }

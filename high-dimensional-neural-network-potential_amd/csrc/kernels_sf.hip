@@ -1160,6 +1160,8 @@ void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga
         if (ga == 4) HDNNP_SFV(64, 4);
         else if (ga == 2) HDNNP_SFV(64, 2);
         else HDNNP_SFV(64, 1);
+    } else if (nmax <= 96 && ga == 1) {
+        HDNNP_SFV(96, 1);
     } else if (nmax <= 128) {
         if (ga == 4) HDNNP_SFV(128, 4);
         else if (ga == 2) HDNNP_SFV(128, 2);
@@ -1181,6 +1183,8 @@ void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, 
         if (ga == 4) HDNNP_SFG(64, 4);
         else if (ga == 2) HDNNP_SFG(64, 2);
         else HDNNP_SFG(64, 1);
+    } else if (nmax <= 96 && ga == 1) {
+        HDNNP_SFG(96, 1);
     } else if (nmax <= 128) {
         if (ga == 4) HDNNP_SFG(128, 4);
         else if (ga == 2) HDNNP_SFG(128, 2);
@@ -1199,9 +1203,12 @@ template void launch_sf_grad<float>(const DevModel &, const Geo &, Work &, int, 
                                     hipStream_t);
 
 // staging capacity of the kernel instance that serves nmax neighbours
+// the staging size of the one-atom-per-wavefront kernels for nmax entries
 int sf_bucket(int nmax)
 {
-    int b = 64;
+    if (nmax <= 64) return 64;
+    if (nmax <= 96) return 96;
+    int b = 128;
     while (b < nmax) b *= 2;
     return b;
 }
@@ -1220,9 +1227,10 @@ size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool gr
 // passes, else the largest smaller one that does (1 always does)
 int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max, int nel)
 {
-    const int b = sf_bucket(nmax);
+    // tiles of 2 and 4 atoms are built for staging sizes 64 and 128
+    const int b = sf_bucket(nmax) <= 64 ? 64 : sf_bucket(nmax) <= 128 ? 128 : 0;
     for (int a = ga >= 4 ? 4 : (ga >= 2 ? 2 : 1); a > 1; a /= 2) {
-        if (b > 128) continue;
+        if (b == 0) continue;
         const size_t v = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, nel, false)
                               : lds_bytes<float>(b, a, nsfmax, g4max, nel, false);
         const size_t gr = fp64 ? lds_bytes<double>(b, a, nsfmax, g4max, nel, true)
