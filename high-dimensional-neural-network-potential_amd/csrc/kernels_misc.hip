@@ -48,27 +48,29 @@ __global__ __launch_bounds__(64) void k_force_gather(Work w, const R *g)
 }
 
 // entries whose partner is missing (pair at the exact cutoff seen from one
-// side only): add their contribution to the neighbour directly. One thread
-// per entry slot; slot q belongs to atom q >> cap_log2.
+// side only): add their contribution to the neighbour directly. One
+// wavefront per atom (four per block), its lanes over the atom's entries.
 template <typename R>
 __global__ __launch_bounds__(256) void k_orphans(Work w, const R *g)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long)w.nat * w.cap) return;
-    const int i = (int)(q >> w.cap_log2);
-    if ((int)(q - ((long)i << w.cap_log2)) >= w.count[i] || w.rev[q] >= 0) return;
-    const int j = w.ent_j[q];
-    atomicAdd(&w.force[3 * j + 0], -(double)g[3 * (size_t)q + 0]);
-    atomicAdd(&w.force[3 * j + 1], -(double)g[3 * (size_t)q + 1]);
-    atomicAdd(&w.force[3 * j + 2], -(double)g[3 * (size_t)q + 2]);
+    const int i = (int)blockIdx.x * 4 + (int)threadIdx.x / 64;
+    if (i >= w.nat) return;
+    const int cnt = w.count[i];
+    for (int t = (int)threadIdx.x % 64; t < cnt; t += 64) {
+        const size_t q = (size_t)i * w.cap + t;
+        if (w.rev[q] >= 0) continue;
+        const int j = w.ent_j[q];
+        atomicAdd(&w.force[3 * j + 0], -(double)g[3 * q + 0]);
+        atomicAdd(&w.force[3 * j + 1], -(double)g[3 * q + 1]);
+        atomicAdd(&w.force[3 * j + 2], -(double)g[3 * q + 2]);
+    }
 }
 
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, hipStream_t s)
 {
     hipLaunchKernelGGL(k_force_gather<R>, dim3(w.nat), dim3(64), 0, s, w, gent);
-    const long slots = (long)w.nat * w.cap;
-    hipLaunchKernelGGL(k_orphans<R>, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, w, gent);
+    hipLaunchKernelGGL(k_orphans<R>, dim3((w.nat + 3) / 4), dim3(256), 0, s, w, gent);
 }
 template void launch_force_gather_t<double>(Work &, const double *, hipStream_t);
 template void launch_force_gather_t<float>(Work &, const float *, hipStream_t);

@@ -460,44 +460,45 @@ void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s)
 // without it, lists are sorted by (stencil cell, atom) and are searched
 // by bisection. The strict distance test is evaluated separately in each
 // direction, so a pair at the exact cutoff can lack its partner: rev = -1.
-// One thread per entry slot; slot q belongs to atom q >> cap_log2.
+// One wavefront per atom (four per block), its lanes over the atom's entries.
 __global__ __launch_bounds__(256) void k_nl_rev(Geo g, Work w)
 {
-    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (q >= (long)w.nat * w.cap) return;
-    const int i = (int)(q >> w.cap_log2);
-    if ((int)(q - ((long)i << w.cap_log2)) >= w.count[i]) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + (int)threadIdx.x / 64;
+    if (i >= w.nat) return;
+    const int cnt = w.count[i];
     const StructGeo &G = g.sg[w.sid[i]];
     const int S = G.S;
-    const int j = w.ent_j[q];
-    const int sp = S - 1 - w.ent_s[q];
-    const int bj = j * w.cap, nj = w.count[j];
-    int r = -1;
-    if (w.pos_start) {
-        for (int t = w.pos_start[G.ps_off + (long long)(j - G.atom_off) * S + sp]; t < nj; ++t) {
-            if (w.ent_s[bj + t] != sp) break;
-            if (w.ent_j[bj + t] == i) {
-                r = bj + t;
-                break;
+    for (int t = (int)threadIdx.x % 64; t < cnt; t += 64) {
+        const long q = (long)i * w.cap + t;
+        const int j = w.ent_j[q];
+        const int sp = S - 1 - w.ent_s[q];
+        const int bj = j * w.cap, nj = w.count[j];
+        int r = -1;
+        if (w.pos_start) {
+            for (int u = w.pos_start[G.ps_off + (long long)(j - G.atom_off) * S + sp]; u < nj; ++u) {
+                if (w.ent_s[bj + u] != sp) break;
+                if (w.ent_j[bj + u] == i) {
+                    r = bj + u;
+                    break;
+                }
             }
+        } else {
+            int lo = 0, hi = nj;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const int st = w.ent_s[bj + mid];
+                if (st < sp || (st == sp && w.ent_j[bj + mid] < i)) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < nj && w.ent_s[bj + lo] == sp && w.ent_j[bj + lo] == i) r = bj + lo;
         }
-    } else {
-        int lo = 0, hi = nj;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const int st = w.ent_s[bj + mid];
-            if (st < sp || (st == sp && w.ent_j[bj + mid] < i)) lo = mid + 1;
-            else hi = mid;
-        }
-        if (lo < nj && w.ent_s[bj + lo] == sp && w.ent_j[bj + lo] == i) r = bj + lo;
+        w.rev[q] = r;
     }
-    w.rev[q] = r;
 }
 
 void launch_nl_rev(const Geo &g, Work &w, hipStream_t s)
 {
-    const long slots = (long)w.nat * w.cap;
-    hipLaunchKernelGGL(k_nl_rev, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, g, w);
+    if (w.nat > 0) hipLaunchKernelGGL(k_nl_rev, dim3((w.nat + 3) / 4), dim3(256), 0, s, g, w);
 }
 
 }  // namespace hdnnp
