@@ -107,7 +107,7 @@ struct Work {
     // reference's list order: stencil cell, then ascending atom index. An
     // atom holds at most lim <= cap entries: lim is the staging size of the
     // symmetry-function kernels (64, 96, 128, 256, 512 or 1024).
-    int cap, cap_log2, lim;
+    int cap, lim;
     int *count;          // [nat] neighbours per atom (at most lim)
     int *ent_j;          // [nat*cap] neighbour atom
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)

@@ -712,8 +712,6 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     // and the evaluation is repeated with room for it.
     w.lim = entry_capacity(est, cap_hint_, lds_lim);
     w.cap = slot_stride(w.lim);
-    w.cap_log2 = 0;
-    while ((1 << w.cap_log2) < w.cap) ++w.cap_log2;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
     const size_t slots = (size_t)nat * w.cap;
     ent_j_.ensure(4 * slots);
