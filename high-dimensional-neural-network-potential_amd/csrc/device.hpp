@@ -109,6 +109,13 @@ struct Work {
     // symmetry-function kernels (64, 96, 128, 256, 512 or 1024).
     int cap, lim;
     int *count;          // [nat] neighbours per atom (at most lim)
+    // G4 triplets of each atom as the value pass compacts them (one atom per
+    // wavefront only): packed j | k << 10 | block << 20 over staged
+    // neighbours, at most tcap per atom; tcount[i] may exceed tcap, and the
+    // gradient pass then compacts that atom's triplets itself. trip = null:
+    // not written.
+    int *trip, *tcount;
+    int tcap;
     int *ent_j;          // [nat*cap] neighbour atom
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)
     int *ent_s;          // [nat*cap] stencil-cell index of the entry

@@ -140,7 +140,8 @@ std::vector<DevMem *> Engine::work_buffers()
     return {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
             &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
             &iota_, &cell_start_, &cell_end_, &count_, &perm_, &Ei_,
-            &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &elem_chunks_};
+            &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &elem_chunks_,
+            &trip_, &tcount_};
 }
 
 size_t Engine::device_bytes() const
@@ -406,6 +407,18 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
     const int ga = sf_tile(sizeof(R) == 8, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
+    // FP64 with one atom per wavefront: pass A hands its compacted triplets to
+    // pass B. (In FP32 the list's stores cost pass A about what they save
+    // pass B: its barriers wait for them.)
+    if (sizeof(R) == 8 && ga == 1 && (long)nat * w.tcap < (1L << 31)) {
+        trip_.ensure(4 * (size_t)nat * w.tcap);
+        tcount_.ensure(4 * (size_t)nat);
+        w.trip = trip_.as<int>();
+        w.tcount = tcount_.as<int>();
+    } else {
+        w.trip = nullptr;
+        w.tcount = nullptr;
+    }
     launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
@@ -514,6 +527,18 @@ static int entry_capacity(double est, int hint, int lim)
     return cap;
 }
 
+// triplets per atom the value pass keeps for the gradient pass: 0.4 want^2
+// (about 0.6 of the want^2/2 neighbour pairs lie within the cutoff of each
+// other), at most the pairs of lim neighbours; an atom with more falls back
+// to the gradient pass's own compaction
+static int triplet_capacity(double est, int lim)
+{
+    const double want = std::min(1.2 * est + 8.0, (double)lim);
+    const long pairs = (long)lim * (lim - 1) / 2;
+    const long t = std::min(pairs, std::max(256L, (long)(0.4 * want * want)));
+    return (int)((t + 63) / 64 * 64);
+}
+
 // the slot stride of a list holding `lim` entries per atom: a power of two
 static int slot_stride(int lim)
 {
@@ -542,6 +567,7 @@ size_t Engine::footprint_bytes(const Footprint &f) const
     const size_t nat = (size_t)f.nat;
     size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
     b += nat * cap * (16 + 3 * R);                                                           // list slots
+    if (R == 8) b += nat * (4 * (size_t)triplet_capacity(f.est, entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
     b += (size_t)f.ncell * 12;                                                               // cells
     if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
     b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure
@@ -712,6 +738,9 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     // and the evaluation is repeated with room for it.
     w.lim = entry_capacity(est, cap_hint_, lds_lim);
     w.cap = slot_stride(w.lim);
+    w.tcap = triplet_capacity(est, w.lim);
+    w.trip = nullptr;               // set by compute when the passes share triplets
+    w.tcount = nullptr;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
     const size_t slots = (size_t)nat * w.cap;
     ent_j_.ensure(4 * slots);
@@ -749,7 +778,7 @@ bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dE
 
 std::vector<char> Engine::launch_key() const
 {
-    const void *ptrs[] = {G_.p, dEdG_.p, gent_.p, chunks_.p, sc_.p, partial_.p, res_.p, misc_.p};
+    const void *ptrs[] = {G_.p, dEdG_.p, gent_.p, chunks_.p, sc_.p, partial_.p, res_.p, misc_.p, trip_.p, tcount_.p};
     const int ints[] = {prep_.nat, prep_.nchunk, (int)prep_.st.size(), precision_, prep_.valid ? 1 : 0, tile_};
     std::vector<char> k(sizeof(Work) + sizeof(Geo) + sizeof ptrs + sizeof ints);
     char *p = k.data();

@@ -160,6 +160,7 @@ class Engine {
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
     DevMem count_, perm_, Ei_, vir_, misc_;
     DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_, elem_chunks_;
+    DevMem trip_, tcount_;   // triplets pass A hands to pass B
     DevMem geo_, sid_, csid_, chunks_, sc_, res_, partial_, bbox_;
     std::vector<StructGeo> hgeo_;
     struct Prepared {

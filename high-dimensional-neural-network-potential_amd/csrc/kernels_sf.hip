@@ -824,9 +824,13 @@ __device__ __forceinline__ void g4_grad(const DevModel &m, const Lds<R> &L, int 
 // All G4 blocks of the centre element e, for every atom of the wavefront:
 // per block, the candidates in chunks of kPairChunk<R> are compacted, then the
 // block's functions are evaluated kSfChunk at a time. GRAD selects pass B.
+// Pass A with one atom per wavefront also writes the compacted triplets to
+// tl (at most tcap) for pass B; it returns how many there are.
 template <typename R, int GL, bool GRAD>
-__device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, int e, R rce)
+__device__ __forceinline__ int g4_blocks(const DevModel &m, const Lds<R> &L, int e, R rce, int *tl = nullptr,
+                                         int tcap = 0)
 {
+    int tbase = 0;
     const R rce2 = rce * rce, rrc = R(1) / rce;
     for (int f = 0; f < m.nel; ++f) {
         for (int h = f; h < m.nel; ++h) {
@@ -853,6 +857,14 @@ __device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, in
                                               na, inv_na, sb, nb, rce2);
                 const int nvw = GL == 64 ? uni(nv) : wave_max(nv);
                 __syncthreads();
+                if (!GRAD && GL == 64 && sizeof(R) == 8 && tl) {   // FP64 only (the engine passes none in FP32)
+                    const int blk = f * m.nel + h;
+                    for (int q = lane_id(); q < nvw; q += 64) {
+                        const int pk = L.pairs[q];
+                        if (tbase + q < tcap) tl[tbase + q] = (pk & 0xFFFF) | ((pk >> 16) << 10) | (blk << 20);
+                    }
+                    tbase += nvw;
+                }
                 for (int c0 = 0; c0 < tby; c0 += kSfChunk) {
                     const int x0 = tbx + c0;
 #define HDNNP_G4(NC)                                                                       \
@@ -872,6 +884,7 @@ __device__ __forceinline__ void g4_blocks(const DevModel &m, const Lds<R> &L, in
             }
         }
     }
+    return tbase;
 }
 
 // ---------------------------------------------------------------- pass B, all pairs
@@ -1053,7 +1066,9 @@ __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *
         L.acc[cld(m.g2_s, x)] += sum;
     }
     // G4, symmetryFunctions.f90:231-305
-    g4_blocks<R, GL, false>(m, L, e, rce);
+    int *tl = (sizeof(R) == 8 && GA == 1 && w.trip) ? w.trip + (size_t)i * w.tcap : nullptr;
+    const int ntrip = g4_blocks<R, GL, false>(m, L, e, rce, tl, w.tcap);
+    if (sizeof(R) == 8 && tl && gl == 0) w.tcount[i] = ntrip;
     __syncthreads();
     // centre and scale, hdnnps.f90:471-481
     if (i < 0) return;
@@ -1113,13 +1128,26 @@ __global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const
         grad_tables<R>(m, L, e, ei.g4_start, ei.g4_count);
         __syncthreads();
         const R rce2 = rce * rce, rrc = R(1) / rce;
-        const int np = n * (n - 1) / 2;
-        const float inv_n = 1.0f / (float)max(n, 1);
-        for (int cb = 0; cb < np; cb += kPairChunk<R>) {
-            const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk<R>), n, inv_n, rce2, m.nel));
-            __syncthreads();
-            g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
-            __syncthreads();
+        const int tc = (sizeof(R) == 8 && w.trip) ? uni(w.tcount[i]) : -1;
+        if (tc >= 0 && tc <= w.tcap) {
+            // the triplets pass A compacted
+            const int *tl = w.trip + (size_t)i * w.tcap;
+            for (int cb = 0; cb < tc; cb += kPairChunk<R>) {
+                const int nv = min(kPairChunk<R>, tc - cb);
+                for (int q = gl; q < nv; q += 64) L.pairs[q] = tl[cb + q];
+                __syncthreads();
+                g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
+                __syncthreads();
+            }
+        } else {
+            const int np = n * (n - 1) / 2;
+            const float inv_n = 1.0f / (float)max(n, 1);
+            for (int cb = 0; cb < np; cb += kPairChunk<R>) {
+                const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk<R>), n, inv_n, rce2, m.nel));
+                __syncthreads();
+                g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
+                __syncthreads();
+            }
         }
     } else {
         g4_blocks<R, GL, true>(m, L, e, rce);
