@@ -204,6 +204,13 @@ void launch_md_refresh(double4 *wpos, const double4 *wposb, const double *pos, c
 void launch_md_save(double *slot, const double *pos, const double *mom, const double *F, int nat, const int *step,
                     hipStream_t s);
 void launch_md_tick(int *step, hipStream_t s);
+// a reused-list step's start (drift, displacement into maxd2[*step], wrapped
+// positions) and end (kick, state saved after step *step), each one launch
+void launch_md_begin(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,
+                     const double *posb, double4 *wpos, const double4 *wposb, int nat, unsigned long long *maxd2,
+                     const int *step, hipStream_t s);
+void launch_md_end(double *mom, const double *F, const double *Fold, double dt, double *slot, const double *pos,
+                   int nat, const int *step, hipStream_t s);
 void launch_md_load(const double *slot, double *pos, double *mom, double *F, int nat, hipStream_t s);
 void launch_md_ekin(const double *mom, const double *mass, const int *off, int nsys, double *ekin, hipStream_t s);
 

@@ -11,6 +11,10 @@
 
 namespace hdnnp {
 
+// every product and sum rounded as written, so that the fused kernels of a
+// reused-list step and the separate ones of a careful step agree bit for bit
+#pragma clang fp contract(off)
+
 // drift, keeping the forces of the step before in Fold for the kick
 __global__ __launch_bounds__(256) void k_md_drift(double *pos, const double *mom, const double *F, double *Fold,
                                                   const double *mass, double dt, int n3)
@@ -63,6 +67,56 @@ __global__ __launch_bounds__(256) void k_md_refresh(double4 *wpos, const double4
     const double4 b = wposb[i];
     wpos[i] = make_double4(b.x + (pos[3 * i] - posb[3 * i]), b.y + (pos[3 * i + 1] - posb[3 * i + 1]),
                            b.z + (pos[3 * i + 2] - posb[3 * i + 2]), 0.0);
+}
+
+// The start of a step on reused lists, one thread per atom: k_md_drift,
+// k_md_disp (into maxd2[*step]) and k_md_refresh in one pass, with their
+// arithmetic.
+__global__ __launch_bounds__(256) void k_md_begin(double *pos, const double *mom, const double *F, double *Fold,
+                                                  const double *mass, double dt, const double *posb, double4 *wpos,
+                                                  const double4 *wposb, int nat, unsigned long long *maxd2,
+                                                  const int *step)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < nat) {
+        const double m = mass[i];
+        double p[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int x = 3 * i + c;
+            const double f = F[x];
+            p[c] = (pos[x] + mom[x] / m * dt) + 0.5 * f / m * (dt * dt);
+            pos[x] = p[c];
+            Fold[x] = f;
+        }
+        const double a = p[0] - posb[3 * i], b = p[1] - posb[3 * i + 1], c = p[2] - posb[3 * i + 2];
+        d2 = a * a + b * b + c * c;
+        const double4 w = wposb[i];
+        wpos[i] = make_double4(w.x + a, w.y + b, w.z + c, 0.0);
+    }
+    red[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(maxd2 + *step, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// The end of a step: k_md_kick, then k_md_save of the state after step *step.
+__global__ __launch_bounds__(256) void k_md_end(double *mom, const double *F, const double *Fold, double dt,
+                                                double *slot, const double *pos, int n3, const int *step)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n3) return;
+    const double p = mom[x] + dt / 2.0 * (F[x] + Fold[x]);
+    mom[x] = p;
+    slot += (size_t)(*step + 1) * 3 * n3;
+    slot[x] = pos[x];
+    slot[n3 + x] = p;
+    slot[2 * n3 + x] = F[x];
 }
 
 // state of a step (positions, momenta, forces) to or from one saved slot;
@@ -143,6 +197,23 @@ void launch_md_save(double *slot, const double *pos, const double *mom, const do
 {
     if (nat > 0)
         hipLaunchKernelGGL(k_md_save, dim3((3 * nat + 255) / 256), dim3(256), 0, s, slot, pos, mom, F, 3 * nat, step);
+}
+
+void launch_md_begin(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,
+                     const double *posb, double4 *wpos, const double4 *wposb, int nat, unsigned long long *maxd2,
+                     const int *step, hipStream_t s)
+{
+    if (nat > 0)
+        hipLaunchKernelGGL(k_md_begin, dim3((nat + 255) / 256), dim3(256), 0, s, pos, mom, F, Fold, mass, dt, posb, wpos,
+                           wposb, nat, maxd2, step);
+}
+
+void launch_md_end(double *mom, const double *F, const double *Fold, double dt, double *slot, const double *pos,
+                   int nat, const int *step, hipStream_t s)
+{
+    if (nat > 0)
+        hipLaunchKernelGGL(k_md_end, dim3((3 * nat + 255) / 256), dim3(256), 0, s, mom, F, Fold, dt, slot, pos, 3 * nat,
+                           step);
 }
 
 void launch_md_tick(int *step, hipStream_t s) { hipLaunchKernelGGL(k_md_tick, dim3(1), dim3(64), 0, s, step); }

@@ -159,13 +159,11 @@ void MdSystems::step_launch(bool timed)
 {
     hipStream_t s = eng_.stream();
     const int *ctr = ctr_.as<int>();
-    launch_md_drift(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), mass_.as<double>(), dt_,
-                    nat_, s);
-    launch_md_disp(pos_.as<double>(), posb_.as<double>(), nat_, d2_.as<unsigned long long>(), ctr, s);
-    launch_md_refresh(eng_.wpos(), wposb_.as<double4>(), pos_.as<double>(), posb_.as<double>(), nat_, s);
+    launch_md_begin(pos_.as<double>(), mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), mass_.as<double>(), dt_,
+                    posb_.as<double>(), eng_.wpos(), wposb_.as<double4>(), nat_, d2_.as<unsigned long long>(), ctr, s);
     eng_.finish_launch(F_.as<double>(), s, timed);
-    launch_md_kick(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, nat_, s);
-    launch_md_save(snap_.as<double>(), pos_.as<double>(), mom_.as<double>(), F_.as<double>(), nat_, ctr, s);
+    launch_md_end(mom_.as<double>(), F_.as<double>(), Fold_.as<double>(), dt_, snap_.as<double>(), pos_.as<double>(), nat_,
+                  ctr, s);
     launch_md_tick(ctr_.as<int>(), s);
 }
 
