@@ -76,3 +76,32 @@ def test_beyond_staging_capacity(pkg, wl, precision):
     assert ei.value.code == 6 and "neighbours" in str(ei.value)
     E, F, _ = h.energy_and_forces(_lattice(wl, 5.0))
     assert np.isfinite(E) and np.all(np.isfinite(F))
+
+
+def test_triplet_list_fallback_fp64(pkg, wl, oracles):
+    """FP64 hands the value pass's triplets to the gradient pass, in per-atom
+    lists sized from the mean density. A dense 120-atom C/N/H cluster in a
+    60-bohr periodic box, with 40 dilute atoms around it, has ~100
+    neighbours per cluster atom at a mean of ~4: the cluster atoms overflow
+    their lists and are compacted by the gradient pass itself, while the
+    dilute atoms use theirs. Both against the oracle."""
+    c = wl.cluster(120, seed=11, box=14.0, dmin=1.6)
+    rng = np.random.default_rng(12)
+    gas = []
+    while len(gas) < 40:
+        p = rng.random(3) * 60.0
+        if np.min(np.linalg.norm(c.pos + 23.0 - p, axis=1)) > 14.0 and all(np.linalg.norm(p - q) > 3.0 for q in gas):
+            gas.append(p)
+    pos = np.vstack([c.pos + 23.0, np.array(gas)])
+    Z = np.concatenate([c.Z, rng.choice(np.array([1, 6, 7], np.int32), size=len(gas))]).astype(np.int32)
+    s = wl.Structure(pos, Z, np.eye(3) * 60.0, True)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    E, F, L = h.energy_and_forces(s)
+    ent = h.neighbor_entries()
+    cnt = np.bincount(np.asarray(ent)[:, 0], minlength=s.nat)
+    print(f"neighbours: cluster mean {cnt[:120].mean():.0f}, gas mean {cnt[120:].mean():.1f}", flush=True)
+    assert cnt[:120].mean() > 60 and cnt[120:].max() < 20
+    r = oracles["perovskite"].evaluate(s)
+    assert abs(E - r["energy"]) <= 1e-10 * max(1.0, abs(E))
+    np.testing.assert_allclose(F, r["force"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(L, r["dEdLat"], rtol=0, atol=1e-9 * max(1.0, np.abs(r["dEdLat"]).max()))
