@@ -31,10 +31,12 @@
 // are supported (AtomsPerWave) but measured slower here: LDS per atom limits
 // the waves in flight.
 // The FP32 path uses the hardware exp2/rcp/sqrt approximations; the FP64 path
-// keeps the libm-accurate functions.
+// uses exp and tanh polynomials of about 1 ulp (fmath.hpp) and the device
+// library's sqrt, pow and cospi.
 #include <rocprim/intrinsics/warp_shuffle.hpp>
 
 #include "device.hpp"
+#include "fmath.hpp"
 
 namespace hdnnp {
 
@@ -48,11 +50,12 @@ template <typename R> struct M;
 template <> struct M<double> {
     static __device__ double sqrt(double x) { return ::sqrt(x); }
     static __device__ double rcp(double x) { return 1.0 / x; }
-    static __device__ double tanh(double x) { return ::tanh(x); }
+    // tanh of the cutoffs, whose argument 1 - r/rc lies in [0, 1]
+    static __device__ double tanh(double x) { return tanh01(x); }
     static __device__ double pow(double x, double y) { return ::pow(x, y); }
     // exp(-eta x) from the table value a = -eta
-    static __device__ double expsc(double a, double x) { return ::exp(a * x); }
-    static __device__ double exp(double x) { return ::exp(x); }
+    static __device__ double expsc(double a, double x) { return exp_f64(a * x); }
+    static __device__ double exp(double x) { return exp_f64(x); }
     static __device__ double cospi(double x) { return ::cospi(x); }
     static __device__ double sinpi(double x) { return ::sinpi(x); }
 };
@@ -1033,8 +1036,16 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
 }
 
 // ---------------------------------------------------------------- pass A
+// Waves per SIMD the FP64 passes at the 96-neighbour staging, one atom per
+// wavefront, are compiled for: their LDS allows 4 (values) and 3
+// (gradients), and without the bound the compiler keeps exp_core's 13
+// constants in VGPRs and lands at 2. With it, a few values spill to scratch
+// outside the triplet loops.
+template <typename R, int NMAX, int GA, bool GRAD>
+constexpr int kSfWaves = sizeof(R) == 8 && NMAX <= 96 && GA == 1 ? (GRAD ? 3 : 4) : 1;
+
 template <typename R, int NMAX, int GA>
-__global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *G)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, NMAX, GA, false>))) void k_sf_values(DevModel m, Geo g, Work w, R *G)
 {
     constexpr int GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
@@ -1080,7 +1091,7 @@ __global__ __launch_bounds__(64) void k_sf_values(DevModel m, Geo g, Work w, R *
 
 // ---------------------------------------------------------------- pass B
 template <typename R, int NMAX, int GA>
-__global__ __launch_bounds__(64) void k_sf_grad(DevModel m, Geo g, Work w, const R *dEdG, R *gent)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, NMAX, GA, true>))) void k_sf_grad(DevModel m, Geo g, Work w, const R *dEdG, R *gent)
 {
     constexpr int GL = 64 / GA;
     extern __shared__ __align__(16) char smem[];
