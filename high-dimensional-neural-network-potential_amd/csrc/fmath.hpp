@@ -1,4 +1,4 @@
-// FP64 exp and tanh of the symmetry-function kernels.
+// FP64 exp and tanh of the symmetry-function and MLP kernels.
 //
 // The device library's FP64 exp and tanh carry double-double steps and
 // special-case branches (about 60 and 190 instructions on gfx950), and the
@@ -19,6 +19,8 @@
 //   exp_core's constants, which the kernels keep in registers; a polynomial
 //   of its own would add 16 more, and the kernels already run at the
 //   register limit of their occupancy.
+// - tanh_f64: the same for any x through |x| and the sign (at most 2.4 ulp;
+//   the device library's takes 186 instructions), for the FP64 MLP.
 //
 // Every step is an explicit fma or a single rounded operation, so the result
 // is the same in every kernel that inlines it and in a host build of this
@@ -65,6 +67,17 @@ __host__ __device__ __forceinline__ double tanh01(double x)
     // e = 2^n (1 + m) - 1 = 2^n m + (2^n - 1), the bracket exact (n >= -3)
     const double e = __builtin_ldexp(m, (int)n) + (__builtin_ldexp(1.0, (int)n) - 1.0);
     return -e / (2.0 + e);
+}
+
+// tanh for any x, as sign(x) tanh01's formula at |x| (the MLP's activation):
+// for |x| > 400, expm1(-2|x|) is -1 in double and the result +-1
+__host__ __device__ __forceinline__ double tanh_f64(double x)
+{
+    const double a = __builtin_fabs(x);
+    double m;
+    const double n = exp_core(a > 400.0 ? -800.0 : -2.0 * a, m);
+    const double e = __builtin_ldexp(m, (int)n) + (__builtin_ldexp(1.0, (int)n) - 1.0);
+    return __builtin_copysign(-e / (2.0 + e), x);
 }
 
 }  // namespace hdnnp

@@ -16,6 +16,7 @@
 // The tile's activations and pre-activations stay in LDS (row stride odd, so
 // the 16 rows of one column fall in distinct banks).
 #include "device.hpp"
+#include "fmath.hpp"
 
 namespace hdnnp {
 
@@ -40,7 +41,7 @@ template <> struct Mma<double> {
 __device__ __forceinline__ double act_f(char a, double x)
 {
     switch (a) {
-    case 't': return tanh(x);
+    case 't': return tanh_f64(x);
     case 'p': return log(1.0 + exp(x));
     case 'r': return x > 0.0 ? x : 0.0;
     default: return x;
@@ -49,7 +50,7 @@ __device__ __forceinline__ double act_f(char a, double x)
 __device__ __forceinline__ double act_d(char a, double x)
 {
     switch (a) {
-    case 't': { const double t = tanh(x); return 1.0 - t * t; }
+    case 't': { const double t = tanh_f64(x); return 1.0 - t * t; }
     case 'p': return 1.0 / (1.0 + exp(-x));
     case 'r': return x > 0.0 ? 1.0 : 0.0;
     default: return 1.0;

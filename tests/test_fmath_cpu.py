@@ -31,7 +31,8 @@ int main(int argc, char **argv)
 {
     const int mode = std::atoi(argv[1]);
     double x;
-    while (std::scanf("%lf", &x) == 1) std::printf("%.17g\n", mode ? hdnnp::exp_f64(x) : hdnnp::tanh01(x));
+    while (std::scanf("%lf", &x) == 1)
+        std::printf("%.17g\n", mode == 1 ? hdnnp::exp_f64(x) : mode == 2 ? hdnnp::tanh_f64(x) : hdnnp::tanh01(x));
     return 0;
 }
 """
@@ -96,3 +97,18 @@ def test_tanh_cubed_cutoff_against_libm(fm):
     x = 1 - r / 12.0
     t = fm(0, x)
     np.testing.assert_allclose(t ** 3, np.tanh(x) ** 3, rtol=4e-15, atol=0)   # ~3 x (1.4 + 1) ulp
+
+
+def test_tanh_f64_accuracy(fm):
+    """The MLP's tanh activation (neuralNetworks.f90:33-55) over any argument."""
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([np.linspace(-30, 30, 3001), rng.standard_normal(1500) * 3, rng.random(300) * 1e-4 - 5e-5,
+                         [1e-300, -1e-300, 18.0, -18.0, 19.5, 400.0, 401.0]])
+    xs = xs[xs != 0]
+    e = _ulps(fm(2, xs), xs, mp.tanh)
+    print(f"tanh_f64: max {e.max():.3f} ulp, mean {e.mean():.3f} ulp over {len(xs)} points")
+    assert e.max() <= 2.5
+    got = fm(2, [0.0, -0.0, float("inf"), -float("inf"), 1e300, -1e300])
+    np.testing.assert_array_equal(got, [0.0, -0.0, 1.0, -1.0, 1.0, -1.0])
+    assert np.signbit(got[1])
+    assert np.isnan(fm(2, [float("nan")])[0])
