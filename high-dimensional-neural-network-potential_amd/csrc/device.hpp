@@ -189,9 +189,10 @@ size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, hipStream_t s);
 // per-structure sums of Ei and vir: chunks [c.x, c.y) of atoms never cross a
-// structure; structure s owns chunks [sc[s], sc[s+1]); partial: 10 per chunk
+// structure; structure s owns chunks [sc[s], sc[s+1]); partial: 10 per chunk;
+// flag_out (or null) receives a copy of w.flags[0..8)
 void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   hipStream_t s);
+                   int *flag_out, hipStream_t s);
 
 // ---- kernels_md.hip: velocity Verlet and Verlet-skin bookkeeping ----
 void launch_md_drift(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,

@@ -138,7 +138,7 @@ Engine::Engine(std::vector<HostModel> members, int precision) : precision_(preci
 std::vector<DevMem *> Engine::work_buffers()
 {
     return {&bbox_, &geo_, &sid_, &csid_, &chunks_, &sc_, &res_, &partial_,
-            &zin_, &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
+            &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
             &iota_, &cell_start_, &cell_end_, &count_, &perm_, &Ei_,
             &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &elem_chunks_,
             &trip_, &tcount_};
@@ -168,6 +168,7 @@ Engine::~Engine()
     for (DevMem *d : work_buffers()) d->release();
     misc_.release();
     if (pinned_) (void)hipHostFree(pinned_);
+    if (io_pinned_) (void)hipHostFree(io_pinned_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -391,6 +392,9 @@ void Engine::init_device()
 
 // misc buffer layout
 static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // ints
+static constexpr size_t kFlagBytes = 8 * sizeof(int);                          // w.flags[0..8)
+// host-array evaluations of at most this many atoms stage through pinned memory
+static constexpr long kSmallHostEval = 4096;
 static constexpr size_t kDblOffset = 256;                                    // bytes
 // largest per-(atom, stencil cell) slot table; beyond it partners are found by bisection
 static constexpr size_t kPosStartBytes = size_t(1) << 30;
@@ -446,18 +450,20 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
 }
 
 // pinned host staging, grown on demand (only touched between synchronised runs)
-void *Engine::staging(size_t bytes)
+static void *grow_pinned(void *&p, size_t &cap, size_t bytes)
 {
-    if (bytes > pinned_bytes_) {
-        if (pinned_) HIPCHK(hipHostFree(pinned_));
-        pinned_ = nullptr;
-        pinned_bytes_ = 0;
+    if (bytes > cap) {
+        if (p) HIPCHK(hipHostFree(p));
+        p = nullptr;
+        cap = 0;
         const size_t want = std::max<size_t>(4096, bytes + bytes / 2);
-        HIPCHK(hipHostMalloc(&pinned_, want, hipHostMallocDefault));
-        pinned_bytes_ = want;
+        HIPCHK(hipHostMalloc(&p, want, hipHostMallocDefault));
+        cap = want;
     }
-    return pinned_;
+    return p;
 }
+void *Engine::staging(size_t bytes) { return grow_pinned(pinned_, pinned_bytes_, bytes); }
+void *Engine::io_staging(size_t bytes) { return grow_pinned(io_pinned_, io_pinned_bytes_, bytes); }
 
 // Cell grid of one structure (neighborLists.f90:131-169): lattice (periodic)
 // or artificial box from the bounding box (nonperiodic), grid cells of at
@@ -656,7 +662,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     const int nm = (int)dmm_.size();
     const size_t res_b = 8 * 10 * (size_t)nstruct * nm;
     const size_t o_geo = 0, o_ch = (geo_b + 255) / 256 * 256, o_sc = o_ch + (ch_b + 255) / 256 * 256,
-                 o_res = o_sc + (sc_b + 255) / 256 * 256, o_flags = o_res + (res_b + 255) / 256 * 256,
+                 o_res = o_sc + (sc_b + 255) / 256 * 256, o_flags = o_res + (res_b + kFlagBytes + 255) / 256 * 256,
                  o_bbox = o_flags + 256;
     char *hs = (char *)staging(o_bbox + 256);
     // the tables are uploaded only when they differ from the last upload
@@ -665,7 +671,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
                       std::memcmp(uploaded_.data(), hgeo_.data(), geo_b) == 0 &&
                       std::memcmp(uploaded_.data() + o_ch, chunks.data(), ch_b) == 0 &&
                       std::memcmp(uploaded_.data() + o_sc, sc.data(), sc_b) == 0;
-    res_.ensure(res_b);
+    res_.ensure(res_b + kFlagBytes);   // the results, then a copy of the flags
     partial_.ensure(8 * 10 * (size_t)std::max(nchunk, 1));
     if (!same) {
         std::memcpy(hs + o_geo, hgeo_.data(), geo_b);
@@ -811,7 +817,8 @@ void Engine::finish_launch(double *force_dev, hipStream_t s, bool timed)
         Work wk = w;
         wk.Ei = w.Ei + (size_t)k * nat;
         wk.result = w.result + 10 * (size_t)k * nstruct;
-        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), s);
+        int *flag_out = k == 0 ? (int *)(w.result + 10 * (size_t)nm * nstruct) : nullptr;
+        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), flag_out, s);
     }
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[6], s));
@@ -829,9 +836,11 @@ bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)
     const std::vector<StructIn> &st = prep_.st;
     char *hs = (char *)pinned_;
     double *hres = (double *)(hs + prep_.o_res);
-    int *hflags = (int *)(hs + prep_.o_flags);
-    HIPCHK(hipMemcpyAsync(hres, w.result, prep_.res_b, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(hflags, w.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+    int *hflags = (int *)(hs + prep_.o_res + prep_.res_b);   // the reduction's copy of w.flags
+    HIPCHK(hipMemcpyAsync(hres, w.result, prep_.res_b + kFlagBytes, hipMemcpyDeviceToHost, s));
+    if (pending_force_.bytes)
+        HIPCHK(hipMemcpyAsync(pending_force_.host, pending_force_.dev, pending_force_.bytes, hipMemcpyDeviceToHost,
+                              s));
     HIPCHK(hipStreamSynchronize(s));
     if (hflags[0]) throw Error(4, "structure contains element Z=" + std::to_string(hflags[0]) +
                                       " that the potential does not describe");
@@ -933,16 +942,36 @@ void Engine::evaluate_batch_host(int nstruct, const int *nat, const int *Z, cons
     if (total > 0 && (!Z || !pos || !force)) throw Error(1, "null array");
     if (total >= (1L << 31)) throw Error(6, "more than 2^31 atoms in one evaluation");
     const size_t n = (size_t)std::max<long>(total, 1);
-    zin_.ensure(4 * n);
-    posin_.ensure(24 * n);
+    const size_t pos_b = 24 * (size_t)total, z_off = (24 * n + 255) / 256 * 256, in_b = z_off + 4 * (size_t)total;
+    posin_.ensure(z_off + 4 * n);
     fout_.ensure(24 * n);
-    if (total > 0) {
-        HIPCHK(hipMemcpyAsync(zin_.p, Z, 4 * (size_t)total, hipMemcpyHostToDevice, stream_));
-        HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)total, hipMemcpyHostToDevice, stream_));
+    const int *Z_dev = (const int *)(posin_.as<char>() + z_off);
+    // Small inputs go through pinned staging: one upload, and the forces come
+    // back with the results before the one synchronisation. Large ones are
+    // copied straight from the caller's arrays (a staging copy would cost more
+    // than the launches it saves).
+    const bool small = total > 0 && total <= kSmallHostEval;
+    char *io = small ? (char *)io_staging(std::max(in_b, pos_b)) : nullptr;
+    if (small) {
+        std::memcpy(io, pos, pos_b);
+        std::memcpy(io + z_off, Z, 4 * (size_t)total);
+        HIPCHK(hipMemcpyAsync(posin_.p, io, in_b, hipMemcpyHostToDevice, stream_));
+        pending_force_ = {(double *)io, fout_.as<double>(), pos_b};
+    } else if (total > 0) {
+        HIPCHK(hipMemcpyAsync((void *)Z_dev, Z, 4 * (size_t)total, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(posin_.p, pos, pos_b, hipMemcpyHostToDevice, stream_));
     }
-    run(st, zin_.as<int>(), posin_.as<double>(), fout_.as<double>(), stream_, energy, dEdLat, 1);
-    if (total > 0) {
-        HIPCHK(hipMemcpyAsync(force, fout_.p, 24 * (size_t)total, hipMemcpyDeviceToHost, stream_));
+    try {
+        run(st, Z_dev, posin_.as<double>(), fout_.as<double>(), stream_, energy, dEdLat, 1);
+    } catch (...) {
+        pending_force_ = {};
+        throw;
+    }
+    pending_force_ = {};
+    if (small) {
+        std::memcpy(force, io, pos_b);
+    } else if (total > 0) {
+        HIPCHK(hipMemcpyAsync(force, fout_.p, pos_b, hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
     }
 }

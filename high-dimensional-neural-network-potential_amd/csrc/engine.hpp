@@ -134,6 +134,7 @@ class Engine {
 
   private:
     void *staging(size_t bytes);
+    void *io_staging(size_t bytes);
     std::vector<DevMem *> work_buffers();
     template <typename R>
     void compute(Work &w, const Geo &g, hipStream_t s, bool timed);
@@ -156,7 +157,7 @@ class Engine {
     int mlp_maxw_ = 0, mlp_sumc_ = 0;
 
     // work buffers
-    DevMem zin_, posin_, fout_;
+    DevMem posin_, fout_;   // host-array evaluations: positions, then Z (256-aligned); forces
     DevMem eidx_, wpos_, cellkey_, cellkey_sorted_, cell_atoms_, iota_, cell_start_, cell_end_;
     DevMem count_, perm_, Ei_, vir_, misc_;
     DevMem ent_j_, ent_shift_, ent_s_, rev_, pos_start_, gent_, G_, dEdG_, cub_tmp_, elem_chunks_;
@@ -176,6 +177,17 @@ class Engine {
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
     size_t pinned_bytes_ = 0;
+    // pinned staging of a small host-array evaluation's inputs and forces
+    // (apart from pinned_, which build() rewrites while those copies run)
+    void *io_pinned_ = nullptr;
+    size_t io_pinned_bytes_ = 0;
+    // a force read-back that finish_collect enqueues before its one
+    // synchronisation (host-array evaluations)
+    struct {
+        double *host = nullptr;
+        const double *dev = nullptr;
+        size_t bytes = 0;
+    } pending_force_;
     int last_nat_ = 0, last_cap_ = 0, last_lim_ = 0;
     int64_t last_total_ = 0;   // -1: not yet summed
     int cap_hint_ = 0;         // most neighbours of one atom seen so far

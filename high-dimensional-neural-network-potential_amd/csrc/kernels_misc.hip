@@ -113,7 +113,9 @@ __global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chun
 
 // one wave per structure: lane l adds chunks l, l+64, ... of the structure,
 // then a fixed butterfly over the lanes
-__global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, const double *partial)
+// flag_out (block 0, when given) receives a copy of the 8 flag words, so that
+// the host reads results and flags back in one copy
+__global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, const double *partial, int *flag_out)
 {
     const int s = blockIdx.x, lane = threadIdx.x;
     double a[10];
@@ -131,14 +133,15 @@ __global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, cons
         for (int k = 1; k < 10; ++k) v = lane == k ? a[k] : v;
         w.result[10 * (size_t)s + lane] = v;
     }
+    if (flag_out && s == 0 && lane >= 16 && lane < 24) flag_out[lane - 16] = w.flags[lane - 16];
 }
 
 void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   hipStream_t s)
+                   int *flag_out, hipStream_t s)
 {
     if (nchunk > 0)
         hipLaunchKernelGGL(k_reduce_partial, dim3(nchunk), dim3(256), 0, s, w, chunks, partial);
-    hipLaunchKernelGGL(k_reduce_final, dim3(nstruct), dim3(64), 0, s, w, sc, (const double *)partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(nstruct), dim3(64), 0, s, w, sc, (const double *)partial, flag_out);
 }
 
 }  // namespace hdnnp
