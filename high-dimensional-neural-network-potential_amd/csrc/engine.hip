@@ -720,18 +720,23 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     while ((1L << bits) < ncell) ++bits;
 
     launch_prep(dm_, g, w, iota_.as<int>(), s);
-    size_t b1 = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
-                                              w.cell_atoms, nat, 0, bits, s));
-    cub_tmp_.ensure(b1);
-    size_t bt = cub_tmp_.bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
-                                              w.cell_atoms, nat, 0, bits, s));
     cell_start_.ensure(4 * (size_t)ncell);
     cell_end_.ensure(4 * (size_t)ncell);
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
-    launch_cell_bounds(w, ncell, s);
+    if (nat <= kSmallSortAtoms && ncell <= kSmallSortCells) {
+        // one workgroup; the radix sort's input and output keys serve as scratch
+        launch_cell_sort_small(w, (int)ncell, iota_.as<int>(), w.cellkey_sorted, s);
+    } else {
+        size_t b1 = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                                  w.cell_atoms, nat, 0, bits, s));
+        cub_tmp_.ensure(b1);
+        size_t bt = cub_tmp_.bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                                  w.cell_atoms, nat, 0, bits, s));
+        launch_cell_bounds(w, ncell, s);
+    }
     // atoms sorted by element (MLP tiles); needs no host round trip
     elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
     launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);

@@ -192,6 +192,60 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
     if (ncell > 0) hipLaunchKernelGGL(k_cell_bounds, dim3((ncell + 255) / 256), dim3(256), 0, s, w, ncell);
 }
 
+// The cell sort of a small evaluation in one workgroup (one launch where the
+// radix sort and the range search take eight): per-cell counts in LDS, their
+// exclusive scan, placement by arrival slot, then the order within a cell by
+// atom index, which is the stable sort's order (atom i's rank is the number
+// of its cell's atoms with a smaller index). Needs nat <= kSmallSortAtoms and
+// ncell <= kSmallSortCells; slot and tmp hold nat ints each.
+__global__ __launch_bounds__(kSmallSortThreads) void k_cell_sort_small(Work w, int ncell, int *slot, int *tmp)
+{
+    __shared__ int start[kSmallSortCells + 1];
+    __shared__ int part[kSmallSortThreads];
+    const int t = threadIdx.x, nat = w.nat;
+    for (int c = t; c <= ncell; c += kSmallSortThreads) start[c] = 0;
+    __syncthreads();
+    for (int i = t; i < nat; i += kSmallSortThreads) slot[i] = atomicAdd(&start[w.cellkey[i]], 1);
+    __syncthreads();
+    // exclusive scan of the counts: thread t owns cells [c0, c1)
+    const int per = (ncell + kSmallSortThreads - 1) / kSmallSortThreads;
+    const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
+    int sum = 0;
+    for (int c = c0; c < c1; ++c) sum += start[c];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < kSmallSortThreads; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int c = c0; c < c1; ++c) {
+        const int n = start[c];
+        start[c] = run;
+        w.cell_start[c] = run;
+        run += n;
+        w.cell_end[c] = run;
+    }
+    if (t == 0) start[ncell] = nat;
+    __syncthreads();
+    for (int i = t; i < nat; i += kSmallSortThreads) tmp[start[w.cellkey[i]] + slot[i]] = i;
+    __syncthreads();
+    for (int i = t; i < nat; i += kSmallSortThreads) {
+        const int c = w.cellkey[i], s0 = start[c], s1 = start[c + 1];
+        int r = 0;
+        for (int q = s0; q < s1; ++q) r += tmp[q] < i;
+        w.cell_atoms[s0 + r] = i;
+    }
+}
+
+void launch_cell_sort_small(Work &w, int ncell, int *slot, int *tmp, hipStream_t s)
+{
+    if (w.nat > 0)
+        hipLaunchKernelGGL(k_cell_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, w, ncell, slot, tmp);
+}
+
 // Atoms ordered by element, in index order within an element (the tiles of
 // the MLP), by a stable counting sort over chunks of kSortChunk atoms:
 // per-chunk counts, their offsets, then each chunk places its atoms.
