@@ -724,9 +724,11 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     cell_end_.ensure(4 * (size_t)ncell);
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
-    if (nat <= kSmallSortAtoms && ncell <= kSmallSortCells) {
-        // one workgroup; the radix sort's input and output keys serve as scratch
-        launch_cell_sort_small(w, (int)ncell, iota_.as<int>(), w.cellkey_sorted, s);
+    const bool small_sort = nat <= kSmallSortAtoms && ncell <= kSmallSortCells;
+    if (small_sort) {
+        // one workgroup sorts by cell and by element; the radix sort's input
+        // and output keys serve as scratch
+        launch_sort_small(w, (int)ncell, dm_.nel, iota_.as<int>(), w.cellkey_sorted, s);
     } else {
         size_t b1 = 0;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
@@ -738,8 +740,10 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
         launch_cell_bounds(w, ncell, s);
     }
     // atoms sorted by element (MLP tiles); needs no host round trip
-    elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
-    launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
+    if (!small_sort) {
+        elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
+        launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_[1], s));
 

@@ -192,13 +192,16 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
     if (ncell > 0) hipLaunchKernelGGL(k_cell_bounds, dim3((ncell + 255) / 256), dim3(256), 0, s, w, ncell);
 }
 
-// The cell sort of a small evaluation in one workgroup (one launch where the
-// radix sort and the range search take eight): per-cell counts in LDS, their
-// exclusive scan, placement by arrival slot, then the order within a cell by
-// atom index, which is the stable sort's order (atom i's rank is the number
-// of its cell's atoms with a smaller index). Needs nat <= kSmallSortAtoms and
-// ncell <= kSmallSortCells; slot and tmp hold nat ints each.
-__global__ __launch_bounds__(kSmallSortThreads) void k_cell_sort_small(Work w, int ncell, int *slot, int *tmp)
+// The sorts of a small evaluation in one workgroup (one launch where the
+// radix sort, the range search and the element sort take eleven).
+// By cell: per-cell counts in LDS, their exclusive scan, placement by arrival
+// slot, then the order within a cell by atom index, which is the stable
+// sort's order (atom i's rank is the number of its cell's atoms with a
+// smaller index). By element: counts, offsets, then placement in rounds of
+// the workgroup's size in index order, ranked by wave ballots as in
+// k_elem_place. Needs nat <= kSmallSortAtoms and ncell <= kSmallSortCells;
+// slot and tmp hold nat ints each.
+__global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(Work w, int ncell, int nel, int *slot, int *tmp)
 {
     __shared__ int start[kSmallSortCells + 1];
     __shared__ int part[kSmallSortThreads];
@@ -238,12 +241,55 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_cell_sort_small(Work w, i
         for (int q = s0; q < s1; ++q) r += tmp[q] < i;
         w.cell_atoms[s0 + r] = i;
     }
+
+    // by element (the MLP tiles)
+    constexpr int kWaves = kSmallSortThreads / 64;
+    __shared__ int ecnt[kMaxEl], ebase[kMaxEl], wcnt[kWaves][kMaxEl];
+    const int lane = t % 64, wave = t / 64;
+    if (t < kMaxEl) ecnt[t] = 0;
+    __syncthreads();
+    for (int i = t; i < nat; i += kSmallSortThreads) atomicAdd(&ecnt[w.eidx[i]], 1);
+    __syncthreads();
+    if (t <= kMaxEl) {
+        int off = 0;
+        for (int f = 0; f < t; ++f) off += ecnt[f];
+        w.elem_off[t] = off;
+        if (t < kMaxEl) {
+            w.elem_count[t] = ecnt[t];
+            ebase[t] = off;
+        }
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1;
+    for (int r0 = 0; r0 < nat; r0 += kSmallSortThreads) {
+        const int i = r0 + t;
+        const int e = i < nat ? w.eidx[i] : -1;
+        int rank = 0;
+        for (int f = 0; f < nel; ++f) {
+            const unsigned long long mk = __ballot(e == f);
+            if (e == f) rank = __popcll(mk & below);
+            if (lane == 0) wcnt[wave][f] = __popcll(mk);
+        }
+        __syncthreads();                           // wcnt of this round, ebase of the rounds before
+        if (e >= 0) {
+            int pos = ebase[e] + rank;
+            for (int v = 0; v < wave; ++v) pos += wcnt[v][e];
+            w.perm[pos] = i;
+        }
+        __syncthreads();                           // every thread has read ebase and wcnt
+        if (t < nel) {
+            int add = 0;
+            for (int v = 0; v < kWaves; ++v) add += wcnt[v][t];
+            ebase[t] += add;
+        }
+        __syncthreads();                           // ebase updated before the next round's wcnt
+    }
 }
 
-void launch_cell_sort_small(Work &w, int ncell, int *slot, int *tmp, hipStream_t s)
+void launch_sort_small(Work &w, int ncell, int nel, int *slot, int *tmp, hipStream_t s)
 {
     if (w.nat > 0)
-        hipLaunchKernelGGL(k_cell_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, w, ncell, slot, tmp);
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, w, ncell, nel, slot, tmp);
 }
 
 // Atoms ordered by element, in index order within an element (the tiles of
