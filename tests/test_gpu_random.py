@@ -106,3 +106,25 @@ def test_random_batch_equals_single(pkg, sweep, precision):
         assert np.array_equal(Fb[k], single[k][1])
         if s.periodic:
             assert np.array_equal(Lb[k], single[k][2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_workgroup_sort_equals_radix_sort(pkg, precision):
+    """A build of up to 8,192 atoms sorts atoms by cell and by element in one
+    workgroup; a larger one uses the radix sort and the chunked element sort.
+    Each structure here is below the limit alone and the batch of all three is
+    above it. The results must be bit-identical both ways, so both sorts give
+    the same atom orders."""
+    wl = pkg.workloads
+    structs = [wl.perovskite(6, 6, 6, seed=11), wl.methylammonium(), wl.perovskite(8, 8, 8, seed=12)]
+    assert max(len(s.Z) for s in structs) <= 8192 < sum(len(s.Z) for s in structs)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    hm = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    single = [h.energy_and_forces(s) for s in structs]
+    Eb, Fb, Lb = hm.energy_and_forces_batch(structs)
+    for k, s in enumerate(structs):
+        print(f"{precision} structure {k} nat={len(s.Z)}: E alone {single[k][0]!r} in batch {Eb[k]!r}", flush=True)
+        assert Eb[k] == single[k][0]
+        assert np.array_equal(Fb[k], single[k][1])
+        if s.periodic:
+            assert np.array_equal(Lb[k], single[k][2])
