@@ -165,11 +165,12 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s);
 // and cells, wrapped positions and cell keys, the identity order
 void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s);
 // cell_start / cell_end from the key-sorted atoms
-// cell and element sorts of evaluations of at most kSmallSortAtoms atoms in
-// kSmallSortCells cells, in one workgroup (cell_atoms, cell_start, cell_end,
-// perm, elem_off, elem_count; slot and tmp are nat-int scratch)
+// set-up (as launch_prep, without iota) and cell and element sorts of
+// evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in
+// one workgroup (cell_atoms, cell_start, cell_end, perm, elem_off,
+// elem_count; slot and tmp are nat-int scratch)
 constexpr int kSmallSortThreads = 1024, kSmallSortAtoms = 8192, kSmallSortCells = 12288;
-void launch_sort_small(Work &w, int ncell, int nel, int *slot, int *tmp, hipStream_t s);
+void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, hipStream_t s);
 void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
 // perm (atoms by element, index order within one), elem_off, elem_count;
 // chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk

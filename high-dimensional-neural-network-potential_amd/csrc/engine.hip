@@ -719,17 +719,17 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     int bits = 1;
     while ((1L << bits) < ncell) ++bits;
 
-    launch_prep(dm_, g, w, iota_.as<int>(), s);
     cell_start_.ensure(4 * (size_t)ncell);
     cell_end_.ensure(4 * (size_t)ncell);
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
     const bool small_sort = nat <= kSmallSortAtoms && ncell <= kSmallSortCells;
     if (small_sort) {
-        // one workgroup sorts by cell and by element; the radix sort's input
-        // and output keys serve as scratch
-        launch_sort_small(w, (int)ncell, dm_.nel, iota_.as<int>(), w.cellkey_sorted, s);
+        // one workgroup does the set-up and sorts by cell and by element; the
+        // radix sort's input and output keys serve as scratch
+        launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, s);
     } else {
+        launch_prep(dm_, g, w, iota_.as<int>(), s);
         size_t b1 = 0;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                                   w.cell_atoms, nat, 0, bits, s));

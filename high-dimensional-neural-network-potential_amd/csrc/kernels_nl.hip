@@ -125,9 +125,9 @@ __device__ inline int segment_of(const StructGeo *sg, int nstruct, int x, bool c
 //   shifted by the artificial origin and wrapped once in the artificial box.
 //   Cell index int(s*ngrid) (neighborLists.f90:161-166), clamped into the
 //   grid, offset by the structure's first cell.
-__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota)
+// item x of the set-up (cell x, atom x); iota may be null
+__device__ __forceinline__ void prep_one(const DevModel &m, const Geo &g, const Work &w, int *iota, int x)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x < g.ncell) w.csid[x] = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, true);
     if (x >= w.nat) return;
     const int z = w.Z[x];
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *io
     w.eidx[x] = e < 0 ? m.fallback_el : e;
     const int sid = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, false);
     w.sid[x] = sid;
-    iota[x] = x;
+    if (iota) iota[x] = x;
     const StructGeo &G = g.sg[sid];
     double px = w.pos[3 * x + 0], py = w.pos[3 * x + 1], pz = w.pos[3 * x + 2];
     if (G.periodic) {
@@ -157,6 +157,11 @@ __global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *io
     c1 = min(max(c1, 0), G.ng[1] - 1);
     c2 = min(max(c2, 0), G.ng[2] - 1);
     w.cellkey[x] = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
+}
+
+__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota)
+{
+    prep_one(m, g, w, iota, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s)
@@ -199,13 +204,17 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
 // sort's order (atom i's rank is the number of its cell's atoms with a
 // smaller index). By element: counts, offsets, then placement in rounds of
 // the workgroup's size in index order, ranked by wave ballots as in
-// k_elem_place. Needs nat <= kSmallSortAtoms and ncell <= kSmallSortCells;
+// k_elem_place. The per-atom set-up of k_prep runs first, in the same
+// workgroup. Needs nat <= kSmallSortAtoms and ncell <= kSmallSortCells;
 // slot and tmp hold nat ints each.
-__global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(Work w, int ncell, int nel, int *slot, int *tmp)
+__global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Geo g, Work w, int nel, int *slot,
+                                                                  int *tmp)
 {
     __shared__ int start[kSmallSortCells + 1];
     __shared__ int part[kSmallSortThreads];
-    const int t = threadIdx.x, nat = w.nat;
+    const int t = threadIdx.x, nat = w.nat, ncell = g.ncell;
+    for (int x = t; x < max(nat, ncell); x += kSmallSortThreads) prep_one(m, g, w, nullptr, x);
+    __syncthreads();
     for (int c = t; c <= ncell; c += kSmallSortThreads) start[c] = 0;
     __syncthreads();
     for (int i = t; i < nat; i += kSmallSortThreads) slot[i] = atomicAdd(&start[w.cellkey[i]], 1);
@@ -286,10 +295,10 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(Work w, int nc
     }
 }
 
-void launch_sort_small(Work &w, int ncell, int nel, int *slot, int *tmp, hipStream_t s)
+void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, hipStream_t s)
 {
-    if (w.nat > 0)
-        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, w, ncell, nel, slot, tmp);
+    if (w.nat > 0 || g.ncell > 0)
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, m, g, w, m.nel, slot, tmp);
 }
 
 // Atoms ordered by element, in index order within an element (the tiles of
