@@ -39,6 +39,8 @@ PARAMS = os.path.join(PKG_DIR, "data", "perovskite")
 # same symmetry functions and network shapes with repulsive weights: velocity
 # Verlet on it conserves energy (on the random weights it does not)
 PARAMS_MD = os.path.join(PKG_DIR, "data", "perovskite_md")
+# the same potential with the cosine cutoff (cutoff_type 1), its scaling from the oracle
+PARAMS_COS = os.path.join(PKG_DIR, "data", "perovskite_cos")
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}      # MI355X FP64 / FP32 vector peaks
 FORCE_RMSE_BOUND = 1e-5                           # Ha/bohr
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "perovskite16_hbm_traffic.txt")
@@ -117,11 +119,38 @@ def algorithmic_flop(h, s, ent):
     return flop, dict(N_pair=n_pair, N_G2=n_g2, N_trip=n_trip, N_G4=n_g4, mlp=mlp)
 
 
-def cpu_baseline(s, nrep=2):
+def host_cores():
+    """The cores this process may use: its affinity mask, limited by the
+    cgroup's CPU quota where one is set (cpu.max), with nproc and the CPU
+    model for the record."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = min(affinity, quota) if quota else affinity
+    return usable, {"nproc": os.cpu_count(), "affinity": affinity, "cgroup_quota_cores": quota, "cpu_model": model}
+
+
+def cpu_baseline(s, nrep=5):
     """The reference (oracle/_ref/refdump, built from the reference's sources)
-    timed on this host's cores; falls back to the oracle's C restatement."""
+    timed on every core this host gives the process (affinity and cgroup
+    quota); falls back to the oracle's C restatement."""
     import resource
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, host = host_cores()
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="64M")
 
     def big_stack():
@@ -156,13 +185,14 @@ def cpu_baseline(s, nrep=2):
             kind = "port"
     t = float(np.median(secs))
     return {"value": s.nat / t, "unit": "atoms/s", "cores": threads, "kind": kind,
-            "sample": f"the same {s.nat}-atom cell, FP64 E+F+dE/dh, 1 warm-up + median of {nrep} "
-                      f"calls of hdnnpEnergyAndForces, OMP_NUM_THREADS={threads}",
-            "seconds_per_eval": t}
+            "sample": f"the same {s.nat}-atom cell (tanh^3 cutoff, the reference's only one), FP64 E+F+dE/dh, "
+                      f"1 warm-up + median of {nrep} calls of hdnnpEnergyAndForces, OMP_NUM_THREADS={threads} "
+                      f"on {host['cpu_model']}",
+            "seconds_per_eval": t, "seconds_all": secs, "host": host}
 
 
-def run_mode(pkg, torch, s, precision, steps, warmup):
-    h = pkg.parseRuNNerInput(PARAMS, precision)
+def run_mode(pkg, torch, s, precision, steps, warmup, params=PARAMS):
+    h = pkg.parseRuNNerInput(params, precision)
     dev = torch.device("cuda:0")
     Z = torch.from_numpy(s.Z.astype(np.int32)).to(dev)
     pos = torch.from_numpy(np.ascontiguousarray(s.pos)).to(dev)
@@ -182,6 +212,24 @@ def run_mode(pkg, torch, s, precision, steps, warmup):
     wall = time.perf_counter() - t0
     return h, dict(E=E, F=force.cpu().numpy(), dEdLat=L, wall=wall, lat_ms=np.array(lat_ms),
                    stage=np.array(stage))
+
+
+def mixed_errors(pkg, s, rm, r64):
+    """Errors of a mixed-precision result against the FP64 one of the same
+    structure: force RMSE and max |dF| (Ha/bohr), |dE| per atom (Ha), and the
+    largest stress difference, sigma = -dEdLat lat^T / det(lat) (Ha/bohr^3),
+    absolute and relative to the largest FP64 stress component."""
+    dF = rm["F"] - r64["F"]
+    out = {"force_rmse": float(np.sqrt(np.mean(dF ** 2))), "force_max": float(np.abs(dF).max()),
+           "energy_err_per_atom": abs(rm["E"] - r64["E"]) / s.nat}
+    if s.periodic:
+        sm = pkg.dEdLat2Stress(s.lat, rm["dEdLat"])
+        s64 = pkg.dEdLat2Stress(s.lat, r64["dEdLat"])
+        out["stress_err"] = float(np.abs(sm - s64).max())
+        out["stress_err_rel"] = float(np.abs(sm - s64).max() / np.abs(s64).max())
+        # the stress of a coherent force error at the 1e-5 Ha/bohr bound over the cutoff
+        out["stress_bound"] = FORCE_RMSE_BOUND * 12.0 * s.nat / abs(np.linalg.det(np.asarray(s.lat).reshape(3, 3)))
+    return out
 
 
 def bench_batch(args):
@@ -220,12 +268,29 @@ def bench_batch(args):
         res[mode] = {"atoms_per_s": nat / t, "structures_per_s": len(ss) / t, "ms_per_batch": t * 1e3,
                      "p99_ms": float(np.percentile(lat, 99)) * 1e3,
                      "autotune": {"atoms_per_wave": tile, "sf_ms_per_tile": tile_ms},
-                     "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy()}
+                     "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy(), "E": E, "L": L}
         print(f"{mode}: {t * 1e3:.3f} ms per batch -> {nat / t:.4g} atoms/s, {len(ss) / t:.4g} structures/s; "
               f"stage ms {res[mode]['stage_ms']}", file=sys.stderr, flush=True)
     rmse = None
+    per_kind = None
     if "fp64" in res and "mixed" in res:
         rmse = float(np.sqrt(np.mean((res["mixed"]["F"] - res["fp64"]["F"]) ** 2)))
+        per_kind = {}
+        off = np.concatenate([[0], np.cumsum(nat_s)])
+        for kind, periodic in (("periodic", True), ("cluster", False)):
+            ks = [k for k in range(len(ss)) if bool(ss[k].periodic) == periodic]
+            rows = np.concatenate([np.arange(off[k], off[k + 1]) for k in ks])
+            dF = res["mixed"]["F"][rows] - res["fp64"]["F"][rows]
+            e = {"structures": len(ks), "force_rmse": float(np.sqrt(np.mean(dF ** 2))),
+                 "force_max": float(np.abs(dF).max()),
+                 "energy_err_per_atom_max": float(max(abs(res["mixed"]["E"][k] - res["fp64"]["E"][k]) / ss[k].nat
+                                                      for k in ks))}
+            if periodic:
+                ds = [np.abs(pkg.dEdLat2Stress(ss[k].lat, res["mixed"]["L"][k]) -
+                             pkg.dEdLat2Stress(ss[k].lat, res["fp64"]["L"][k])).max() for k in ks]
+                e["stress_err_max"] = float(max(ds))
+            per_kind[kind] = e
+        print(f"mixed vs fp64 per kind: {per_kind}", file=sys.stderr, flush=True)
     head = "mixed" if ("mixed" in res and rmse is not None and rmse <= FORCE_RMSE_BOUND) else next(iter(res))
     out = {"metric": "atoms/s (E+F+stress), batched independent structures", "value": res[head]["atoms_per_s"],
            "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -234,8 +299,8 @@ def bench_batch(args):
            "data": "synthetic (seeded structures, seeded random-weight potential)",
            "config": {"workload": f"batch of {len(ss)} structures ({nat} atoms): perovskite n1xn2xn3 (n in 1..4) "
                                   f"and 8-64 atom clusters, atoms resident in device memory", "precision": head},
-           "force_rmse_mixed_vs_fp64": rmse,
-           "modes": {m: {k: v for k, v in r.items() if k != "F"} for m, r in res.items()}}
+           "force_rmse_mixed_vs_fp64": rmse, "mixed_vs_fp64": per_kind,
+           "modes": {m: {k: v for k, v in r.items() if k not in ("F", "E", "L")} for m, r in res.items()}}
     print(json.dumps(out), flush=True)
 
 
@@ -400,6 +465,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=16, help="supercell edge (16 -> 49,152 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cosine", action="store_true", help="skip the cosine-cutoff variant")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--workload", default="cell", choices=["cell", "batch", "serve"],
@@ -445,8 +511,24 @@ def main():
     ent = h0.neighbor_entries()
     flop, counts = algorithmic_flop(h0, s, ent)
     rmse = None
+    errors = None
     if "fp64" in res and "mixed" in res:
         rmse = float(np.sqrt(np.mean((res["mixed"]["F"] - res["fp64"]["F"]) ** 2)))
+        errors = mixed_errors(pkg, s, res["mixed"], res["fp64"])
+        print(f"mixed vs fp64: {errors}", file=sys.stderr, flush=True)
+    # the cosine cutoff (cutoff_type 1) the configuration names, beside the
+    # reference's tanh^3: same cell, same network, its own scaling
+    cosine = {}
+    if not args.no_cosine:
+        cres = {}
+        for mode in args.modes.split(","):
+            _, r = run_mode(pkg, torch, s, mode, args.steps, args.warmup, params=PARAMS_COS)
+            cres[mode] = r
+            cosine[mode] = {"atoms_per_s": s.nat * args.steps / r["wall"], "ms_per_step": r["wall"] / args.steps * 1e3,
+                            "stage_ms": np.round(r["stage"].mean(axis=0), 4).tolist()}
+            print(f"cosine cutoff, {mode}: {cosine[mode]['atoms_per_s']:.4g} atoms/s", file=sys.stderr, flush=True)
+        if "fp64" in cres and "mixed" in cres:
+            cosine["mixed_vs_fp64"] = mixed_errors(pkg, s, cres["mixed"], cres["fp64"])
     head = "mixed" if ("mixed" in res and rmse is not None and rmse <= FORCE_RMSE_BOUND) else \
         ("fp64" if "fp64" in res else "mixed")
     r = res[head]
@@ -463,11 +545,14 @@ def main():
         "data": "synthetic (seeded perovskite supercell, seeded random-weight potential)",
         "config": {"workload": f"perovskite {args.cells}^3 supercell, {s.nat} atoms, 5 elements, "
                                f"80 SFs/element (40 G2 + 40 G4), MLP 80-25-25-1, rc 12 bohr, "
+                               f"cutoff tanh^3 (RuNNer cutoff_type 2, the reference's), "
                                f"E+F+dE/dh, precision {head}",
-                   "atoms": s.nat, "precision": head},
+                   "atoms": s.nat, "precision": head, "cutoff_type": 2},
         "latency_ms": {"p50": float(np.percentile(r["lat_ms"], 50)),
                        "p99": float(np.percentile(r["lat_ms"], 99))},
         "force_rmse_mixed_vs_fp64": rmse,
+        "mixed_vs_fp64": errors,
+        "cosine_cutoff": cosine or None,
         "modes": {m: {"atoms_per_s": s.nat * args.steps / v["wall"],
                       "ms_per_step": v["wall"] / args.steps * 1e3,
                       "stage_ms": np.round(v["stage"].mean(axis=0), 4).tolist()} for m, v in res.items()},

@@ -134,6 +134,33 @@ def _p(a, t):
     return a.ctypes.data_as(t) if a is not None else None
 
 
+def _check_device_inputs(torch, nat, Z, pos, force):
+    """The device entry points read Z as int32[nat] and pos / write force as
+    float64[nat][3] at their data pointers: anything else would be misread or
+    overrun, so it is refused here, before any launch."""
+    for name, t in (("Z", Z), ("pos", pos), ("force", force)):
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a torch tensor")
+    if Z.dtype != torch.int32:
+        raise ValueError(f"Z must be int32 (got {Z.dtype})")
+    for name, t in (("pos", pos), ("force", force)):
+        if t.dtype != torch.float64:
+            raise ValueError(f"{name} must be float64 (got {t.dtype})")
+    if tuple(Z.shape) != (nat,):
+        raise ValueError(f"Z has shape {tuple(Z.shape)}, expected ({nat},)")
+    for name, t in (("pos", pos), ("force", force)):
+        if tuple(t.shape) != (nat, 3):
+            raise ValueError(f"{name} has shape {tuple(t.shape)}, expected ({nat}, 3)")
+    for name, t in (("Z", Z), ("pos", pos), ("force", force)):
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    for name, t in (("Z", Z), ("pos", pos), ("force", force)):
+        if t.device.type != "cuda":
+            raise ValueError(f"{name} must be in GPU memory (got {t.device})")
+    if not (Z.device == pos.device == force.device):
+        raise ValueError(f"Z, pos and force are on different devices ({Z.device}, {pos.device}, {force.device})")
+
+
 class Hdnnp:
     """Handle of one potential (the reference's type(hdnnp), hdnnps.f90:13-21)."""
 
@@ -270,15 +297,25 @@ class Hdnnp:
 
     def evaluate_batch_torch(self, nat, Z, pos, periodic, lat, force=None, stream=None):
         """hdnnp_evaluate_batch_device: nat/periodic (nstruct,), lat (nstruct, 3, 3)
-        host arrays; Z (int32), pos (float64 [sum nat, 3]) torch tensors on the GPU.
-        Returns (E[nstruct], force tensor, dEdLat[nstruct, 3, 3])."""
+        host arrays; Z (int32 [sum nat]), pos and force (float64 [sum nat, 3])
+        contiguous torch tensors on one GPU. Returns (E[nstruct], force tensor,
+        dEdLat[nstruct, 3, 3]). Raises ValueError for any other input before
+        anything is launched."""
         import torch
-        nat = np.ascontiguousarray(nat, dtype=np.int32)
-        per = np.ascontiguousarray(periodic, dtype=np.int32)
-        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        nat = np.ascontiguousarray(nat, dtype=np.int32).reshape(-1)
+        per = np.ascontiguousarray(periodic, dtype=np.int32).reshape(-1)
         ns = nat.shape[0]
+        if per.shape[0] != ns:
+            raise ValueError(f"periodic has {per.shape[0]} entries for {ns} structures")
+        if (nat < 0).any():
+            raise ValueError("negative atom count")
+        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        if lat.size != 9 * ns:
+            raise ValueError(f"lat must hold 9 values per structure ({9 * ns}), got {lat.size}")
+        total = int(nat.sum())
         if force is None:
-            force = torch.empty((int(nat.sum()), 3), dtype=torch.float64, device=pos.device)
+            force = torch.empty((total, 3), dtype=torch.float64, device=getattr(pos, "device", None))
+        _check_device_inputs(torch, total, Z, pos, force)
         E = np.zeros(ns)
         L = np.zeros((ns, 3, 3))
         self._check(self.lib.hdnnp_evaluate_batch_device(
@@ -287,13 +324,19 @@ class Hdnnp:
             ctypes.c_void_p(stream or 0)))
         return E, force, L
 
-    def evaluate_device(self, Z_dev, pos_dev, force_dev, periodic: bool, lat, stream=None):
-        """Device-resident evaluation: Z_dev/pos_dev/force_dev are device
-        pointers (ints), lat a (3,3) host array. Returns (E, dEdLat)."""
+    def evaluate_device(self, nat: int, Z_dev, pos_dev, force_dev, periodic: bool, lat, stream=None):
+        """hdnnp_evaluate_device on raw device pointers (ints): Z_dev int32[nat],
+        pos_dev and force_dev float64[nat][3], all in device memory; lat a (3,3)
+        host array. The caller guarantees the buffers hold nat atoms. Returns
+        (E, dEdLat)."""
+        nat = int(nat)
+        if nat < 0:
+            raise ValueError("nat < 0")
         lat = np.ascontiguousarray(lat, dtype=np.float64)
+        if lat.size != 9:
+            raise ValueError("lat must hold 9 values")
         E = ctypes.c_double()
         L = np.zeros((3, 3))
-        nat = self._nat_dev
         self._check(self.lib.hdnnp_evaluate_device(self.ctx, nat, ctypes.c_void_p(Z_dev), ctypes.c_void_p(pos_dev),
                                                    int(bool(periodic)), _p(lat, _dp), ctypes.byref(E),
                                                    ctypes.c_void_p(force_dev), _p(L, _dp),
@@ -301,13 +344,17 @@ class Hdnnp:
         return E.value, L
 
     def evaluate_torch(self, Z, pos, periodic: bool, lat, force=None, stream=None):
-        """Z (int32), pos (float64 [nat,3]) torch tensors on the GPU; returns (E, force, dEdLat)."""
+        """Z (int32 [nat]), pos and force (float64 [nat, 3]) contiguous torch
+        tensors on one GPU; returns (E, force, dEdLat). Raises ValueError for
+        any other input before anything is launched."""
         import torch
+        if not isinstance(Z, torch.Tensor) or Z.dim() != 1:
+            raise ValueError("Z must be a one-dimensional torch tensor")
         nat = int(Z.shape[0])
         if force is None:
-            force = torch.empty((nat, 3), dtype=torch.float64, device=pos.device)
-        self._nat_dev = nat
-        E, L = self.evaluate_device(Z.data_ptr(), pos.data_ptr(), force.data_ptr(), periodic, lat, stream)
+            force = torch.empty((nat, 3), dtype=torch.float64, device=getattr(pos, "device", None))
+        _check_device_inputs(torch, nat, Z, pos, force)
+        E, L = self.evaluate_device(nat, Z.data_ptr(), pos.data_ptr(), force.data_ptr(), periodic, lat, stream)
         return E, force, L
 
     # --- stage outputs of the last evaluation ------------------------------------

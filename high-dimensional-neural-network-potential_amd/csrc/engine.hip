@@ -27,9 +27,38 @@ namespace hdnnp {
             throw Error(5, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);     \
     } while (0)
 
+std::shared_mutex &capture_mutex()
+{
+    static std::shared_mutex mu;
+    return mu;
+}
+
+static thread_local int t_unsafe_depth = 0;     // nested HipUnsafe scopes of this thread
+static thread_local bool t_capturing = false;   // this thread holds the mutex for a capture
+
+HipUnsafe::HipUnsafe()
+{
+    if (t_capturing) return;
+    counted_ = true;
+    if (t_unsafe_depth++ == 0) {
+        capture_mutex().lock_shared();
+        locked_ = true;
+    }
+}
+
+HipUnsafe::~HipUnsafe()
+{
+    if (locked_) capture_mutex().unlock_shared();
+    if (counted_) --t_unsafe_depth;
+}
+
+HipUnsafe::Capture::Capture() : lk(capture_mutex()) { t_capturing = true; }
+HipUnsafe::Capture::~Capture() { t_capturing = false; }
+
 void DevMem::ensure(size_t b)
 {
     if (b <= bytes && p) return;
+    HipUnsafe guard;
     release();
     const size_t want = std::max<size_t>(256, b + b / 4);
     HIPCHK(hipMalloc(&p, want));
@@ -38,7 +67,10 @@ void DevMem::ensure(size_t b)
 
 void DevMem::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        HipUnsafe guard;
+        (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -164,6 +196,7 @@ void Engine::release_work()
 
 Engine::~Engine()
 {
+    HipUnsafe guard;
     for (auto &d : model_mem_) d.release();
     for (DevMem *d : work_buffers()) d->release();
     misc_.release();
@@ -179,6 +212,7 @@ static const T *upload(std::vector<DevMem> &keep, const std::vector<T> &v)
 {
     keep.emplace_back();
     keep.back().ensure(sizeof(T) * std::max<size_t>(1, v.size()));
+    HipUnsafe guard;
     if (!v.empty()) HIPCHK(hipMemcpy(keep.back().p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return keep.back().as<T>();
 }
@@ -187,8 +221,11 @@ void Engine::init_device()
 {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(5, "no HIP device available");
-    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
+    {
+        HipUnsafe guard;
+        HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
+    }
 
     const HostModel &m = model_;
     DevModel d{};
@@ -414,7 +451,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     // FP64 with one atom per wavefront: pass A hands its compacted triplets to
     // pass B. (In FP32 the list's stores cost pass A about what they save
     // pass B: its barriers wait for them.)
-    if (sizeof(R) == 8 && ga == 1 && (long)nat * w.tcap < (1L << 31)) {
+    if (sizeof(R) == 8 && ga == 1) {
         trip_.ensure(4 * (size_t)nat * w.tcap);
         tcount_.ensure(4 * (size_t)nat);
         w.trip = trip_.as<int>();
@@ -453,6 +490,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
 static void *grow_pinned(void *&p, size_t &cap, size_t bytes)
 {
     if (bytes > cap) {
+        HipUnsafe guard;
         if (p) HIPCHK(hipHostFree(p));
         p = nullptr;
         cap = 0;
@@ -533,16 +571,19 @@ static int entry_capacity(double est, int hint, int lim)
     return cap;
 }
 
-// triplets per atom the value pass keeps for the gradient pass: 0.4 want^2
-// (about 0.6 of the want^2/2 neighbour pairs lie within the cutoff of each
-// other), at most the pairs of lim neighbours; an atom with more falls back
-// to the gradient pass's own compaction
-static int triplet_capacity(double est, int lim)
+// Triplets per atom the value pass keeps for the gradient pass: the pairs of
+// lim neighbours, at most kTripletSlots. An atom with more falls back to the
+// gradient pass's own compaction, which adds its triplets in another order.
+// Whether an atom falls back therefore must not depend on the evaluation it
+// is part of (a batch result equals the structure's result alone, bit for
+// bit): an atom has at most n(n-1)/2 <= lim(lim-1)/2 triplets, so the test
+// count <= min(pairs(lim), kTripletSlots) is count <= kTripletSlots for every
+// lim. 2048 holds the ~700-1,330 triplets of every perovskite atom.
+constexpr int kTripletSlots = 2048;
+static int triplet_capacity(int lim)
 {
-    const double want = std::min(1.2 * est + 8.0, (double)lim);
     const long pairs = (long)lim * (lim - 1) / 2;
-    const long t = std::min(pairs, std::max(256L, (long)(0.4 * want * want)));
-    return (int)((t + 63) / 64 * 64);
+    return (int)std::min<long>(pairs, kTripletSlots);
 }
 
 // the slot stride of a list holding `lim` entries per atom: a power of two
@@ -573,7 +614,7 @@ size_t Engine::footprint_bytes(const Footprint &f) const
     const size_t nat = (size_t)f.nat;
     size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
     b += nat * cap * (16 + 3 * R);                                                           // list slots
-    if (R == 8) b += nat * (4 * (size_t)triplet_capacity(f.est, entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
+    if (R == 8) b += nat * (4 * (size_t)triplet_capacity(entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
     b += (size_t)f.ncell * 12;                                                               // cells
     if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
     b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure
@@ -753,7 +794,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     // and the evaluation is repeated with room for it.
     w.lim = entry_capacity(est, cap_hint_, lds_lim);
     w.cap = slot_stride(w.lim);
-    w.tcap = triplet_capacity(est, w.lim);
+    w.tcap = triplet_capacity(w.lim);
     w.trip = nullptr;               // set by compute when the passes share triplets
     w.tcount = nullptr;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
@@ -1101,11 +1142,18 @@ void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, i
     evaluate_batch_device(1, &nat, Z_dev, pos_dev, &periodic, lat, energy, force_dev, dEdLat, stream);
 }
 
+// a device-to-host copy ordered on the engine's stream (not the legacy one)
+void Engine::d2h(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
 int64_t Engine::neighbor_count()
 {
     if (last_total_ < 0) {
         std::vector<int> cnt(last_nat_);
-        if (last_nat_) HIPCHK(hipMemcpy(cnt.data(), count_.p, 4 * (size_t)last_nat_, hipMemcpyDeviceToHost));
+        if (last_nat_) d2h(cnt.data(), count_.p, 4 * (size_t)last_nat_);
         last_total_ = 0;
         for (int c : cnt) last_total_ += c;
     }
@@ -1119,9 +1167,9 @@ void Engine::get_neighbors(int64_t cap, int32_t *ent)
     if (total == 0) return;
     const size_t slots = (size_t)last_nat_ * last_cap_;
     std::vector<int> cnt(last_nat_), j(slots), sh(slots);
-    HIPCHK(hipMemcpy(cnt.data(), count_.p, 4 * (size_t)last_nat_, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(j.data(), ent_j_.p, 4 * slots, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(sh.data(), ent_shift_.p, 4 * slots, hipMemcpyDeviceToHost));
+    d2h(cnt.data(), count_.p, 4 * (size_t)last_nat_);
+    d2h(j.data(), ent_j_.p, 4 * slots);
+    d2h(sh.data(), ent_shift_.p, 4 * slots);
     int64_t q = 0;
     for (int i = 0; i < last_nat_; ++i)
         for (int n = 0; n < cnt[i]; ++n, ++q) {
@@ -1145,10 +1193,10 @@ void Engine::get_symfunctions(int ld, double *G, double *dEdG, double *Ei)
     std::vector<double> tmp((size_t)nat * ns);
     auto fetch = [&](const DevMem &src, double *dst) {
         if (fp64) {
-            HIPCHK(hipMemcpy(tmp.data(), src.p, 8 * tmp.size(), hipMemcpyDeviceToHost));
+            d2h(tmp.data(), src.p, 8 * tmp.size());
         } else {
             std::vector<float> t(tmp.size());
-            HIPCHK(hipMemcpy(t.data(), src.p, 4 * t.size(), hipMemcpyDeviceToHost));
+            d2h(t.data(), src.p, 4 * t.size());
             for (size_t k = 0; k < t.size(); ++k) tmp[k] = t[k];
         }
         for (int i = 0; i < nat; ++i)
@@ -1159,7 +1207,7 @@ void Engine::get_symfunctions(int ld, double *G, double *dEdG, double *Ei)
     if (Ei) {   // committee mean of the atomic energies
         const int nm = (int)dmm_.size();
         std::vector<double> all((size_t)nat * nm);
-        HIPCHK(hipMemcpy(all.data(), Ei_.p, 8 * all.size(), hipMemcpyDeviceToHost));
+        d2h(all.data(), Ei_.p, 8 * all.size());
         for (int i = 0; i < nat; ++i) {
             double a = 0;
             for (int k = 0; k < nm; ++k) a += all[(size_t)k * nat + i];

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -12,6 +14,30 @@
 #include "runner.hpp"
 
 namespace hdnnp {
+
+// HIP graph capture (MdSystems::replay) and the library's calls that a capture
+// in another thread makes illegal, or that invalidate it: allocations and
+// frees, synchronous copies on the legacy stream, stream, event and graph
+// creation and destruction. A capture holds the mutex exclusively; those calls
+// hold it shared through HipUnsafe. The thread that captures takes no shared
+// lock (it already holds the mutex); nested HipUnsafe scopes lock once.
+std::shared_mutex &capture_mutex();
+class HipUnsafe {
+  public:
+    HipUnsafe();
+    ~HipUnsafe();
+    HipUnsafe(const HipUnsafe &) = delete;
+    HipUnsafe &operator=(const HipUnsafe &) = delete;
+    // the calling thread holds the mutex exclusively for a capture
+    struct Capture {
+        Capture();
+        ~Capture();
+        std::unique_lock<std::shared_mutex> lk;
+    };
+
+  private:
+    bool locked_ = false, counted_ = false;
+};
 
 // grow-only device allocation
 struct DevMem {
@@ -71,6 +97,7 @@ class Engine {
 
   private:
     void init_device();
+    void d2h(void *dst, const void *src, size_t bytes);
     // The structures st (atoms concatenated in Z_dev / pos_dev / force_dev).
     // retry: how often an evaluation may be repeated with a larger neighbour capacity
     void run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, double *force_dev,

@@ -207,13 +207,24 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
 // k_elem_place. The per-atom set-up of k_prep runs first, in the same
 // workgroup. Needs nat <= kSmallSortAtoms and ncell <= kSmallSortCells;
 // slot and tmp hold nat ints each.
+// Ranking an atom within its cell by a scan of the cell costs k loads for a
+// cell of k atoms, k^2 for the cell. When the fullest cell holds more than
+// kScanRankMax atoms, the atoms are placed instead in rounds of the
+// workgroup's size in index order: an atom's rank among the round's atoms of
+// its cell comes from a scan of the round's cell keys in LDS, and the first
+// of them takes the round's positions of its cell from a running counter
+// (cell_end, which ends at the cell's end). That costs nat loads from LDS per
+// thread whatever the distribution.
+constexpr int kScanRankMax = 256;
 __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Geo g, Work w, int nel, int *slot,
                                                                   int *tmp)
 {
     __shared__ int start[kSmallSortCells + 1];
     __shared__ int part[kSmallSortThreads];
+    __shared__ int fullest;
     const int t = threadIdx.x, nat = w.nat, ncell = g.ncell;
     for (int x = t; x < max(nat, ncell); x += kSmallSortThreads) prep_one(m, g, w, nullptr, x);
+    if (t == 0) fullest = 0;
     __syncthreads();
     for (int c = t; c <= ncell; c += kSmallSortThreads) start[c] = 0;
     __syncthreads();
@@ -222,9 +233,13 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Ge
     // exclusive scan of the counts: thread t owns cells [c0, c1)
     const int per = (ncell + kSmallSortThreads - 1) / kSmallSortThreads;
     const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
-    int sum = 0;
-    for (int c = c0; c < c1; ++c) sum += start[c];
+    int sum = 0, most = 0;
+    for (int c = c0; c < c1; ++c) {
+        sum += start[c];
+        most = max(most, start[c]);
+    }
     part[t] = sum;
+    atomicMax(&fullest, most);
     __syncthreads();
     for (int o = 1; o < kSmallSortThreads; o <<= 1) {
         const int v = t >= o ? part[t - o] : 0;
@@ -232,23 +247,52 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Ge
         part[t] += v;
         __syncthreads();
     }
+    const bool by_scan = fullest <= kScanRankMax;   // workgroup-uniform
     int run = part[t] - sum;
     for (int c = c0; c < c1; ++c) {
         const int n = start[c];
         start[c] = run;
         w.cell_start[c] = run;
+        w.cell_end[c] = by_scan ? run + n : run;   // else a running counter from the cell's start
         run += n;
-        w.cell_end[c] = run;
     }
     if (t == 0) start[ncell] = nat;
+    // the running counters are advanced by atomics, which act in L2: the
+    // stores above must have reached it before any thread adds to them
+    if (!by_scan) __threadfence();
     __syncthreads();
-    for (int i = t; i < nat; i += kSmallSortThreads) tmp[start[w.cellkey[i]] + slot[i]] = i;
-    __syncthreads();
-    for (int i = t; i < nat; i += kSmallSortThreads) {
-        const int c = w.cellkey[i], s0 = start[c], s1 = start[c + 1];
-        int r = 0;
-        for (int q = s0; q < s1; ++q) r += tmp[q] < i;
-        w.cell_atoms[s0 + r] = i;
+    if (by_scan) {
+        for (int i = t; i < nat; i += kSmallSortThreads) tmp[start[w.cellkey[i]] + slot[i]] = i;
+        __syncthreads();
+        for (int i = t; i < nat; i += kSmallSortThreads) {
+            const int c = w.cellkey[i], s0 = start[c], s1 = start[c + 1];
+            int r = 0;
+            for (int q = s0; q < s1; ++q) r += tmp[q] < i;
+            w.cell_atoms[s0 + r] = i;
+        }
+    } else {
+        __shared__ int rbase[kSmallSortThreads];   // at a cell's first atom of the round: its base
+        int *rkey = part;                           // the round's cell keys
+        for (int r0 = 0; r0 < nat; r0 += kSmallSortThreads) {
+            const int i = r0 + t, n = min(kSmallSortThreads, nat - r0);
+            const int c = i < nat ? w.cellkey[i] : -1;
+            __syncthreads();                 // the previous round has read rkey
+            rkey[t] = c;
+            __syncthreads();
+            int rank = 0, tot = 0, first = t;
+            if (c >= 0)
+                for (int j = 0; j < n; ++j) {
+                    const bool same = rkey[j] == c;
+                    tot += same;
+                    if (same && j < t) {
+                        ++rank;
+                        first = min(first, j);
+                    }
+                }
+            if (c >= 0 && rank == 0) rbase[t] = atomicAdd(&w.cell_end[c], tot);
+            __syncthreads();
+            if (c >= 0) w.cell_atoms[rbase[first] + rank] = i;
+        }
     }
 
     // by element (the MLP tiles)

@@ -69,12 +69,17 @@ void MdSystems::init(int nsys, const int *nat, const int *Z, const double *pos, 
     d2_.ensure(8 * kMaxChunk);
     ctr_.ensure(256);
     if (const char *g = std::getenv("HDNNP_MD_GRAPHS")) graphs_ = std::strcmp(g, "0") != 0;
-    MDCHK(hipHostMalloc((void **)&d2_host_, 8 * kMaxChunk, hipHostMallocDefault));
-    MDCHK(hipMemcpy(Z_.p, Z, 4 * (size_t)nat_, hipMemcpyHostToDevice));
-    MDCHK(hipMemcpy(pos_.p, pos, 24 * (size_t)nat_, hipMemcpyHostToDevice));
-    MDCHK(hipMemcpy(mom_.p, mom, 24 * (size_t)nat_, hipMemcpyHostToDevice));
-    MDCHK(hipMemcpy(mass_.p, mass, 8 * (size_t)nat_, hipMemcpyHostToDevice));
-    MDCHK(hipMemcpy(off_.p, off.data(), 4 * (size_t)(nsys + 1), hipMemcpyHostToDevice));
+    {
+        HipUnsafe guard;
+        MDCHK(hipHostMalloc((void **)&d2_host_, 8 * kMaxChunk, hipHostMallocDefault));
+    }
+    const hipStream_t s = eng_.stream();          // copies ordered on it, not on the legacy stream
+    MDCHK(hipMemcpyAsync(Z_.p, Z, 4 * (size_t)nat_, hipMemcpyHostToDevice, s));
+    MDCHK(hipMemcpyAsync(pos_.p, pos, 24 * (size_t)nat_, hipMemcpyHostToDevice, s));
+    MDCHK(hipMemcpyAsync(mom_.p, mom, 24 * (size_t)nat_, hipMemcpyHostToDevice, s));
+    MDCHK(hipMemcpyAsync(mass_.p, mass, 8 * (size_t)nat_, hipMemcpyHostToDevice, s));
+    MDCHK(hipMemcpyAsync(off_.p, off.data(), 4 * (size_t)(nsys + 1), hipMemcpyHostToDevice, s));
+    MDCHK(hipStreamSynchronize(s));
     epot_.assign(nsys, 0.0);
     ekin_h_.assign(nsys, 0.0);
     forces(nullptr);     // initial forces (the reference's call before the loop)
@@ -84,6 +89,7 @@ MdSystems::~MdSystems() { release(); }
 
 void MdSystems::release()
 {
+    HipUnsafe guard;
     for (DevMem *d : {&Z_, &pos_, &mom_, &mass_, &F_, &Fold_, &posb_, &wposb_, &off_, &ekin_, &maxd2_, &snap_, &d2_,
                       &ctr_})
         d->release();
@@ -173,6 +179,9 @@ bool MdSystems::replay(int n)
     hipStream_t s = eng_.stream();
     std::vector<char> key = eng_.launch_key();
     if (!exec_ || key != key_) {
+        // no other thread allocates, frees or copies synchronously through
+        // this library while the capture runs (HipUnsafe)
+        HipUnsafe::Capture excl;
         if (exec_) (void)hipGraphExecDestroy(exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
         exec_ = nullptr;
@@ -271,10 +280,10 @@ void MdSystems::step(int nsteps, double *epot, double *ekin)
 void MdSystems::get(double *pos, double *mom, double *force)
 {
     hipStream_t s = eng_.stream();
+    if (pos) MDCHK(hipMemcpyAsync(pos, pos_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost, s));
+    if (mom) MDCHK(hipMemcpyAsync(mom, mom_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost, s));
+    if (force) MDCHK(hipMemcpyAsync(force, F_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost, s));
     MDCHK(hipStreamSynchronize(s));
-    if (pos) MDCHK(hipMemcpy(pos, pos_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
-    if (mom) MDCHK(hipMemcpy(mom, mom_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
-    if (force) MDCHK(hipMemcpy(force, F_.p, 24 * (size_t)nat_, hipMemcpyDeviceToHost));
 }
 
 }  // namespace hdnnp

@@ -10,6 +10,7 @@ files are copied as data fixtures into tests/golden/methylammonium/.
     python tests/golden/make_golden.py
     python tests/golden/make_golden.py md-potential   # only data/perovskite_md
     python tests/golden/make_golden.py readers        # only tests/golden/readers.npz
+    python tests/golden/make_golden.py cos-potential  # only data/perovskite_cos
 """
 from __future__ import annotations
 
@@ -34,6 +35,7 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 MA_DIR = os.path.join(GOLD, "methylammonium")
 PEROV_DIR = os.path.join(PKG, "data", "perovskite")
 MD_DIR = os.path.join(PKG, "data", "perovskite_md")
+COS_DIR = os.path.join(PKG, "data", "perovskite_cos")
 KB = 3.166811965e-6           # constants.f90:10
 
 
@@ -196,6 +198,45 @@ def make_md_potential():
     print(f"perovskite_md: output scale {0.02 / fmax:.6g}, max force {f2:.4g} Ha/bohr")
 
 
+def make_cosine_potential():
+    """data/perovskite_cos: the perovskite potential with the cosine cutoff
+    (RuNNer cutoff_type 1, the form the 49,152-atom configuration names) and
+    the same weights. The reference evaluates only tanh^3, so the scaling of
+    this variant comes from the oracle (oracle/runner.py, pinned against the
+    reference for tanh^3 and by closed forms for the cosine) on the same
+    scaling set: the 6^3 cell and 64 C/N/H clusters."""
+    import runner
+    os.makedirs(COS_DIR, exist_ok=True)
+    nsf = 80
+    for fn in os.listdir(PEROV_DIR):
+        if fn.startswith("weights."):
+            shutil.copy(os.path.join(PEROV_DIR, fn), os.path.join(COS_DIR, fn))
+    wl.write_input_nn(os.path.join(COS_DIR, "input.nn"), scale=False, cutoff_type=1)
+    dummy = [np.tile([0.0, 1.0, 0.0], (nsf, 1)) for _ in wl.ELEMENTS]
+    wl.write_scaling(os.path.join(COS_DIR, "scaling.data"), dummy)
+    orc = runner.Oracle(runner.read_potential(os.path.join(COS_DIR, "")))
+    structs = [wl.perovskite(6, 6, 6)] + [wl.cluster(8 + (k * 7) % 57, seed=5000 + k) for k in range(64)]
+    rows = {z: [] for z in wl.ELEMENTS}
+    for s in structs:
+        G = orc.evaluate(s, want_sf=True, nthreads=8)["G"]
+        for z in wl.ELEMENTS:
+            if np.any(s.Z == z):
+                rows[z].append(G[s.Z == z][:, :nsf])
+    stats = []
+    for z in wl.ELEMENTS:
+        G = np.concatenate(rows[z])
+        mn, mx, me = G.min(0), G.max(0), G.mean(0)
+        mx = np.where(mx - mn > 0, mx, mn + 1.0)
+        stats.append(np.stack([mn, mx, me], axis=1))
+    # atom energies are zero in this potential; the energy range line is kept
+    # from the tanh^3 variant (parsed, unused)
+    with open(os.path.join(PEROV_DIR, "scaling.data")) as f:
+        erange = [float(t) for t in f.read().split("\n")[-2].split()]
+    wl.write_scaling(os.path.join(COS_DIR, "scaling.data"), stats, (erange[0], erange[1]))
+    wl.write_input_nn(os.path.join(COS_DIR, "input.nn"), scale=True, cutoff_type=1)
+    print("perovskite_cos written")
+
+
 READER_CASES = (("fhiaims", "geometry.in"), ("fhiaims", "molecule.in"), ("ascii", "cell_ang.ascii"),
                 ("ascii", "cell_bohr.ascii"), ("asciiang", "cell_bohr.ascii"), ("poscar", "POSCAR"),
                 ("data", "input.data"), ("data", "big.data"))
@@ -337,5 +378,7 @@ if __name__ == "__main__":
         make_md_potential()
     elif sys.argv[1:] == ["readers"]:         # only tests/golden/readers.npz
         make_reader_goldens()
+    elif sys.argv[1:] == ["cos-potential"]:   # only data/perovskite_cos
+        make_cosine_potential()
     else:
         main()

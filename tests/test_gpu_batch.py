@@ -83,14 +83,20 @@ def test_device_batch_equals_host_batch(pkg, wl):
 def test_mixed_batch_force_error_bound(pkg, wl):
     """Mixed precision over a batch of perovskite cells and C/N/H clusters:
     force RMSE against FP64 within 1e-5 Ha/bohr for either kind (the bound of
-    the headline metric), and energies within 1e-5 Ha per atom."""
+    the headline metric), energies within 1e-5 Ha per atom, and the stress of
+    every periodic structure within 1e-5 Ha/bohr * rc * nat / V (the stress
+    of a coherent force error at the force bound over the cutoff radius)."""
     ss = wl.batch_mix(400, seed=2024)
-    E64, F64, _ = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64").energy_and_forces_batch(ss)
-    E32, F32, _ = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed").energy_and_forces_batch(ss)
+    E64, F64, L64 = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64").energy_and_forces_batch(ss)
+    E32, F32, L32 = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed").energy_and_forces_batch(ss)
     for periodic in (True, False):
         ks = [k for k, s in enumerate(ss) if bool(s.periodic) == periodic]
         d = np.concatenate([(F32[k] - F64[k]).ravel() for k in ks])
         rmse = float(np.sqrt(np.mean(d ** 2)))
         de = max(abs(E32[k] - E64[k]) / ss[k].nat for k in ks)
-        print(f"periodic {periodic}: {len(ks)} structures, F rmse {rmse:.3e}, max dE/nat {de:.3e}", flush=True)
-        assert rmse <= 1e-5 and de <= 1e-5
+        # stress against its bound 1e-5 Ha/bohr * rc * nat / V, as a ratio
+        ds = max(np.abs(pkg.dEdLat2Stress(ss[k].lat, L32[k]) - pkg.dEdLat2Stress(ss[k].lat, L64[k])).max()
+                 / (1e-5 * 12.0 * ss[k].nat / abs(np.linalg.det(ss[k].lat))) for k in ks) if periodic else 0.0
+        print(f"periodic {periodic}: {len(ks)} structures, F rmse {rmse:.3e}, max dE/nat {de:.3e}, "
+              f"stress diff / bound {ds:.3f}", flush=True)
+        assert rmse <= 1e-5 and de <= 1e-5 and ds <= 1.0

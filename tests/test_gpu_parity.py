@@ -5,7 +5,12 @@ Tolerances (FP64 path): energy |dE| <= 1e-10 max(1,|E|); forces
 |dF| <= 1e-9 Ha/bohr; dEdLat |d| <= 1e-9 max(1,|dEdLat|); scaled SFs and
 dE/dG <= 1e-11 absolute. Neighbour sets: bit-exact (sorted (i, j, L) lists).
 Mixed path (FP32 symmetry functions and MLP, FP64 wrap/list/sums): force RMSE
-<= 1e-5 Ha/bohr and max |dF| <= 2e-4 Ha/bohr vs FP64; |dE|/nat <= 1e-5 Ha.
+<= 1e-5 Ha/bohr and max |dF| <= 2e-4 Ha/bohr vs FP64; |dE|/nat <= 1e-5 Ha;
+stress (dEdLat2Stress) max |d sigma| <= 1e-5 Ha/bohr * rc * nat / V, the stress
+that a coherent force error at the force bound acting over the cutoff radius
+would give (8.6e-7 Ha/bohr^3 = 25 MPa at the perovskite's density; the
+errors measured are 2e-10 to 2e-9 Ha/bohr^3 on these cells and up to 3e-7 on
+the 12-atom cells of the batches).
 """
 from __future__ import annotations
 
@@ -19,6 +24,12 @@ import pytest
 from conftest import EVAL_CASES, MA_PARAMS, PEROV_PARAMS, POTENTIALS, ROOT, GOLD, golden, golden_structure
 
 pytestmark = pytest.mark.gpu
+
+
+
+def stress_tol(s, rc=12.0):
+    """Ha/bohr^3, mixed against FP64: 1e-5 Ha/bohr * rc * nat / V"""
+    return 1e-5 * rc * s.nat / abs(np.linalg.det(np.asarray(s.lat).reshape(3, 3)))
 
 _HANDLES = {}
 
@@ -67,11 +78,14 @@ def test_mixed_matches_fp64(pkg, wl, case):
     E, F, L = h.energy_and_forces(s)
     dF = F - d["force"]
     rmse = float(np.sqrt(np.mean(dF ** 2)))
+    dsig = np.abs(pkg.dEdLat2Stress(s.lat, L) - pkg.dEdLat2Stress(s.lat, d["dEdLat"])).max()
     print(f"mixed {case}: dE/nat={abs(E - float(d['energy'])) / s.nat:.3e} F rmse={rmse:.3e} "
-          f"max={np.abs(dF).max():.3e} dEdLat max={np.abs(L - d['dEdLat']).max():.3e}", flush=True)
+          f"max={np.abs(dF).max():.3e} dEdLat max={np.abs(L - d['dEdLat']).max():.3e} "
+          f"stress max={dsig:.3e} Ha/bohr^3", flush=True)
     assert abs(E - float(d["energy"])) / s.nat <= 1e-5
     assert rmse <= 1e-5
     assert np.abs(dF).max() <= 2e-4
+    assert dsig <= stress_tol(s)
     ent = canon(h.neighbor_entries())
     assert hashlib.sha256(ent.astype("<i4").tobytes()).hexdigest() == str(d["neighbors_sha256"])
 
@@ -124,9 +138,12 @@ def test_large_cell_against_oracle(pkg, wl, oracle, oracles):
     hm = handle(pkg, "perovskite", "mixed")
     Em, Fm, Lm = hm.energy_and_forces(s)
     rmse = float(np.sqrt(np.mean((Fm - F) ** 2)))
+    dsig = np.abs(pkg.dEdLat2Stress(s.lat, Lm) - pkg.dEdLat2Stress(s.lat, L)).max()
     print(f"C3 mixed vs fp64: F rmse {rmse:.3e} max {np.abs(Fm - F).max():.3e} "
-          f"dE/nat {abs(Em - E) / s.nat:.3e}", flush=True)
+          f"dE/nat {abs(Em - E) / s.nat:.3e} stress max {dsig:.3e} Ha/bohr^3", flush=True)
     assert rmse <= 1e-5
+    assert abs(Em - E) / s.nat <= 1e-5
+    assert dsig <= stress_tol(s)
 
 
 @pytest.mark.slow

@@ -128,3 +128,50 @@ def test_one_workgroup_sort_equals_radix_sort(pkg, precision):
         assert np.array_equal(Fb[k], single[k][1])
         if s.periodic:
             assert np.array_equal(Lb[k], single[k][2])
+
+
+def _dense_cell_cluster(wl, nblob=300, nspread=1500, seed=31):
+    """A nonperiodic structure whose cell grid holds ~2.5 atoms per cell except
+    one cell with `nblob` atoms: a 1.5-bohr blob placed in the middle of one
+    grid cell (cells are at least 0.6 rc = 7.2 bohr wide) inside a spread
+    cluster of `nspread` atoms."""
+    s = wl.cluster_scaled(nspread, seed=seed)
+    lo, hi = s.pos.min(axis=0), s.pos.max(axis=0)
+    length = (hi + 1e-4) - (lo - 1e-4)
+    ng = np.floor(length / (0.6 * 12.0)) + 1
+    centre = (lo - 1e-4) + (np.floor(ng / 2) + 0.5) * length / ng
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((nblob, 3))
+    d *= (1.5 * rng.random(nblob) ** (1 / 3) / np.linalg.norm(d, axis=1))[:, None]
+    far = np.linalg.norm(s.pos - centre, axis=1) > 4.0       # room for the blob
+    pos = np.vstack([s.pos[far], centre + d])
+    Z = np.concatenate([s.Z[far], rng.choice(np.array([1, 6, 7], np.int32), nblob)]).astype(np.int32)
+    return wl.Structure(pos, Z, np.zeros((3, 3)), False)
+
+
+def test_one_workgroup_sort_with_a_crowded_cell(pkg):
+    """A cell holding more atoms than the one-workgroup sort ranks by scanning
+    the cell (256) switches it to ranking in rounds of the workgroup's size.
+    The orders must still be the radix sort's: the structure alone (one
+    workgroup) and in a batch above 8,192 atoms (radix sort) give bit-identical
+    results and the same neighbour entries."""
+    wl = pkg.workloads
+    s = _dense_cell_cluster(wl)
+    big = [wl.perovskite(8, 8, 8, seed=41), wl.perovskite(4, 4, 4, seed=42)]
+    assert s.nat <= 8192 < s.nat + sum(b.nat for b in big)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    e, f, _ = h.energy_and_forces(s)
+    ent = h.neighbor_entries()
+    per_atom = np.bincount(ent[:, 0], minlength=s.nat)
+    print(f"crowded cell: nat={s.nat}, most neighbours {per_atom.max()}", flush=True)
+    assert per_atom.max() >= 299
+    hb = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
+    Eb, Fb, _ = hb.energy_and_forces_batch([big[0], s, big[1]])
+    assert Eb[1] == e
+    assert np.array_equal(Fb[1], f)
+    eb = hb.neighbor_entries()
+    off = big[0].nat
+    mine = eb[(eb[:, 0] >= off) & (eb[:, 0] < off + s.nat)].copy()
+    mine[:, :2] -= off
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    np.testing.assert_array_equal(key(mine), key(ent))
