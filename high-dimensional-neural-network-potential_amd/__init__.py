@@ -31,6 +31,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device",
+           "hdnnp_evaluate_batch_device_noalloc",
            "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee",
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
@@ -95,6 +96,8 @@ def load_library():
         lib.hdnnp_committee_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         lib.hdnnp_last_committee.argtypes = [vp, ctypes.c_int32, _dp, _dp]
         lib.hdnnp_evaluate_batch_device.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp, vp]
+        lib.hdnnp_evaluate_batch_device_noalloc.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp,
+                                                            vp]
         lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
         lib.hdnnp_get_symfunctions.argtypes = [vp, ctypes.c_int32, _dp, _dp, _dp]
@@ -295,12 +298,14 @@ class Hdnnp:
         off = np.concatenate([[0], np.cumsum(nat)])
         return E, [F[off[k]:off[k + 1]] for k in range(ns)], L
 
-    def evaluate_batch_torch(self, nat, Z, pos, periodic, lat, force=None, stream=None):
+    def evaluate_batch_torch(self, nat, Z, pos, periodic, lat, force=None, stream=None, noalloc=False):
         """hdnnp_evaluate_batch_device: nat/periodic (nstruct,), lat (nstruct, 3, 3)
         host arrays; Z (int32 [sum nat]), pos and force (float64 [sum nat, 3])
         contiguous torch tensors on one GPU. Returns (E[nstruct], force tensor,
         dEdLat[nstruct, 3, 3]). Raises ValueError for any other input before
-        anything is launched."""
+        anything is launched. noalloc: through
+        hdnnp_evaluate_batch_device_noalloc (no device allocation; pass
+        `force` to allocate nothing here either)."""
         import torch
         nat = np.ascontiguousarray(nat, dtype=np.int32).reshape(-1)
         per = np.ascontiguousarray(periodic, dtype=np.int32).reshape(-1)
@@ -318,7 +323,8 @@ class Hdnnp:
         _check_device_inputs(torch, total, Z, pos, force)
         E = np.zeros(ns)
         L = np.zeros((ns, 3, 3))
-        self._check(self.lib.hdnnp_evaluate_batch_device(
+        entry = self.lib.hdnnp_evaluate_batch_device_noalloc if noalloc else self.lib.hdnnp_evaluate_batch_device
+        self._check(entry(
             self.ctx, ns, _p(nat, _ip), ctypes.c_void_p(Z.data_ptr()), ctypes.c_void_p(pos.data_ptr()),
             _p(per, _ip), _p(lat, _dp), _p(E, _dp), ctypes.c_void_p(force.data_ptr()), _p(L, _dp),
             ctypes.c_void_p(stream or 0)))

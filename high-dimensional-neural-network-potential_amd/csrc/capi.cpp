@@ -68,6 +68,16 @@ static int guarded(hdnnp_ctx *ctx, F &&f)
     }
 }
 
+// A context's device copy of the potential is set up when it is created if a
+// GPU is visible (else on its first evaluation), so that an evaluation that
+// may not allocate (hdnnp_evaluate_batch_device_noalloc) finds it there.
+static void set_up_device(hdnnp::Engine &eng)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) eng.ensure_device();
+    (void)hipGetLastError();
+}
+
 extern "C" {
 
 int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out)
@@ -83,7 +93,9 @@ int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out)
         auto *c = new hdnnp_ctx;
         try {
             c->eng = new hdnnp::Engine(std::move(ms), precision);
+            set_up_device(*c->eng);
         } catch (...) {
+            delete c->eng;
             delete c;
             throw;
         }
@@ -107,7 +119,9 @@ int hdnnp_create_committee(int32_t n, const char *const *param_dirs, int32_t pre
         auto *c = new hdnnp_ctx;
         try {
             c->eng = new hdnnp::Engine(std::move(ms), precision);
+            set_up_device(*c->eng);
         } catch (...) {
+            delete c->eng;
             delete c;
             throw;
         }
@@ -217,6 +231,17 @@ int hdnnp_evaluate_batch_device(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *
     return guarded(ctx, [&] {
         ctx->eng->evaluate_batch_device(nstruct, nat, Z_dev, pos_dev, periodic, lat, energy, force_dev, dEdLat,
                                         (hipStream_t)stream);
+    });
+}
+
+int hdnnp_evaluate_batch_device_noalloc(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z_dev,
+                                        const double *pos_dev, const int32_t *periodic, const double *lat,
+                                        double *energy, double *force_dev, double *dEdLat, void *stream)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng->evaluate_batch_device_noalloc(nstruct, nat, Z_dev, pos_dev, periodic, lat, energy, force_dev,
+                                                dEdLat, (hipStream_t)stream);
     });
 }
 

@@ -55,9 +55,52 @@ HipUnsafe::~HipUnsafe()
 HipUnsafe::Capture::Capture() : lk(capture_mutex()) { t_capturing = true; }
 HipUnsafe::Capture::~Capture() { t_capturing = false; }
 
+// Work buffers of Engine::evaluate_batch_device_noalloc: carved from this
+// array instead of allocated. Every device the library runs on holds its own
+// copy until the process ends. In host code the name stands for no device
+// address: host code takes the address per device from hipGetSymbolAddress
+// (scratch_address) and uses only that.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// the scratch address on device dev (resolved once per device, with it current)
+static void *scratch_address(int dev)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    if (dev < 0 || dev >= kMaxDevices) throw Error(5, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        HipUnsafe guard;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(5, std::string("hipGetSymbolAddress of the device scratch failed: ") + hipGetErrorString(e));
+        addr[dev] = p;
+    }
+    return addr[dev];
+}
+
+// carving state of a no-allocation call on this thread (null otherwise)
+struct Arena {
+    char *base;
+    size_t size, used;
+};
+static thread_local Arena *t_arena = nullptr;
+
 void DevMem::ensure(size_t b)
 {
     if (b <= bytes && p) return;
+    if (t_arena) {
+        const size_t need = (std::max<size_t>(b, 1) + 255) / 256 * 256;
+        if (need > t_arena->size - t_arena->used)
+            throw Error(6, "the evaluation needs more than the " + std::to_string(t_arena->size >> 20) +
+                               " MiB of device scratch of the entry point that allocates nothing");
+        p = t_arena->base + t_arena->used;
+        t_arena->used += need;
+        bytes = need;
+        return;
+    }
     HipUnsafe guard;
     release();
     const size_t want = std::max<size_t>(256, b + b / 4);
@@ -67,6 +110,11 @@ void DevMem::ensure(size_t b)
 
 void DevMem::release()
 {
+    if (t_arena) {   // carved from the scratch: nothing to free
+        p = nullptr;
+        bytes = 0;
+        return;
+    }
     if (p) {
         HipUnsafe guard;
         (void)hipFree(p);
@@ -226,6 +274,8 @@ void Engine::init_device()
         HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         for (auto &e : ev_) HIPCHK(hipEventCreate(&e));
     }
+    HIPCHK(hipGetDevice(&dev_));
+    scratch_ = scratch_address(dev_);
 
     const HostModel &m = model_;
     DevModel d{};
@@ -500,7 +550,16 @@ static void *grow_pinned(void *&p, size_t &cap, size_t bytes)
     }
     return p;
 }
-void *Engine::staging(size_t bytes) { return grow_pinned(pinned_, pinned_bytes_, bytes); }
+void *Engine::staging(size_t bytes)
+{
+    if (t_arena) {   // a no-allocation call stages through pageable memory
+        if (host_stage_.size() < bytes) host_stage_.resize(bytes);
+        stage_ptr_ = host_stage_.data();
+    } else {
+        stage_ptr_ = grow_pinned(pinned_, pinned_bytes_, bytes);
+    }
+    return stage_ptr_;
+}
 void *Engine::io_staging(size_t bytes) { return grow_pinned(io_pinned_, io_pinned_bytes_, bytes); }
 
 // Cell grid of one structure (neighborLists.f90:131-169): lattice (periodic)
@@ -605,7 +664,7 @@ void Engine::footprint_add(Footprint &f, const StructIn &in, double rc_list) con
     f.est = std::max(f.est, neighbour_estimate(G, rc_list));
 }
 
-size_t Engine::footprint_bytes(const Footprint &f) const
+size_t Engine::footprint_bytes(const Footprint &f, bool slack) const
 {
     const size_t R = precision_ == 0 ? 8 : 4, nm = (size_t)committee_size();
     size_t nsfmax = 0;
@@ -619,6 +678,7 @@ size_t Engine::footprint_bytes(const Footprint &f) const
     if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
     b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure
     b += (nat / kReduceChunk + (size_t)f.nstruct + 1) * 88;                                 // reduction chunks
+    if (!slack) return b;
     return b + b / 4 + (size_t(8) << 20);   // growth slack of each buffer, sort temporaries
 }
 
@@ -749,7 +809,9 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.force = nullptr;              // set by finish
     w.result = res_.as<double>();
 
-    HIPCHK(hipMemsetAsync(misc_.p, 0, kDblOffset, s));
+    // flags and element counts start at zero (a fill kernel: no memset node
+    // when a caller captures the launches)
+    launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
 
     Geo g{};
     g.sg = geo_.as<StructGeo>();
@@ -884,7 +946,7 @@ bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)
     const Work &w = prep_.w;
     const int nat = prep_.nat, nm = (int)dmm_.size();
     const std::vector<StructIn> &st = prep_.st;
-    char *hs = (char *)pinned_;
+    char *hs = (char *)stage_ptr_;   // where build staged its tables
     double *hres = (double *)(hs + prep_.o_res);
     int *hflags = (int *)(hs + prep_.o_res + prep_.res_b);   // the reduction's copy of w.flags
     HIPCHK(hipMemcpyAsync(hres, w.result, prep_.res_b + kFlagBytes, hipMemcpyDeviceToHost, s));
@@ -1102,7 +1164,60 @@ void Engine::evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev
     }
     if (total > 0 && (!Z_dev || !pos_dev || !force_dev)) throw Error(1, "null array");
     device_bboxes(st, pos_dev, s);
+    if (t_arena) {   // refuse what cannot fit before anything is launched
+        Footprint f;
+        for (const StructIn &in : st) footprint_add(f, in, model_.maxrc);
+        const size_t need = footprint_bytes(f, false);
+        if (need > t_arena->size - t_arena->used)
+            throw Error(6, "the evaluation needs about " + std::to_string((need + (1 << 20) - 1) >> 20) +
+                               " MiB of device scratch; the entry point that allocates nothing has " +
+                               std::to_string(t_arena->size >> 20) + " MiB");
+    }
     run(st, Z_dev, pos_dev, force_dev, s, energy, dEdLat, 1);
+}
+
+void Engine::evaluate_batch_device_noalloc(int nstruct, const int *nat, const int *Z_dev, const double *pos_dev,
+                                           const int *periodic, const double *lat, double *energy, double *force_dev,
+                                           double *dEdLat, hipStream_t stream)
+{
+    if (!dev_ready_)
+        throw Error(5, "the context has no device state (no GPU was visible when it was created), and the "
+                       "entry point that allocates nothing cannot set it up");
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev_) throw Error(1, "the current device is not the one the context was set up on");
+    // one call at a time per device: they share its scratch
+    static std::mutex dev_mu[64];
+    std::lock_guard<std::mutex> one(dev_mu[dev_ % 64]);
+    // the context's own buffers are set aside and put back afterwards
+    std::vector<DevMem *> bufs = work_buffers();
+    bufs.push_back(&misc_);
+    std::vector<DevMem> saved(bufs.size());
+    for (size_t k = 0; k < bufs.size(); ++k) {
+        saved[k] = *bufs[k];
+        *bufs[k] = DevMem{};
+    }
+    Arena arena{(char *)scratch_, kScratchBytes, 0};
+    auto restore = [&] {
+        t_arena = nullptr;
+        for (size_t k = 0; k < bufs.size(); ++k) *bufs[k] = saved[k];
+        uploaded_.clear();   // the tables were staged into the scratch
+        prep_ = Prepared{};
+        ++build_gen_;        // lists built from the scratch are gone
+        last_nat_ = 0;
+        last_total_ = 0;
+    };
+    t_arena = &arena;
+    try {
+        uploaded_.clear();
+        misc_.ensure(1024);
+        evaluate_batch_device(nstruct, nat, Z_dev, pos_dev, periodic, lat, energy, force_dev, dEdLat, stream);
+    } catch (...) {
+        (void)hipStreamSynchronize(stream ? stream : stream_);   // nothing left running on the scratch
+        restore();
+        throw;
+    }
+    restore();
 }
 
 // bounding boxes of the nonperiodic structures of st (atoms concatenated in

@@ -39,7 +39,12 @@ class HipUnsafe {
     bool locked_ = false, counted_ = false;
 };
 
-// grow-only device allocation
+// Device scratch of the entry point that allocates nothing
+// (Engine::evaluate_batch_device_noalloc): bytes per device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+
+// grow-only device allocation; inside evaluate_batch_device_noalloc it carves
+// from the device scratch instead (no allocation, nothing freed)
 struct DevMem {
     void *p = nullptr;
     size_t bytes = 0;
@@ -85,6 +90,15 @@ class Engine {
     void evaluate_batch_device(int nstruct, const int *nat, const int *Z_dev, const double *pos_dev,
                                const int *periodic, const double *lat, double *energy, double *force_dev,
                                double *dEdLat, hipStream_t stream);
+    // evaluate_batch_device with every work buffer carved from the device
+    // scratch (kScratchBytes per device): no device or pinned allocation, no
+    // memset. Needs the context's device state (set up by create when a GPU is
+    // visible); inputs that need more scratch are refused (capacity error).
+    // Calls on one device are serialised. The context's own work buffers are
+    // left as they were, and stage outputs of the call are not kept.
+    void evaluate_batch_device_noalloc(int nstruct, const int *nat, const int *Z_dev, const double *pos_dev,
+                                       const int *periodic, const double *lat, double *energy, double *force_dev,
+                                       double *dEdLat, hipStream_t stream);
     // device arrays Z, pos, force; lat/energy/dEdLat on the host
     void evaluate_device(int nat, const int *Z_dev, const double *pos_dev, int periodic,
                          const double *lat, double *energy, double *force_dev, double *dEdLat,
@@ -157,7 +171,8 @@ class Engine {
         double est = 0;   // largest neighbour-count estimate
     };
     void footprint_add(Footprint &f, const StructIn &in, double rc_list) const;
-    size_t footprint_bytes(const Footprint &f) const;
+    // slack: the buffers' growth room and the sort temporaries on top
+    size_t footprint_bytes(const Footprint &f, bool slack = true) const;
 
   private:
     void *staging(size_t bytes);
@@ -204,6 +219,10 @@ class Engine {
     std::vector<char> uploaded_;   // the staged tables of the last upload
     void *pinned_ = nullptr;
     size_t pinned_bytes_ = 0;
+    void *stage_ptr_ = nullptr;          // what staging() returned last
+    std::vector<char> host_stage_;       // staging without pinned memory (no-allocation calls)
+    int dev_ = -1;                       // the device of init_device
+    void *scratch_ = nullptr;            // that device's scratch (hipGetSymbolAddress)
     // pinned staging of a small host-array evaluation's inputs and forces
     // (apart from pinned_, which build() rewrites while those copies run)
     void *io_pinned_ = nullptr;

@@ -47,8 +47,9 @@ enum {
 
 /* Replaces parseRuNNerInput(handle, path) (src/potentials/hdnnps.f90:34-233):
  * reads path/input.nn, path/scaling.data, path/weights.ZZZ.data (param_dir
- * is the directory; a trailing '/' is optional). Parsing needs no GPU; device
- * memory is set up on the first evaluation. */
+ * is the directory; a trailing '/' is optional). Parsing needs no GPU. When
+ * a GPU is visible, the device copy of the potential is set up here; else on
+ * the first evaluation. */
 int hdnnp_create(const char *param_dir, int32_t precision, hdnnp_ctx **out);
 
 /* Replaces initGlobalHdnnpsFromInputFile (src/potentials/globalHdnnps.f90:13-29,
@@ -94,6 +95,29 @@ int hdnnp_evaluate_batch(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, co
 int hdnnp_evaluate_batch_device(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z_dev,
                                 const double *pos_dev, const int32_t *periodic, const double *lat,
                                 double *energy, double *force_dev, double *dEdLat, void *stream);
+
+/* hdnnp_evaluate_batch_device for callers that cannot allocate: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference. Same arguments and results (bit for bit)
+ * as hdnnp_evaluate_batch_device, but the call makes no device or pinned
+ * allocation and frees nothing: its work buffers are carved from a
+ * file-scope device array of HDNNP_SCRATCH_BYTES (128 MiB) that every GPU the
+ * library runs on holds until the process ends, host staging uses pageable
+ * memory, and the flags are cleared by a fill kernel. The context's device
+ * copy of the potential must exist already: hdnnp_create sets it up when a
+ * GPU is visible (otherwise the call fails with HDNNP_ERR_DEVICE). Inputs
+ * needing more scratch than the array holds are refused with
+ * HDNNP_ERR_CAPACITY before anything is launched (about 25,000 atoms of the
+ * 80-SF perovskite potential in mixed precision, 8,000 in FP64). All calls on
+ * one GPU share the array, which keeps the last call's contents; the library
+ * serialises such calls (any contexts, any threads), so overlapping calls
+ * wait for each other instead of racing. The context's own work buffers are
+ * left as they were; stage outputs (hdnnp_get_neighbors, ...) of the call
+ * are not kept. */
+#define HDNNP_SCRATCH_BYTES (128u << 20)
+int hdnnp_evaluate_batch_device_noalloc(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *nat, const int32_t *Z_dev,
+                                        const double *pos_dev, const int32_t *periodic, const double *lat,
+                                        double *energy, double *force_dev, double *dEdLat, void *stream);
 
 /* Same evaluation with Z, pos and force resident in device memory (HBM);
  * lat, energy and dEdLat are host pointers. `stream` is a hipStream_t or
