@@ -32,6 +32,8 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_evaluate_batch_device", "hdnnp_evaluate_device",
            "hdnnp_evaluate_batch_device_noalloc",
+           "hdnnp_stage_neighbor_lists", "hdnnp_stage_get_neighbor_lists", "hdnnp_stage_symfunctions",
+           "hdnnp_stage_nns",
            "hdnnp_create_committee", "hdnnp_committee_size", "hdnnp_last_committee",
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
@@ -98,6 +100,13 @@ def load_library():
         lib.hdnnp_evaluate_batch_device.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp, vp]
         lib.hdnnp_evaluate_batch_device_noalloc.argtypes = [vp, ctypes.c_int32, _ip, vp, vp, _ip, _dp, _dp, vp, _dp,
                                                             vp]
+        i64p_ = ctypes.POINTER(ctypes.c_int64)
+        lib.hdnnp_stage_neighbor_lists.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, i64p_, _ip]
+        lib.hdnnp_stage_get_neighbor_lists.argtypes = [vp, ctypes.c_int64, _ip, _dp]
+        lib.hdnnp_stage_symfunctions.argtypes = [vp, i64p_, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp,
+                                                 ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+        lib.hdnnp_stage_nns.argtypes = [vp, vp, i64p_, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp,
+                                        ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.hdnnp_neighbor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_get_neighbors.argtypes = [vp, ctypes.c_int64, _ip]
         lib.hdnnp_get_symfunctions.argtypes = [vp, ctypes.c_int32, _dp, _dp, _dp]
@@ -386,6 +395,99 @@ class Hdnnp:
 
 def parseRuNNerInput(path: str, precision="fp64") -> Hdnnp:
     return Hdnnp(path, precision)
+
+
+class NeighborList:
+    """type neighborList (neighborLists.f90:10-16): nNeis entries, slot 0 the
+    atom itself; neis[n] the position of the image, ineis[n] its atom
+    (0-based here, 1-based in the Fortran type)."""
+
+    def __init__(self, neis, ineis, stage):
+        self.nNeis = int(ineis.shape[0])
+        self.neis = neis
+        self.ineis = ineis
+        self.stage = stage
+
+
+class SymFunctionContainer:
+    """type symFunctionContainer (symmetryFunctions.f90:24-33) of one atom:
+    sf[s] scaled values, dsf[s, n] = dG_s/dR of list slot n (3 components)."""
+
+    def __init__(self, sf, dsf, ctx, stage):
+        self.nsf = int(sf.shape[0])
+        self.sf = sf
+        self.dsf = dsf
+        self.ctx = ctx
+        self.stage = stage
+
+
+def _c_structure(ats):
+    Z = np.ascontiguousarray(ats.Z, dtype=np.int32)
+    pos = np.ascontiguousarray(ats.pos, dtype=np.float64).reshape(-1, 3)
+    lat = np.ascontiguousarray(ats.lat, dtype=np.float64).reshape(3, 3)
+    return Z, pos, int(bool(ats.periodic)), lat
+
+
+def hdnnpCalcNeighborLists(handle: Hdnnp, ats):
+    """hdnnpCalcNeighborLists (hdnnps.f90:373-388): the lists of every atom, in
+    the reference's order and layout, built on the device."""
+    Z, pos, per, lat = _c_structure(ats)
+    nat = Z.shape[0]
+    stage = ctypes.c_int64()
+    nn = np.zeros(max(nat, 1), np.int32)
+    handle._check(handle.lib.hdnnp_stage_neighbor_lists(handle.ctx, nat, _p(Z, _ip), _p(pos, _dp), per,
+                                                        _p(lat, _dp), ctypes.byref(stage), _p(nn, _ip)))
+    tot = int(nn[:nat].sum())
+    ineis = np.zeros(max(tot, 1), np.int32)
+    neis = np.zeros((max(tot, 1), 3))
+    handle._check(handle.lib.hdnnp_stage_get_neighbor_lists(handle.ctx, stage.value, _p(ineis, _ip), _p(neis, _dp)))
+    off = np.concatenate([[0], np.cumsum(nn[:nat])])
+    return [NeighborList(neis[off[i]:off[i + 1]], ineis[off[i]:off[i + 1]] - 1, stage.value) for i in range(nat)]
+
+
+def hdnnpCalcSymfunctions(handle: Hdnnp, ats, neiLists, calcSfsTogether: bool = True):
+    """hdnnpCalcSymfunctions (hdnnps.f90:390-485): scaled values and dG/dR of
+    every atom's symmetry functions. calcSfsTogether selects the fused pass
+    (calcAllSymmetryFunctions) or the per-function kernel (symmetryFunction2
+    / 3) for the values; the gradients come from the per-function kernel."""
+    Z, pos, per, lat = _c_structure(ats)
+    nat = Z.shape[0]
+    rank = {z: e for e, z in enumerate(handle.elements)}
+    nsf = np.array([handle.info.nsf[rank[int(z)]] for z in Z], dtype=np.int64)
+    nn = np.array([nl.nNeis for nl in neiLists], dtype=np.int64)
+    ld = max(handle.maxsf, 1)
+    G = np.zeros((max(nat, 1), ld))
+    off = np.concatenate([[0], np.cumsum(3 * nsf * nn)]).astype(np.int64)
+    dsf = np.zeros(max(int(off[-1]), 1))
+    stage = ctypes.c_int64(neiLists[0].stage if nat else 0)
+    handle._check(handle.lib.hdnnp_stage_symfunctions(handle.ctx, ctypes.byref(stage), nat, _p(Z, _ip), _p(pos, _dp),
+                                                      per, _p(lat, _dp), int(bool(calcSfsTogether)), ld,
+                                                      _p(G, _dp), _p(dsf, _dp)))
+    return [SymFunctionContainer(G[i, :nsf[i]].copy(), dsf[off[i]:off[i + 1]].reshape(nsf[i], nn[i], 3),
+                                 handle.ctx, stage.value) for i in range(nat)]
+
+
+def hdnnpCalcNNs(handle: Hdnnp, ats, neiLists, sfs, want_dEdLat: bool = True):
+    """hdnnpCalcNNs (hdnnps.f90:487-562): this handle's networks on the
+    symmetry functions sfs (computed by any handle of the same precision,
+    elements and SF counts, as globalHdnnps.f90:53-55 does); forces and
+    dEdLat from the gradients of the symmetry functions of the handle that
+    computed sfs. Returns (E, F[nat,3], dEdLat|None)."""
+    Z, pos, per, lat = _c_structure(ats)
+    nat = Z.shape[0]
+    ld = max(max((c.nsf for c in sfs), default=1), 1)
+    G = np.zeros((max(nat, 1), ld))
+    for i, c in enumerate(sfs):
+        G[i, :c.nsf] = c.sf
+    stage = ctypes.c_int64(sfs[0].stage if nat else 0)
+    sfctx = sfs[0].ctx if nat else None
+    E = ctypes.c_double()
+    F = np.zeros((max(nat, 1), 3))
+    L = np.zeros((3, 3)) if want_dEdLat else None
+    handle._check(handle.lib.hdnnp_stage_nns(handle.ctx, sfctx, ctypes.byref(stage), nat, _p(Z, _ip), _p(pos, _dp),
+                                             per, _p(lat, _dp), ld, _p(G, _dp), ctypes.byref(E), _p(F, _dp),
+                                             _p(L, _dp)))
+    return E.value, F[:nat], L
 
 
 def hdnnpEnergyAndForces(handle: Hdnnp, ats, want_dEdLat: bool = True):

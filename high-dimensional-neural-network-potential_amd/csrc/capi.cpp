@@ -256,6 +256,59 @@ int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev, con
     });
 }
 
+// the stage's lists are the engine's current ones, else they are built again
+// from the structure given
+static void stage_ensure(hdnnp_ctx *ctx, int64_t *stage, int32_t nat, const int32_t *Z, const double *pos,
+                         int32_t periodic, const double *lat)
+{
+    if (!ctx->eng->stage_current(*stage)) *stage = ctx->eng->stage_build(nat, Z, pos, periodic, lat);
+}
+
+int hdnnp_stage_neighbor_lists(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *pos, int32_t periodic,
+                               const double *lat, int64_t *stage, int32_t *nneis)
+{
+    if (!ctx || !stage || (nat > 0 && !nneis)) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        *stage = ctx->eng->stage_build(nat, Z, pos, periodic, lat);
+        ctx->eng->stage_neighbors(nneis, nullptr, nullptr);
+    });
+}
+
+int hdnnp_stage_get_neighbor_lists(hdnnp_ctx *ctx, int64_t stage, int32_t *ineis, double *neis)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->eng->stage_current(stage))
+            throw hdnnp::Error(HDNNP_ERR_ARG, "the stage is not the context's current one (it built lists since)");
+        std::vector<int32_t> nn(std::max(ctx->eng->prepared_nat(), 1));
+        ctx->eng->stage_neighbors(nn.data(), ineis, neis);
+    });
+}
+
+int hdnnp_stage_symfunctions(hdnnp_ctx *ctx, int64_t *stage, int32_t nat, const int32_t *Z, const double *pos,
+                             int32_t periodic, const double *lat, int32_t together, int32_t ld, double *G, double *dsf)
+{
+    if (!ctx || !stage || (nat > 0 && !G)) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] {
+        stage_ensure(ctx, stage, nat, Z, pos, periodic, lat);
+        ctx->eng->stage_symfunctions(together, ld, G, dsf);
+    });
+}
+
+int hdnnp_stage_nns(hdnnp_ctx *net, hdnnp_ctx *sfctx, int64_t *stage, int32_t nat, const int32_t *Z, const double *pos,
+                    int32_t periodic, const double *lat, int32_t ld, const double *G, double *energy, double *force,
+                    double *dEdLat)
+{
+    if (!net || !stage || !energy || (nat > 0 && (!G || !force))) return HDNNP_ERR_ARG;
+    hdnnp_ctx *sc = sfctx ? sfctx : net;
+    const int rc = guarded(sc, [&] {
+        stage_ensure(sc, stage, nat, Z, pos, periodic, lat);
+        sc->eng->stage_nns(*net->eng, ld, G, energy, force, dEdLat);
+    });
+    if (rc && sc != net) net->err = sc->err;
+    return rc;
+}
+
 int hdnnp_neighbor_count(hdnnp_ctx *ctx, int64_t *count)
 {
     if (!ctx || !count) return HDNNP_ERR_ARG;

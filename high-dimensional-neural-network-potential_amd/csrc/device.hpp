@@ -184,6 +184,22 @@ template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, const R *dEdG, R *gent,
                     hipStream_t s);
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad);
+// One symmetry function of the per-function path, in the sorted order of its
+// element's list: type 2 (G2, neighbour element rank f) or 3 (G4, element
+// pair {f, g}); pre = 2^(1 - zeta).
+struct SfParam {
+    int type, f, g, pad;
+    double eta, rs, lambda, zeta, pre;
+};
+// The per-function path (symmetryFunction2 / symmetryFunction3 of the
+// reference): for every atom and symmetry function, the centred / scaled
+// value G[i * ldG + s] and, when dsf is not null, its gradient rows
+// dsf[dsf_off[i] + 3 (s (count_i + 1) + slot)], slot 0 the atom, slot 1 + n
+// its list entry n (derivatives with respect to those positions, scaled).
+// par[par_off[e] ..] lists element e's functions.
+template <typename R>
+void launch_sf_single(const DevModel &m, const Geo &g, const Work &w, const SfParam *par, const int *par_off,
+                      const long long *dsf_off, int ldG, double *G, double *dsf, hipStream_t s);
 // atoms per wavefront the SF kernels use for a requested tile (1, 2 or 4)
 int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max, int nel);
 // ---- kernels_mlp.hip: 16 atoms per wavefront, maxw = widest layer/input ----

@@ -221,7 +221,7 @@ std::vector<DevMem *> Engine::work_buffers()
             &posin_, &fout_, &eidx_, &wpos_, &cellkey_, &cellkey_sorted_, &cell_atoms_,
             &iota_, &cell_start_, &cell_end_, &count_, &perm_, &Ei_,
             &vir_, &ent_j_, &ent_shift_, &ent_s_, &rev_, &pos_start_, &gent_, &G_, &dEdG_, &cub_tmp_, &elem_chunks_,
-            &trip_, &tcount_};
+            &trip_, &tcount_, &sfpar_, &sfpar_off_, &stage_G_, &stage_dsf_, &stage_off_};
 }
 
 size_t Engine::device_bytes() const
@@ -510,8 +510,14 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
         w.trip = nullptr;
         w.tcount = nullptr;
     }
-    launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
-    HIPCHK(hipGetLastError());
+    if (!skip_values_) {
+        launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
+        HIPCHK(hipGetLastError());
+        trip_valid_ = w.trip != nullptr;
+    } else if (!trip_valid_) {   // G came from elsewhere: pass B compacts its own triplets
+        w.trip = nullptr;
+        w.tcount = nullptr;
+    }
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
 
     // one block per 16-atom tile of one element; the element counts stay on
@@ -690,6 +696,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     prep_ = Prepared{};
     prep_.st = st;
     ++build_gen_;
+    trip_valid_ = false;
     if (!(model_.maxrc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
     if (dm_.fallback_el < 0) throw Error(3, elem_why_[0].empty() ? "no element of the potential can be evaluated"
                                                                  : elem_why_[0]);
@@ -1255,6 +1262,270 @@ void Engine::evaluate_device(int nat, const int *Z_dev, const double *pos_dev, i
 {
     if (periodic && !lat) throw Error(1, "periodic structure without lattice");
     evaluate_batch_device(1, &nat, Z_dev, pos_dev, &periodic, lat, energy, force_dev, dEdLat, stream);
+}
+
+// ---------------------------------------------------------------- stage API
+// hdnnpCalcNeighborLists / hdnnpCalcSymfunctions / hdnnpCalcNNs
+// (hdnnps.f90:373-562) on the engine's one stage.
+
+int64_t Engine::stage_build(int nat, const int *Z, const double *pos, int periodic, const double *lat)
+{
+    if (!dev_ready_) init_device();
+    if (nat < 0) throw Error(1, "nat < 0");
+    if (nat > 0 && (!Z || !pos)) throw Error(1, "null array");
+    StructIn in;
+    in.nat = nat;
+    in.periodic = periodic ? 1 : 0;
+    if (in.periodic) {
+        if (!lat) throw Error(1, "periodic structure without lattice");
+        std::memcpy(in.lat, lat, sizeof in.lat);
+    } else if (nat > 0) {
+        for (int a = 0; a < 3; ++a) in.bbox[a] = in.bbox[3 + a] = pos[a];
+        for (int i = 1; i < nat; ++i)
+            for (int a = 0; a < 3; ++a) {
+                in.bbox[a] = std::min(in.bbox[a], pos[3 * i + a]);
+                in.bbox[3 + a] = std::max(in.bbox[3 + a], pos[3 * i + a]);
+            }
+    }
+    const size_t n = (size_t)std::max(nat, 1), z_off = (24 * n + 255) / 256 * 256;
+    posin_.ensure(z_off + 4 * n);
+    fout_.ensure(24 * n);
+    const int *Z_dev = (const int *)(posin_.as<char>() + z_off);
+    if (nat > 0) {
+        HIPCHK(hipMemcpyAsync(posin_.p, pos, 24 * (size_t)nat, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync((void *)Z_dev, Z, 4 * (size_t)nat, hipMemcpyHostToDevice, stream_));
+    }
+    std::vector<StructIn> st(1, in);
+    for (int attempt = 0;; ++attempt) {
+        build(st, Z_dev, posin_.as<double>(), stream_, model_.maxrc);
+        if (nat == 0) break;
+        int fl[8];
+        d2h(fl, misc_.as<int>() + kIntFlags, sizeof fl);
+        if (fl[0]) throw Error(4, "structure contains element Z=" + std::to_string(fl[0]) +
+                                      " that the potential does not describe");
+        if (fl[3]) {
+            for (int e = 0; e < dm_.nel; ++e)
+                if (model_.el[e].Z == fl[3]) throw Error(3, elem_why_[e]);
+            throw Error(3, "element Z=" + std::to_string(fl[3]) + " cannot be evaluated");
+        }
+        if (fl[2]) throw Error(6, "periodic image shift beyond +-127 cells");
+        if (fl[1] <= 0) break;
+        if (fl[1] > prep_.lds_lim)
+            throw Error(6, "an atom has " + std::to_string(fl[1]) + " neighbours; this build stages at most " +
+                               std::to_string(prep_.lds_lim) + " per atom in LDS");
+        if (attempt >= 1) throw Error(6, "neighbour capacity did not converge");
+        cap_hint_ = std::max(cap_hint_, fl[1]);
+    }
+    stage_in_ = in;
+    stage_pos_.assign(pos, pos + 3 * (size_t)nat);
+    stage_gen_ = build_gen_;
+    stage_id_ = ++stage_next_;
+    last_nat_ = nat;
+    last_cap_ = prep_.w.cap;
+    last_lim_ = prep_.w.lim;
+    last_total_ = -1;
+    return stage_id_;
+}
+
+bool Engine::stage_current(int64_t id) const
+{
+    return id != 0 && id == stage_id_ && build_gen_ == stage_gen_ && (prep_.valid || prep_.nat == 0);
+}
+
+void Engine::stage_neighbors(int32_t *nneis, int32_t *ineis, double *neis)
+{
+    if (!stage_current(stage_id_)) throw Error(1, "no current neighbour-list stage");
+    const int nat = prep_.nat;
+    if (nat == 0) return;
+    const Work &w = prep_.w;
+    std::vector<int> cnt(nat);
+    d2h(cnt.data(), w.count, 4 * (size_t)nat);
+    for (int i = 0; i < nat; ++i) nneis[i] = 1 + cnt[i];
+    if (!ineis && !neis) return;
+    const size_t slots = (size_t)nat * w.cap;
+    std::vector<int> j(slots), sh(slots);
+    std::vector<double4> wp(nat);
+    d2h(j.data(), w.ent_j, 4 * slots);
+    d2h(sh.data(), w.ent_shift, 4 * slots);
+    d2h(wp.data(), w.wpos, sizeof(double4) * (size_t)nat);
+    const StructGeo &G = hgeo_[0];
+    // slot 1: the position the list was built from, once wrapped into the
+    // cell when periodic (hdnnps.f90:380-382), as given otherwise; the
+    // images: the list-frame position plus the lattice offset summed as
+    // neighborLists.f90:186-209 does (z, y, x, wrapped axes only), plus the
+    // artificial origin (:212)
+    size_t q = 0;
+    for (int i = 0; i < nat; ++i) {
+        double p[3] = {stage_pos_[3 * (size_t)i], stage_pos_[3 * (size_t)i + 1], stage_pos_[3 * (size_t)i + 2]};
+        if (G.periodic) {   // moveAtomsIntoCell (pointsets.f90:201-217) in the kernels' operation order
+            double s3[3];
+            for (int a = 0; a < 3; ++a) s3[a] = (G.inv[a] * p[0] + G.inv[3 + a] * p[1]) + G.inv[6 + a] * p[2];
+            for (int a = 0; a < 3; ++a) s3[a] = s3[a] - std::floor(s3[a]);
+            for (int a = 0; a < 3; ++a) p[a] = (G.lat[a] * s3[0] + G.lat[3 + a] * s3[1]) + G.lat[6 + a] * s3[2];
+        }
+        if (ineis) ineis[q] = i + 1;
+        if (neis) std::memcpy(neis + 3 * q, p, sizeof p);
+        ++q;
+        for (int n = 0; n < cnt[i]; ++n, ++q) {
+            const size_t t = (size_t)i * w.cap + n;
+            if (ineis) ineis[q] = j[t] + 1;
+            if (!neis) continue;
+            int sx, sy, sz;
+            unpack_shift(sh[t], sx, sy, sz);
+            double dl[3] = {0, 0, 0};
+            for (int a = 0; a < 3; ++a) {
+                if (sz) dl[a] = dl[a] + G.lat[6 + a] * (double)sz;
+                if (sy) dl[a] = dl[a] + G.lat[3 + a] * (double)sy;
+                if (sx) dl[a] = dl[a] + G.lat[a] * (double)sx;
+            }
+            const double *pj = &wp[j[t]].x;
+            for (int a = 0; a < 3; ++a) neis[3 * q + a] = (pj[a] + dl[a]) + G.origin[a];
+        }
+    }
+}
+
+void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
+{
+    if (!stage_current(stage_id_)) throw Error(1, "no current neighbour-list stage");
+    const int nat = prep_.nat;
+    if (nat == 0) return;
+    if (ld < dm_.nsfmax) throw Error(1, "ld smaller than the largest SF count");
+    const bool fp64 = precision_ == 0;
+    Work w = prep_.w;
+    const Geo g = prep_.g;
+    hipStream_t s = stream_;
+    if (together) {   // the fused values pass
+        const size_t rb = fp64 ? 8 : 4;
+        G_.ensure(rb * (size_t)nat * dm_.nsfmax);
+        const int ga = sf_tile(fp64, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
+        w.trip = nullptr;
+        w.tcount = nullptr;
+        if (fp64 && ga == 1) {
+            trip_.ensure(4 * (size_t)nat * w.tcap);
+            tcount_.ensure(4 * (size_t)nat);
+            w.trip = trip_.as<int>();
+            w.tcount = tcount_.as<int>();
+        }
+        if (fp64)
+            launch_sf_values<double>(dm_, g, w, w.lim, ga, G_.as<double>(), s);
+        else
+            launch_sf_values<float>(dm_, g, w, w.lim, ga, G_.as<float>(), s);
+        HIPCHK(hipGetLastError());
+        trip_valid_ = w.trip != nullptr;
+        std::vector<double> tmp((size_t)nat * dm_.nsfmax);
+        if (fp64) {
+            d2h(tmp.data(), G_.p, 8 * tmp.size());
+        } else {
+            std::vector<float> t(tmp.size());
+            d2h(t.data(), G_.p, 4 * t.size());
+            for (size_t k = 0; k < t.size(); ++k) tmp[k] = t[k];
+        }
+        for (int i = 0; i < nat; ++i)
+            for (int k = 0; k < ld; ++k) G[(size_t)i * ld + k] = k < dm_.nsfmax ? tmp[(size_t)i * dm_.nsfmax + k] : 0.0;
+    }
+    if (together && !dsf) return;
+    // the per-function path: its parameter table, then per-atom row offsets
+    if (!sfpar_.p) {
+        std::vector<SfParam> par;
+        std::vector<int> off(dm_.nel + 1, 0);
+        for (int e = 0; e < dm_.nel; ++e) {
+            off[e] = (int)par.size();
+            for (const SymFn &f : model_.el[e].sfs) {
+                SfParam p{};
+                p.type = f.type;
+                p.f = model_.rank_of_Z(f.e1);
+                p.g = f.type == 3 ? model_.rank_of_Z(f.e2) : -1;
+                p.eta = f.eta;
+                p.rs = f.type == 2 ? f.rs : 0.0;
+                p.lambda = f.type == 3 ? f.lambda : 0.0;
+                p.zeta = f.type == 3 ? f.zeta : 1.0;
+                p.pre = f.type == 3 ? std::pow(2.0, 1.0 - f.zeta) : 1.0;
+                par.push_back(p);
+            }
+        }
+        off[dm_.nel] = (int)par.size();
+        sfpar_.ensure(sizeof(SfParam) * std::max<size_t>(1, par.size()));
+        sfpar_off_.ensure(4 * off.size());
+        HIPCHK(hipMemcpyAsync(sfpar_.p, par.data(), sizeof(SfParam) * par.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(sfpar_off_.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    std::vector<int> cnt(nat), el(nat);
+    d2h(cnt.data(), w.count, 4 * (size_t)nat);
+    d2h(el.data(), w.eidx, 4 * (size_t)nat);
+    std::vector<long long> off(nat + 1, 0);
+    for (int i = 0; i < nat; ++i) off[i + 1] = off[i] + 3LL * dm_.nsf[el[i]] * (cnt[i] + 1);
+    stage_off_.ensure(8 * off.size());
+    stage_G_.ensure(8 * (size_t)nat * ld);
+    if (dsf) stage_dsf_.ensure(8 * (size_t)std::max<long long>(off[nat], 1));
+    HIPCHK(hipMemcpyAsync(stage_off_.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, s));
+    if (fp64)
+        launch_sf_single<double>(dm_, g, w, sfpar_.as<SfParam>(), sfpar_off_.as<int>(), stage_off_.as<long long>(), ld,
+                                 stage_G_.as<double>(), dsf ? stage_dsf_.as<double>() : nullptr, s);
+    else
+        launch_sf_single<float>(dm_, g, w, sfpar_.as<SfParam>(), sfpar_off_.as<int>(), stage_off_.as<long long>(), ld,
+                                stage_G_.as<double>(), dsf ? stage_dsf_.as<double>() : nullptr, s);
+    HIPCHK(hipGetLastError());
+    if (!together) {
+        std::vector<double> tmp((size_t)nat * ld);
+        d2h(tmp.data(), stage_G_.p, 8 * tmp.size());
+        for (int i = 0; i < nat; ++i)   // the columns an element does not have stay 0
+            for (int k = 0; k < ld; ++k) G[(size_t)i * ld + k] = k < dm_.nsf[el[i]] ? tmp[(size_t)i * ld + k] : 0.0;
+    }
+    if (dsf) d2h(dsf, stage_dsf_.p, 8 * (size_t)off[nat]);
+}
+
+void Engine::stage_nns(Engine &net, int ld, const double *G, double *energy, double *force, double *dEdLat)
+{
+    if (!stage_current(stage_id_)) throw Error(1, "no current neighbour-list stage");
+    if (net.precision_ != precision_) throw Error(1, "the networks' context has another precision");
+    if (net.committee_size() != 1) throw Error(1, "hdnnpCalcNNs takes the networks of one potential");
+    if (net.model_.el.size() != model_.el.size()) throw Error(3, "the potentials describe different elements");
+    for (size_t e = 0; e < model_.el.size(); ++e)
+        if (net.model_.el[e].Z != model_.el[e].Z || net.model_.el[e].sfs.size() != model_.el[e].sfs.size())
+            throw Error(3, "the networks' potential does not take this potential's symmetry functions");
+    net.ensure_device();
+    const int nat = prep_.nat;
+    *energy = 0;
+    if (dEdLat) std::memset(dEdLat, 0, 9 * sizeof(double));
+    if (nat == 0) return;
+    if (ld < dm_.nsfmax) throw Error(1, "ld smaller than the largest SF count");
+    const bool fp64 = precision_ == 0;
+    G_.ensure((fp64 ? 8 : 4) * (size_t)nat * dm_.nsfmax);
+    if (fp64) {
+        std::vector<double> t((size_t)nat * dm_.nsfmax);
+        for (int i = 0; i < nat; ++i)
+            for (int k = 0; k < dm_.nsfmax; ++k) t[(size_t)i * dm_.nsfmax + k] = G[(size_t)i * ld + k];
+        HIPCHK(hipMemcpyAsync(G_.p, t.data(), 8 * t.size(), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<float> t((size_t)nat * dm_.nsfmax);
+        for (int i = 0; i < nat; ++i)
+            for (int k = 0; k < dm_.nsfmax; ++k) t[(size_t)i * dm_.nsfmax + k] = (float)G[(size_t)i * ld + k];
+        HIPCHK(hipMemcpyAsync(G_.p, t.data(), 4 * t.size(), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    // the networks of `net` on this engine's symmetry functions and lists
+    std::vector<DevModel> keep_m(1, net.dm_);
+    std::vector<int> keep_w(1, net.mlp_maxw_);
+    std::swap(dmm_, keep_m);
+    std::swap(maxw_m_, keep_w);
+    skip_values_ = true;
+    auto restore = [&] {
+        std::swap(dmm_, keep_m);
+        std::swap(maxw_m_, keep_w);
+        skip_values_ = false;
+    };
+    double dl[9];
+    try {
+        if (!finish(fout_.as<double>(), stream_, energy, dl)) throw Error(6, "neighbour capacity exceeded");
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
+    if (dEdLat) std::memcpy(dEdLat, dl, sizeof dl);
+    d2h(force, fout_.p, 24 * (size_t)nat);
 }
 
 // a device-to-host copy ordered on the engine's stream (not the legacy one)

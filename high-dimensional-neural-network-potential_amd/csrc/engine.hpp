@@ -104,6 +104,29 @@ class Engine {
                          const double *lat, double *energy, double *force_dev, double *dEdLat,
                          hipStream_t stream);
 
+    // ---- the reference's stage API (hdnnpCalcNeighborLists,
+    // hdnnpCalcSymfunctions, hdnnpCalcNNs; hdnnps.f90:373-562) ----
+    // The engine keeps one stage: the lists of one structure (host arrays in).
+    // stage_build builds them and returns the stage's id; stage_current tells
+    // whether they are still the engine's (no other build since).
+    int64_t stage_build(int nat, const int *Z, const double *pos, int periodic, const double *lat);
+    bool stage_current(int64_t id) const;
+    // per atom nneis[i] = 1 + its entries (slot 1 = the atom itself); with
+    // ineis / neis (may be null) the lists in the reference's layout and
+    // order: 1-based atom numbers, positions of the images (neighborLists.f90:
+    // 118-127, 211-213)
+    void stage_neighbors(int32_t *nneis, int32_t *ineis, double *neis);
+    // scaled G[i * ld + s] of the current stage: together != 0 through the
+    // fused values pass (calcAllSymmetryFunctions), else one function at a
+    // time (symmetryFunction2 / 3); dsf (may be null) gets the scaled dG/dR
+    // rows of every atom, function and slot (both ways through the per-function
+    // kernel)
+    void stage_symfunctions(int together, int ld, double *G, double *dsf);
+    // networks of `net` (one potential, same elements and SF counts, same
+    // precision) on the G given (scaled, G[i * ld + s]) and this engine's
+    // current stage: E, forces, dEdLat (may be null)
+    void stage_nns(Engine &net, int ld, const double *G, double *energy, double *force, double *dEdLat);
+
     int64_t neighbor_count();
     void get_neighbors(int64_t cap, int32_t *ent);
     void get_symfunctions(int ld, double *G, double *dEdG, double *Ei);
@@ -234,6 +257,14 @@ class Engine {
         const double *dev = nullptr;
         size_t bytes = 0;
     } pending_force_;
+    // stage API state
+    int64_t stage_id_ = 0, stage_next_ = 0;
+    unsigned long long stage_gen_ = 0;
+    StructIn stage_in_{};
+    std::vector<double> stage_pos_;     // the stage's input positions
+    bool skip_values_ = false;          // compute: G is already in G_ (stage_nns)
+    bool trip_valid_ = false;           // trip_ holds the values pass's triplets of the current build
+    DevMem sfpar_, sfpar_off_, stage_G_, stage_dsf_, stage_off_;   // per-function path
     int last_nat_ = 0, last_cap_ = 0, last_lim_ = 0;
     int64_t last_total_ = 0;   // -1: not yet summed
     int cap_hint_ = 0;         // most neighbours of one atom seen so far

@@ -1252,6 +1252,147 @@ int sf_bucket(int nmax)
     return b;
 }
 
+// ---------------------------------------------------------------- per-function path
+// symmetryFunction2 / symmetryFunction3 (symmetryFunctions.f90:41-152), which
+// hdnnpCalcSymfunctions runs with calcSfsTogether = .false.
+// (hdnnps.f90:436-467): one function at a time, with dG/dR materialised as
+// the reference stores it (per atom, per function, one row per list slot).
+// One wavefront per atom. The atom's list entries are staged in LDS as
+// d = R_i - R_n, r, fc and fc'/r. Lane q owns entry q (and q + 64, ...): its
+// gradient row is the sum over the partners k of the terms of the pairs
+// (q, k) it belongs to, so no two lanes write one row and every sum runs in
+// a fixed order; the value is half the sum of the lanes' pair sums (each pair
+// is seen from both ends); the self row is minus the sum of the other rows
+// (translation invariance). Values and rows are centred / scaled as the
+// fused pass's.
+template <typename R>
+__global__ __launch_bounds__(64) void k_sf_single(DevModel m, Geo g, Work w, const SfParam *par, const int *par_off,
+                                                  const long long *dsf_off, int ldG, double *Gout, double *dsf)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int n = w.count[i], e = w.eidx[i];
+    R *dx = (R *)smem, *dy = dx + w.lim, *dz = dy + w.lim, *rr = dz + w.lim, *fcs = rr + w.lim, *dfs = fcs + w.lim;
+    unsigned char *el = (unsigned char *)(dfs + w.lim);
+    const double rce = m.rc[e];
+    const R rc = (R)rce, rrc = R(1) / rc;
+    const double4 pi = w.wpos[i];
+    const Lat9 lt = atom_lattice(g, w, i);
+    for (int q = lane; q < n; q += 64) {
+        double d0, d1, d2, r;
+        int j;
+        entry_geometry(lt, w, pi, i * w.cap + q, d0, d1, d2, r, j);
+        R fc, dfc;
+        cutoff_fn<R>(m.cutoff_type, (R)r, rc, rrc, fc, dfc);
+        dx[q] = (R)d0;
+        dy[q] = (R)d1;
+        dz[q] = (R)d2;
+        rr[q] = (R)r;
+        fcs[q] = fc;
+        dfs[q] = dfc;
+        el[q] = (unsigned char)w.eidx[j];
+    }
+    __syncthreads();
+    const int s0 = par_off[e], ns = par_off[e + 1] - s0;
+    const long long row0 = dsf ? dsf_off[i] : 0;
+    for (int s = 0; s < ns; ++s) {
+        const SfParam p = par[s0 + s];
+        const R eta = (R)p.eta, rs = (R)p.rs, lam = (R)p.lambda, zeta = (R)p.zeta, pre = (R)p.pre;
+        const double c = m.sf_mean[m.sf_off[e] + s], den = m.sf_den[m.sf_off[e] + s];
+        const R iden = R(1.0 / den);
+        R val = 0, sx = 0, sy = 0, sz = 0;
+        for (int q = lane; q < n; q += 64) {
+            R gx = 0, gy = 0, gz = 0;
+            const int fq = el[q];
+            const R rq = rr[q];
+            if (p.type == 2) {
+                if (fq == p.f && rq < rc) {
+                    // G2 = exp(-eta (r - Rs)^2) fc(r); d/dR_q with d = R_i - R_q
+                    const R t = M<R>::exp(-eta * (rq - rs) * (rq - rs));
+                    val += t * fcs[q];
+                    const R k = t * (R(2) * eta * (rq - rs) * fcs[q] / rq - dfs[q]);
+                    gx = k * dx[q];
+                    gy = k * dy[q];
+                    gz = k * dz[q];
+                }
+            } else if ((fq == p.f || fq == p.g) && rq < rc) {
+                const int want = fq == p.f ? p.g : p.f;   // the partner's element
+                const R ax = dx[q], ay = dy[q], az = dz[q], fij = fcs[q], dfij = dfs[q];
+                for (int k = 0; k < n; ++k) {
+                    if (k == q || el[k] != want || !(rr[k] < rc)) continue;
+                    const R bx = dx[k], by = dy[k], bz = dz[k], rk = rr[k];
+                    // d_jk = R_q - R_k = d_ik - d_iq
+                    const R cx = bx - ax, cy = by - ay, cz = bz - az;
+                    const R rjk2 = cx * cx + cy * cy + cz * cz;
+                    if (!(rjk2 < rc * rc)) continue;
+                    const R rjk = M<R>::sqrt(rjk2);
+                    R fjk, dfjk;
+                    cutoff_in<R>(m.cutoff_type, rjk, rrc, fjk, dfjk);
+                    const R ab = ax * bx + ay * by + az * bz;
+                    const R inv = R(1) / (rq * rk);
+                    const R cost = ab * inv;
+                    R base = R(1) + lam * cost;
+                    if (base < R(0)) base = R(0);
+                    const R P = M<R>::pow(base, zeta);
+                    const R dP = zeta * lam * M<R>::pow(base, zeta - R(1));
+                    const R E = M<R>::exp(-eta * (rq * rq + rk * rk + rjk2));
+                    const R F = fij * fcs[k] * fjk;
+                    val += pre * P * E * F;
+                    // d cos / dR_q = -d_ik / (rij rik) + d_ij (d_ij . d_ik) / (rij^3 rik)
+                    const R a3 = ab * inv / (rq * rq);
+                    const R dcx = -bx * inv + ax * a3, dcy = -by * inv + ay * a3, dcz = -bz * inv + az * a3;
+                    // dF / dR_q = fc_ik (fc'_jk / r_jk d_jk fc_ij - fc'_ij / r_ij d_ij fc_jk)
+                    const R fk = fcs[k];
+                    const R u = fk * dfjk * fij, v = fk * dfij * fjk;
+                    const R dFx = u * cx - v * ax, dFy = u * cy - v * ay, dFz = u * cz - v * az;
+                    // dE / dR_q = 2 eta E (d_ij - d_jk)
+                    const R k2 = R(2) * eta * E;
+                    const R dEx = k2 * (ax - cx), dEy = k2 * (ay - cy), dEz = k2 * (az - cz);
+                    gx += pre * (P * E * dFx + P * dEx * F + dP * dcx * E * F);
+                    gy += pre * (P * E * dFy + P * dEy * F + dP * dcy * E * F);
+                    gz += pre * (P * E * dFz + P * dEz * F + dP * dcz * E * F);
+                }
+            }
+            sx += gx;
+            sy += gy;
+            sz += gz;
+            if (dsf) {
+                const long long o = row0 + 3 * ((long long)s * (n + 1) + 1 + q);
+                dsf[o + 0] = (double)(gx * iden);
+                dsf[o + 1] = (double)(gy * iden);
+                dsf[o + 2] = (double)(gz * iden);
+            }
+        }
+        val = wave_total(val);
+        sx = wave_total(sx);
+        sy = wave_total(sy);
+        sz = wave_total(sz);
+        if (lane == 0) {
+            const double G = p.type == 2 ? (double)val : 0.5 * (double)val;
+            Gout[(size_t)i * ldG + s] = (G - c) / den;
+            if (dsf) {
+                const long long o = row0 + 3 * ((long long)s * (n + 1));
+                dsf[o + 0] = -(double)(sx * iden);
+                dsf[o + 1] = -(double)(sy * iden);
+                dsf[o + 2] = -(double)(sz * iden);
+            }
+        }
+    }
+}
+
+template <typename R>
+void launch_sf_single(const DevModel &m, const Geo &g, const Work &w, const SfParam *par, const int *par_off,
+                      const long long *dsf_off, int ldG, double *G, double *dsf, hipStream_t s)
+{
+    if (w.nat == 0) return;
+    const size_t lds = (size_t)w.lim * (6 * sizeof(R) + 1) + 16;
+    hipLaunchKernelGGL(k_sf_single<R>, dim3(w.nat), dim3(64), lds, s, m, g, w, par, par_off, dsf_off, ldG, G, dsf);
+}
+template void launch_sf_single<double>(const DevModel &, const Geo &, const Work &, const SfParam *, const int *,
+                                       const long long *, int, double *, double *, hipStream_t);
+template void launch_sf_single<float>(const DevModel &, const Geo &, const Work &, const SfParam *, const int *,
+                                      const long long *, int, double *, double *, hipStream_t);
+
 // LDS of one wavefront of the kernels serving nmax neighbours with one atom
 // per wavefront (this decides the staging capacity limit)
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad)

@@ -8,6 +8,15 @@
 !       one hdnnpEnergyAndForces call with dEdLat, plus dEdLat2Stress
 !   hdnnp_md committee <nnps.txt> <structure> <out>
 !       one globalHdnnpEnergyAndForces call (committee mean, energy std)
+!   hdnnp_md stages <paramdir/> <structure> <T|F> <ndsf> <out> [<paramdir2/>|-] [mixed]
+!       the stage routines in the order of globalHdnnpEnergyAndForces
+!       (globalHdnnps.f90:47-55): hdnnpCalcNeighborLists and
+!       hdnnpCalcSymfunctions (calcSfsTogether T or F) of the first
+!       potential, then hdnnpCalcNNs of it and, given a second directory, of
+!       that potential on the first one's symmetry functions (lines energy2,
+!       force2, dEdLat2). Written as refdump stages writes the reference's.
+!   hdnnp_md repeat <structure> <n1> <n2> <n3> <out>
+!       as_repeatUnitCell of a structure, written as `read` writes one
 !   hdnnp_md read <format> <file> <out>
 !       the structure(s) of a file read by as_readXYZ / as_readFhiaims /
 !       as_readAscii ('ascii', or 'asciiang' with isAngstrom) /
@@ -23,7 +32,7 @@ program hdnnp_md
     use hdnnps
     use globalHdnnps
     implicit none
-    character(len=512) :: mode, pdir, a3, a4, a5, a6, a7
+    character(len=512) :: mode, pdir, a3, a4, a5, a6, a7, a8
     type(hdnnp) :: h
     integer :: prec
 
@@ -34,9 +43,14 @@ program hdnnp_md
     call get_command_argument(5, a5)
     call get_command_argument(6, a6)
     call get_command_argument(7, a7)
+    call get_command_argument(8, a8)
     prec = HDNNP_FP64
     if (trim(mode) == 'read') then
         call runRead(trim(pdir), trim(a3), trim(a4))
+        stop
+    end if
+    if (trim(mode) == 'repeat') then
+        call runRepeat(trim(pdir), trim(a3), trim(a4), trim(a5), trim(a6))
         stop
     end if
     select case (trim(mode))
@@ -51,6 +65,11 @@ program hdnnp_md
     case ('committee')
         call initGlobalHdnnpsFromInputFile(file=trim(pdir))
         call runCommittee(trim(a3), trim(a4))
+    case ('stages')
+        if (trim(a8) == 'mixed') prec = HDNNP_MIXED
+        call initHdnnp(h)
+        call parseRuNNerInput(h, trim(pdir), prec)
+        call runStages(h, trim(a3), trim(a4) == 'T', trim(a5), trim(a6), trim(a7), prec)
     case default
         print*, 'usage: hdnnp_md md|eval|committee ...'
         error stop 2
@@ -78,6 +97,20 @@ contains
             end do
         end if
     end subroutine writeStruct
+
+    subroutine runRepeat(sfile, a1, a2, a3, oname)
+        character(len=*), intent(in) :: sfile, a1, a2, a3, oname
+        type(atStruct) :: ats
+        integer :: n(3), u
+        read(a1, *) n(1)
+        read(a2, *) n(2)
+        read(a3, *) n(3)
+        call readStructure(sfile, ats)
+        call as_repeatUnitCell(ats, n)
+        open(newunit=u, file=oname, action='write', status='replace')
+        call writeStruct(u, ats, .false.)
+        close(u)
+    end subroutine runRepeat
 
     subroutine runRead(fmt, fname, oname)
         character(len=*), intent(in) :: fmt, fname, oname
@@ -189,6 +222,69 @@ contains
         end do
         close(u)
     end subroutine runCommittee
+
+    subroutine runStages(h, sfile, together, nd, fname, pdir2, prec)
+        type(hdnnp), intent(inout) :: h
+        character(len=*), intent(in) :: sfile, nd, fname, pdir2
+        logical, intent(in) :: together
+        integer, intent(in) :: prec
+        type(atStruct) :: ats
+        type(hdnnp) :: h2
+        type(neighborList), allocatable :: nl(:)
+        type(symFunctionContainer), allocatable :: sfs(:)
+        real(dp) :: energy, dEdLat(3,3)
+        real(dp), allocatable :: force(:,:)
+        integer :: u, i, n, s, ndsf
+
+        read(nd, *) ndsf
+        call readStructure(sfile, ats)
+        allocate(force(3, ats%nat), nl(ats%nat), sfs(ats%nat))
+        call hdnnpCalcNeighborLists(h, ats, nl)
+        call hdnnpCalcSymfunctions(h, ats, nl, sfs, together)
+        call hdnnpCalcNNs(h, ats, nl, sfs, energy, force, dEdLat)
+        open(newunit=u, file=fname, action='write', status='replace')
+        write(u, '(A,I8)') 'stages ', ats%nat
+        write(u, '(A,ES25.16E3)') 'energy ', energy
+        write(u, '(A)') 'force'
+        do i = 1, ats%nat
+            write(u, '(3ES25.16E3)') force(:, i)
+        end do
+        write(u, '(A)') 'dEdLat'
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') dEdLat(i, :)
+        end do
+        do i = 1, ats%nat
+            write(u, '(A,3I8)') 'atom ', i, nl(i)%nNeis, sfs(i)%nsf
+            write(u, '(*(I8))') nl(i)%ineis(1:nl(i)%nNeis)
+            do n = 1, nl(i)%nNeis
+                write(u, '(3ES25.16E3)') nl(i)%neis(:, n)
+            end do
+            write(u, '(*(ES25.16E3))') (sfs(i)%sfs(s)%sf, s = 1, sfs(i)%nsf)
+            if (i <= ndsf) then
+                do s = 1, sfs(i)%nsf
+                    write(u, '(*(ES25.16E3))') sfs(i)%sfs(s)%dsf(:, 1:nl(i)%nNeis)
+                end do
+            end if
+        end do
+        if (len_trim(pdir2) > 0 .and. pdir2 /= '-') then
+            ! globalHdnnps.f90:53-55: another potential's networks on these
+            ! symmetry functions
+            call initHdnnp(h2)
+            call parseRuNNerInput(h2, pdir2, prec)
+            call hdnnpCalcNNs(h2, ats, nl, sfs, energy, force, dEdLat)
+            write(u, '(A,ES25.16E3)') 'energy2 ', energy
+            write(u, '(A)') 'force2'
+            do i = 1, ats%nat
+                write(u, '(3ES25.16E3)') force(:, i)
+            end do
+            write(u, '(A)') 'dEdLat2'
+            do i = 1, 3
+                write(u, '(3ES25.16E3)') dEdLat(i, :)
+            end do
+            call hdnnpDestroy(h2)
+        end if
+        close(u)
+    end subroutine runStages
 
     subroutine runEval(h, sfile, fname)
         type(hdnnp), intent(inout) :: h

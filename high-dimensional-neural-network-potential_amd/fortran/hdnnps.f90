@@ -79,11 +79,14 @@ module hdnnp_capi
     use iso_c_binding
     implicit none
     integer(c_int), parameter :: HDNNP_FP64 = 0, HDNNP_MIXED = 1
+    ! hdnnp_model_info of include/hdnnp_capi.h, field for field
     type, bind(C) :: hdnnp_model_info
         integer(c_int) :: nelements, Z(16), nsf(16), nlayers, nodes(8)
         character(kind=c_char) :: act(8)
         integer(c_int) :: scalesf, centersf
         real(c_double) :: maxrc, atom_energy(16), emin, emax
+        integer(c_int) :: cutoff_type, scale_range
+        real(c_double) :: scale_min, scale_max
     end type hdnnp_model_info
     interface
         integer(c_int) function hdnnp_create(dir, precision, ctx) bind(C, name='hdnnp_create')
@@ -135,6 +138,51 @@ module hdnnp_capi
             import :: c_int, c_ptr, hdnnp_model_info
             type(c_ptr), value :: ctx
             type(hdnnp_model_info), intent(out) :: info
+        end function
+        integer(c_int) function hdnnp_get_sf_table(ctx, e, cap_rows, rows) bind(C, name='hdnnp_get_sf_table')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: e, cap_rows
+            real(c_double), intent(out) :: rows(*)
+        end function
+        integer(c_int) function hdnnp_stage_neighbor_lists(ctx, nat, Z, pos, periodic, lat, stage, nneis) &
+                bind(C, name='hdnnp_stage_neighbor_lists')
+            import :: c_int, c_double, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nat, periodic
+            integer(c_int), intent(in) :: Z(*)
+            real(c_double), intent(in) :: pos(*), lat(*)
+            integer(c_int64_t), intent(out) :: stage
+            integer(c_int), intent(out) :: nneis(*)
+        end function
+        integer(c_int) function hdnnp_stage_get_neighbor_lists(ctx, stage, ineis, neis) &
+                bind(C, name='hdnnp_stage_get_neighbor_lists')
+            import :: c_int, c_double, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: stage
+            integer(c_int), intent(out) :: ineis(*)
+            real(c_double), intent(out) :: neis(*)
+        end function
+        integer(c_int) function hdnnp_stage_symfunctions(ctx, stage, nat, Z, pos, periodic, lat, together, ld, G, dsf) &
+                bind(C, name='hdnnp_stage_symfunctions')
+            import :: c_int, c_double, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(inout) :: stage
+            integer(c_int), value :: nat, periodic, together, ld
+            integer(c_int), intent(in) :: Z(*)
+            real(c_double), intent(in) :: pos(*), lat(*)
+            real(c_double), intent(out) :: G(*), dsf(*)
+        end function
+        integer(c_int) function hdnnp_stage_nns(net, sfctx, stage, nat, Z, pos, periodic, lat, ld, G, energy, &
+                force, dEdLat) bind(C, name='hdnnp_stage_nns')
+            import :: c_int, c_double, c_ptr, c_int64_t
+            type(c_ptr), value :: net, sfctx
+            integer(c_int64_t), intent(inout) :: stage
+            integer(c_int), value :: nat, periodic, ld
+            integer(c_int), intent(in) :: Z(*)
+            real(c_double), intent(in) :: pos(*), lat(*), G(*)
+            real(c_double), intent(out) :: energy, force(*)
+            type(c_ptr), value :: dEdLat
         end function
     end interface
 contains
@@ -545,17 +593,76 @@ contains
     end subroutine center
 end module atomicStructure
 
+! type neighborList of the reference (src/potentials/neighborLists.f90:10-16):
+! slot 1 is the atom itself; neis holds the positions of the images, ineis
+! their atom numbers; arrays grow from 16 by doubling (maxNeis).
+module neighborLists
+    use precision
+    implicit none
+    type neighborList
+        integer :: nNeis = 0
+        integer :: maxNeis = 0
+        real(dp), allocatable :: neis(:,:)
+        integer, allocatable :: ineis(:)
+        ! not in the reference: the library stage the list came from
+        integer(c_int64_t) :: stage = 0
+    end type neighborList
+end module neighborLists
+
+! types of the reference's symmetry functions (src/potentials/
+! symmetryFunctions.f90:7-33): the parsed functions of an element
+! (symFunctionList) and the values and gradients of one atom
+! (symFunctionContainer; dsf(:, n) = dG/dR of list slot n).
+module symmetryFunctions
+    use precision
+    implicit none
+    type symFunction
+        integer :: name
+        integer :: sfEls(2)
+        real(dp) :: rcut
+        real(dp) :: eta
+        real(dp) :: lambda
+        real(dp) :: zeta
+        real(dp) :: Rs
+        real(dp) :: sfmin, sfmax, sfmean
+    end type symFunction
+
+    type symFunctionList
+        integer :: nsf = 0
+        type(symFunction), allocatable :: sfs(:)
+    end type symFunctionList
+
+    type symFunctionValue
+        integer :: nNeis
+        real(dp) :: sf
+        real(dp), allocatable :: dsf(:,:)
+    end type symFunctionValue
+
+    type symFunctionContainer
+        integer :: nsf = 0
+        type(symFunctionValue), allocatable :: sfs(:)
+        ! not in the reference: the context whose lists and symmetry functions
+        ! these are, and its stage
+        type(c_ptr) :: ctx = c_null_ptr
+        integer(c_int64_t) :: stage = 0
+    end type symFunctionContainer
+end module symmetryFunctions
+
 module hdnnps
     use precision
     use atomicStructure
     use constants
     use hdnnp_capi
+    use neighborLists
+    use symmetryFunctions
     implicit none
 
-    ! the reference's type(hdnnp) (hdnnps.f90:13-21) holds the parsed tables;
-    ! here they live in the library context behind `ctx`
+    ! the reference's type(hdnnp) (hdnnps.f90:13-21). The parsed tables live
+    ! in the library context behind `ctx`; sfLists holds a copy of the sorted
+    ! symmetry functions (the networks, nnList, stay in the library).
     type hdnnp
         type(c_ptr) :: ctx = c_null_ptr
+        type(symFunctionList) :: sfLists(maxElemNum)
         real(dp) :: maxRc = 0._dp
         real(dp) :: emin = 0._dp, emax = 0._dp
         real(dp) :: atomenergies(maxElemNum) = 0._dp
@@ -564,6 +671,14 @@ module hdnnps
     end type hdnnp
 
 contains
+
+    ! initHdnnp (hdnnps.f90:25-32): an empty handle
+    subroutine initHdnnp(handle)
+        type(hdnnp), intent(out) :: handle
+        handle%atomenergies = 0._dp
+        handle%scalesf = .false.
+        handle%centersf = .false.
+    end subroutine initHdnnp
 
     ! parseRuNNerInput (hdnnps.f90:34-233). `path` is the directory prefix of
     ! input.nn, scaling.data and weights.ZZZ.data. `precision` (optional,
@@ -600,7 +715,200 @@ contains
         do e = 1, info%nelements
             handle%atomenergies(info%Z(e)) = info%atom_energy(e)
         end do
+        call fillSfLists(handle, info)
     end subroutine fillHandle
+
+    ! handle%sfLists from the library's sorted tables (rows of type, Z1, Z2,
+    ! rcut, eta, Rs, lambda, zeta, min, max, mean)
+    subroutine fillSfLists(handle, info)
+        type(hdnnp), intent(inout) :: handle
+        type(hdnnp_model_info), intent(in) :: info
+        real(dp), allocatable :: rows(:,:)
+        integer :: e, s, z, n
+        integer(c_int) :: rc
+        do z = 1, maxElemNum
+            handle%sfLists(z)%nsf = 0
+            if (allocated(handle%sfLists(z)%sfs)) deallocate(handle%sfLists(z)%sfs)
+        end do
+        do e = 1, info%nelements
+            z = info%Z(e)
+            n = info%nsf(e)
+            allocate(rows(11, max(n, 1)), handle%sfLists(z)%sfs(n))
+            rc = hdnnp_get_sf_table(handle%ctx, int(e - 1, c_int), int(n, c_int), rows)
+            handle%sfLists(z)%nsf = n
+            do s = 1, n
+                handle%sfLists(z)%sfs(s)%name = nint(rows(1, s))
+                handle%sfLists(z)%sfs(s)%sfEls = [nint(rows(2, s)), nint(rows(3, s))]
+                handle%sfLists(z)%sfs(s)%rcut = rows(4, s)
+                handle%sfLists(z)%sfs(s)%eta = rows(5, s)
+                handle%sfLists(z)%sfs(s)%Rs = rows(6, s)
+                handle%sfLists(z)%sfs(s)%lambda = rows(7, s)
+                handle%sfLists(z)%sfs(s)%zeta = rows(8, s)
+                handle%sfLists(z)%sfs(s)%sfmin = rows(9, s)
+                handle%sfLists(z)%sfs(s)%sfmax = rows(10, s)
+                handle%sfLists(z)%sfs(s)%sfmean = rows(11, s)
+            end do
+            deallocate(rows)
+        end do
+    end subroutine fillSfLists
+
+    ! the structure as the C ABI takes it
+    subroutine cStructure(ats, per, lat)
+        type(atStruct), intent(in) :: ats
+        integer(c_int), intent(out) :: per
+        real(dp), intent(out) :: lat(3,3)
+        per = 0
+        lat = 0._dp
+        if (ats%periodic) then
+            per = 1
+            lat = ats%lat
+        end if
+    end subroutine cStructure
+
+    subroutine stopOnError(rc, ctx)
+        integer(c_int), intent(in) :: rc
+        type(c_ptr), intent(in) :: ctx
+        if (rc /= 0) then
+            print*, c_message(hdnnp_last_error(ctx))
+            error stop 1
+        end if
+    end subroutine stopOnError
+
+    ! hdnnpCalcNeighborLists (hdnnps.f90:373-388): the linked-cell lists of
+    ! the (wrapped) structure within maxRc, built on the device, in the
+    ! reference's layout and order
+    subroutine hdnnpCalcNeighborLists(handle, ats, neiLists)
+        type(hdnnp), intent(in) :: handle
+        type(atStruct), intent(in) :: ats
+        type(neighborList), intent(out) :: neiLists(ats%nat)
+        integer(c_int) :: per, rc
+        real(dp) :: lat(3,3)
+        integer(c_int64_t) :: stage
+        integer(c_int), allocatable :: nneis(:), ineis(:)
+        real(dp), allocatable :: neis(:,:)
+        integer :: i, q, cap
+        call cStructure(ats, per, lat)
+        allocate(nneis(max(ats%nat, 1)))
+        rc = hdnnp_stage_neighbor_lists(handle%ctx, int(ats%nat, c_int), ats%el, ats%ats, per, lat, stage, nneis)
+        call stopOnError(rc, handle%ctx)
+        allocate(ineis(max(sum(nneis(1:ats%nat)), 1)), neis(3, max(sum(nneis(1:ats%nat)), 1)))
+        rc = hdnnp_stage_get_neighbor_lists(handle%ctx, stage, ineis, neis)
+        call stopOnError(rc, handle%ctx)
+        q = 0
+        do i = 1, ats%nat
+            cap = 16                     ! the reference's capacity: 16, doubled when full
+            do while (cap < nneis(i))
+                cap = 2 * cap
+            end do
+            neiLists(i)%nNeis = nneis(i)
+            neiLists(i)%maxNeis = cap
+            allocate(neiLists(i)%neis(3, cap), neiLists(i)%ineis(cap))
+            neiLists(i)%neis = 0._dp
+            neiLists(i)%ineis = -1
+            neiLists(i)%neis(:, 1:nneis(i)) = neis(:, q + 1:q + nneis(i))
+            neiLists(i)%ineis(1:nneis(i)) = ineis(q + 1:q + nneis(i))
+            neiLists(i)%stage = stage
+            q = q + nneis(i)
+        end do
+    end subroutine hdnnpCalcNeighborLists
+
+    ! hdnnpCalcSymfunctions (hdnnps.f90:390-485): centred and scaled values
+    ! and dG/dR of every atom's symmetry functions. calcSfsTogether (default
+    ! .true.) selects the fused pass (calcAllSymmetryFunctions) for the
+    ! values; .false. the per-function kernel (symmetryFunction2/3), which
+    ! gives the gradients either way.
+    subroutine hdnnpCalcSymfunctions(handle, ats, neiLists, sfs, calcSfsTogether)
+        type(hdnnp), intent(in) :: handle
+        type(atStruct), intent(in) :: ats
+        type(neighborList), intent(in) :: neiLists(ats%nat)
+        type(symFunctionContainer), intent(out) :: sfs(ats%nat)
+        logical, intent(in), optional :: calcSfsTogether
+        integer(c_int) :: per, rc, together, ld
+        real(dp) :: lat(3,3)
+        integer(c_int64_t) :: stage
+        real(dp), allocatable :: G(:,:), dsf(:)
+        integer(c_int64_t) :: total, o
+        integer :: i, s, nsf, nn
+        together = 1
+        if (present(calcSfsTogether)) then
+            if (.not. calcSfsTogether) together = 0
+        end if
+        call cStructure(ats, per, lat)
+        stage = 0
+        if (ats%nat > 0) stage = neiLists(1)%stage
+        ld = 1
+        total = 0
+        do i = 1, ats%nat
+            nsf = handle%sfLists(ats%el(i))%nsf
+            ld = max(ld, int(nsf, c_int))
+            total = total + 3_c_int64_t * nsf * neiLists(i)%nNeis
+        end do
+        allocate(G(ld, max(ats%nat, 1)), dsf(max(total, 1_c_int64_t)))
+        rc = hdnnp_stage_symfunctions(handle%ctx, stage, int(ats%nat, c_int), ats%el, ats%ats, per, lat, &
+                                      together, ld, G, dsf)
+        call stopOnError(rc, handle%ctx)
+        o = 0
+        do i = 1, ats%nat
+            nsf = handle%sfLists(ats%el(i))%nsf
+            nn = neiLists(i)%nNeis
+            sfs(i)%nsf = nsf
+            sfs(i)%ctx = handle%ctx
+            sfs(i)%stage = stage
+            allocate(sfs(i)%sfs(nsf))
+            do s = 1, nsf
+                sfs(i)%sfs(s)%nNeis = nn
+                sfs(i)%sfs(s)%sf = G(s, i)
+                allocate(sfs(i)%sfs(s)%dsf(3, nn))
+                sfs(i)%sfs(s)%dsf = reshape(dsf(o + 1:o + 3 * nn), [3, nn])
+                o = o + 3 * nn
+            end do
+        end do
+    end subroutine hdnnpCalcSymfunctions
+
+    ! hdnnpCalcNNs (hdnnps.f90:487-562): this handle's networks and atom
+    ! energies on the symmetry functions in sfs; forces and dEdLat contract
+    ! dE/dG with the gradients of the symmetry functions of the context that
+    ! computed sfs, recomputed on the device from that context's lists (the
+    ! stage neiLists came from), so neiLists and the dsf in sfs are not read
+    subroutine hdnnpCalcNNs(handle, ats, neiLists, sfs, energy, force, dEdLat)
+        type(hdnnp), intent(inout) :: handle
+        type(atStruct), intent(in) :: ats
+        type(neighborList), intent(in) :: neiLists(ats%nat)
+        type(symFunctionContainer), intent(in) :: sfs(ats%nat)
+        real(dp), intent(out) :: energy
+        real(dp), intent(out) :: force(3, ats%nat)
+        real(dp), intent(out), optional, target :: dEdLat(3,3)
+        integer(c_int) :: per, rc, ld
+        real(dp) :: lat(3,3)
+        integer(c_int64_t) :: stage
+        type(c_ptr) :: sfctx, pl
+        real(dp), allocatable :: G(:,:)
+        integer :: i, s
+        call cStructure(ats, per, lat)
+        ld = 1
+        do i = 1, ats%nat
+            ld = max(ld, int(sfs(i)%nsf, c_int))
+        end do
+        allocate(G(ld, max(ats%nat, 1)))
+        G = 0._dp
+        do i = 1, ats%nat
+            do s = 1, sfs(i)%nsf
+                G(s, i) = sfs(i)%sfs(s)%sf
+            end do
+        end do
+        stage = 0
+        sfctx = c_null_ptr
+        if (ats%nat > 0) then
+            stage = sfs(1)%stage
+            sfctx = sfs(1)%ctx
+        end if
+        if (.not. c_associated(sfctx)) sfctx = handle%ctx
+        pl = c_null_ptr
+        if (present(dEdLat)) pl = c_loc(dEdLat)
+        rc = hdnnp_stage_nns(handle%ctx, sfctx, stage, int(ats%nat, c_int), ats%el, ats%ats, per, lat, ld, G, &
+                             energy, force, pl)
+        call stopOnError(rc, handle%ctx)
+    end subroutine hdnnpCalcNNs
 
     ! hdnnpEnergyAndForces (hdnnps.f90:333-371)
     subroutine hdnnpEnergyAndForces(handle, ats, energy, force, dEdLat)

@@ -127,6 +127,48 @@ int hdnnp_evaluate_device(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z_dev,
                           const double *pos_dev, int32_t periodic, const double *lat,
                           double *energy, double *force_dev, double *dEdLat, void *stream);
 
+/* The reference's stage API (hdnnps.f90:373-562): hdnnpCalcNeighborLists,
+ * hdnnpCalcSymfunctions and hdnnpCalcNNs, which globalHdnnps.f90:47-57 calls
+ * to share one potential's lists and symmetry functions among several
+ * potentials' networks. A context keeps one stage (the lists of one
+ * structure) on the device; `stage` names it. Every call but
+ * hdnnp_stage_get_neighbor_lists takes the structure again (as the
+ * reference's routines take ats): when the context has built other lists
+ * since, the stage's lists are built again from it and *stage is renewed.
+ * Structure arguments as for hdnnp_evaluate.
+ *
+ * hdnnpCalcNeighborLists (hdnnps.f90:373-388): builds the lists;
+ * nneis[nat] = entries per atom with the atom itself in slot 1
+ * (neighborLists.f90:118-127). */
+int hdnnp_stage_neighbor_lists(hdnnp_ctx *ctx, int32_t nat, const int32_t *Z, const double *pos, int32_t periodic,
+                               const double *lat, int64_t *stage, int32_t *nneis);
+/* The lists of the context's current stage in the reference's layout and
+ * order (type neighborList, neighborLists.f90:10-16), atom after atom:
+ * ineis[sum nneis] 1-based atom numbers, neis[sum nneis][3] the positions of
+ * the images (slot 1 the atom as the lists were built from it: wrapped into
+ * the cell when periodic). Either pointer may be NULL. */
+int hdnnp_stage_get_neighbor_lists(hdnnp_ctx *ctx, int64_t stage, int32_t *ineis, double *neis);
+/* hdnnpCalcSymfunctions (hdnnps.f90:390-485): scaled G[nat][ld] (ld >= the
+ * largest SF count; columns an element lacks are 0) and, when dsf is not
+ * NULL, the scaled dG/dR of the reference's symFunctionContainer
+ * (symmetryFunctions.f90:24-33), atom after atom, function after function,
+ * slot after slot: dsf[..][3] for the nneis slots of the atom. together != 0
+ * takes G from the fused pass (calcAllSymmetryFunctions,
+ * symmetryFunctions.f90:158-308), 0 from the per-function kernel
+ * (symmetryFunction2 / symmetryFunction3, :41-152); dsf always comes from the
+ * per-function kernel. */
+int hdnnp_stage_symfunctions(hdnnp_ctx *ctx, int64_t *stage, int32_t nat, const int32_t *Z, const double *pos,
+                             int32_t periodic, const double *lat, int32_t together, int32_t ld, double *G, double *dsf);
+/* hdnnpCalcNNs (hdnnps.f90:487-562): the networks and atom energies of `net`
+ * (one potential) on the symmetry functions G[nat][ld] given and the lists
+ * of sfctx's stage (NULL: net's): energy, force[nat][3], dEdLat (may be
+ * NULL). The forces and dEdLat contract dE/dG with the gradients of sfctx's
+ * symmetry functions, recomputed on the device from its lists. Both contexts
+ * need the same precision, elements and SF counts. */
+int hdnnp_stage_nns(hdnnp_ctx *net, hdnnp_ctx *sfctx, int64_t *stage, int32_t nat, const int32_t *Z, const double *pos,
+                    int32_t periodic, const double *lat, int32_t ld, const double *G, double *energy, double *force,
+                    double *dEdLat);
+
 /* Stage outputs of the last evaluation, for the reference's stage API:
  * hdnnpCalcNeighborLists (hdnnps.f90:373-388) -> neighbour entries
  * (i, j, Lx, Ly, Lz), 0-based atoms, L the integer image of j, self slot

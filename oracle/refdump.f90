@@ -22,6 +22,14 @@
 !       velocity-Verlet loop of hdnnpExample.f90:58-72 with given momenta.
 !   refdump time  <paramdir/> <structure> <nrep> <out>
 !       wall time of hdnnpEnergyAndForces (E+F+dE/dh), warm-up 1, nrep timed.
+!   refdump stages <paramdir/> <structure> <T|F> <ndsf> <out>
+!       the stage routines in the order hdnnpEnergyAndForces calls them:
+!       hdnnpCalcNeighborLists, hdnnpCalcSymfunctions (calcSfsTogether T or
+!       F) and hdnnpCalcNNs (hdnnps.f90:373-562): energy, forces, dEdLat,
+!       then per atom its list (ineis, neis), its scaled G and, for the first
+!       ndsf atoms, dsf of every function.
+!   refdump repeat <structure> <n1> <n2> <n3> <out>
+!       as_repeatUnitCell (atomicStructure.f90:636-671) of a structure.
 !   refdump read  <format> <file> <out>
 !       the structure(s) one of the reference's readers makes of a file:
 !       format xyz (as_readXYZ), fhiaims (as_readFhiaims), ascii / asciiang
@@ -53,6 +61,24 @@ program refdump
         call dumpRead(trim(pdir), trim(sfile), trim(ofile))
         stop
     end if
+    if (trim(mode) == 'repeat') then
+        block
+            type(atStruct) :: ats
+            integer :: n(3), k, u
+            character(len=32) :: a
+            do k = 1, 3
+                call get_command_argument(2 + k, a)
+                read(a, *) n(k)
+            end do
+            call get_command_argument(6, ofile)
+            call readStruct(trim(pdir), ats)
+            call as_repeatUnitCell(ats, n)
+            open(newunit=u, file=trim(ofile), action='write', status='replace')
+            call writeStruct(u, ats, .false.)
+            close(u)
+        end block
+        stop
+    end if
     call parseRuNNerInput(h, trim(pdir))
 
     select case (trim(mode))
@@ -72,6 +98,17 @@ program refdump
         call get_command_argument(3, sfile)
         call get_command_argument(5, ofile)
         call runTime(h, trim(sfile), trim(ofile))
+    case ('stages')
+        call get_command_argument(3, sfile)
+        call get_command_argument(4, arg)
+        call get_command_argument(6, ofile)
+        block
+            character(len=32) :: nd
+            integer :: ndsf
+            call get_command_argument(5, nd)
+            read(nd, *) ndsf
+            call dumpStages(h, trim(sfile), trim(arg) == 'T', ndsf, trim(ofile))
+        end block
     case default
         print*, 'unknown mode'
         stop 2
@@ -263,6 +300,51 @@ contains
         end do
         close(u)
     end subroutine dumpEval
+
+    subroutine dumpStages(h, sfile, together, ndsf, fname)
+        type(hdnnp), intent(inout) :: h
+        character(len=*), intent(in) :: sfile, fname
+        logical, intent(in) :: together
+        integer, intent(in) :: ndsf
+        type(atStruct) :: ats
+        type(neighborList), allocatable :: nl(:)
+        type(symFunctionContainer), allocatable :: sfs(:)
+        real(dp) :: energy, dEdLat(3,3)
+        real(dp), allocatable :: force(:,:)
+        integer :: u, i, n, s
+
+        call readStruct(sfile, ats)
+        allocate(force(3, ats%nat), nl(ats%nat), sfs(ats%nat))
+        call hdnnpCalcNeighborLists(h, ats, nl)
+        call hdnnpCalcSymfunctions(h, ats, nl, sfs, together)
+        call hdnnpCalcNNs(h, ats, nl, sfs, energy, force, dEdLat)
+        if (.not. ats%periodic) dEdLat = 0._dp
+        open(newunit=u, file=fname, action='write', status='replace')
+        write(u, '(A,I8)') 'stages ', ats%nat
+        write(u, '(A,ES25.16E3)') 'energy ', energy
+        write(u, '(A)') 'force'
+        do i = 1, ats%nat
+            write(u, '(3ES25.16E3)') force(:, i)
+        end do
+        write(u, '(A)') 'dEdLat'
+        do i = 1, 3
+            write(u, '(3ES25.16E3)') dEdLat(i, :)
+        end do
+        do i = 1, ats%nat
+            write(u, '(A,3I8)') 'atom ', i, nl(i)%nNeis, sfs(i)%nsf
+            write(u, '(*(I8))') nl(i)%ineis(1:nl(i)%nNeis)
+            do n = 1, nl(i)%nNeis
+                write(u, '(3ES25.16E3)') nl(i)%neis(:, n)
+            end do
+            write(u, '(*(ES25.16E3))') (sfs(i)%sfs(s)%sf, s = 1, sfs(i)%nsf)
+            if (i <= ndsf) then
+                do s = 1, sfs(i)%nsf
+                    write(u, '(*(ES25.16E3))') sfs(i)%sfs(s)%dsf(:, 1:nl(i)%nNeis)
+                end do
+            end if
+        end do
+        close(u)
+    end subroutine dumpStages
 
     subroutine runMD(h, xyz, pfile, fname)
         type(hdnnp), intent(inout) :: h

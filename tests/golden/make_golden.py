@@ -11,6 +11,8 @@ files are copied as data fixtures into tests/golden/methylammonium/.
     python tests/golden/make_golden.py md-potential   # only data/perovskite_md
     python tests/golden/make_golden.py readers        # only tests/golden/readers.npz
     python tests/golden/make_golden.py cos-potential  # only data/perovskite_cos
+    python tests/golden/make_golden.py stages         # only tests/golden/stages_*.npz
+    python tests/golden/make_golden.py repeat         # only tests/golden/repeat.npz
 """
 from __future__ import annotations
 
@@ -237,6 +239,96 @@ def make_cosine_potential():
     print("perovskite_cos written")
 
 
+def parse_stages(path):
+    """The output of `refdump stages` (and of `hdnnp_md stages`): energy,
+    forces, dEdLat, then per atom nNeis, ineis, neis, G and, for the first
+    ndsf atoms, dsf. Returns flat arrays."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    nat = int(lines[0].split()[1])
+    out = {"energy": float(lines[1].split()[1])}
+    k = 3
+    out["force"] = np.array([[float(t) for t in lines[k + i].split()] for i in range(nat)])
+    k += nat + 1
+    out["dEdLat"] = np.array([[float(t) for t in lines[k + i].split()] for i in range(3)]).T.copy()
+    k += 3
+    nneis, nsf, ineis, neis, G, dsf = [], [], [], [], [], []
+    ndsf = 0
+    for i in range(nat):
+        p = lines[k].split()
+        assert p[0] == "atom" and int(p[1]) == i + 1
+        nn, ns = int(p[2]), int(p[3])
+        nneis.append(nn)
+        nsf.append(ns)
+        ineis.extend(int(t) for t in lines[k + 1].split())
+        for n in range(nn):
+            neis.append([float(t) for t in lines[k + 2 + n].split()])
+        G.extend(float(t) for t in lines[k + 2 + nn].split())
+        k += 3 + nn
+        if k < len(lines) and lines[k] and not lines[k].startswith(("atom", "energy2")):
+            for s in range(ns):
+                dsf.extend(float(t) for t in lines[k + s].split())
+            k += ns
+            ndsf += 1
+    out.update(nneis=np.array(nneis), nsf=np.array(nsf), ineis=np.array(ineis), neis=np.array(neis),
+               G=np.array(G), dsf=np.array(dsf), ndsf=np.array(ndsf))
+    if k < len(lines) and lines[k].startswith("energy2"):
+        out["energy2"] = float(lines[k].split()[1])
+        out["force2"] = np.array([[float(t) for t in lines[k + 2 + i].split()] for i in range(nat)])
+        k += nat + 3
+        out["dEdLat2"] = np.array([[float(t) for t in lines[k + i].split()] for i in range(3)]).T.copy()
+    return out
+
+
+STAGE_CASES = (("ma_tric2", "methylammonium", 16), ("cluster30", "methylammonium", 30), ("perov1", "perovskite", 3))
+
+
+REPEAT_CASES = (("perov1", (2, 3, 1)), ("ma_tric2", (1, 2, 2)))
+
+
+def make_repeat_goldens():
+    """tests/golden/repeat.npz: as_repeatUnitCell (atomicStructure.f90:636-671)
+    of two golden cells, as the reference's own routine writes them."""
+    out = {}
+    for case, n in REPEAT_CASES:
+        d = np.load(os.path.join(GOLD, f"eval_{case}.npz"))
+        s = wl.Structure(np.array(d["pos"]), np.array(d["Z"], np.int32), np.array(d["lat"]), bool(d["periodic"]))
+        with tempfile.TemporaryDirectory() as td:
+            sf = os.path.join(td, "s.txt")
+            of = os.path.join(td, "o.txt")
+            wl.write_structure(sf, s)
+            run("repeat", sf, *[str(k) for k in n], of)
+            st = parse_read(of)[0]
+        out[f"{case}_n"] = np.array(n)
+        out[f"{case}_in_pos"], out[f"{case}_in_Z"], out[f"{case}_in_lat"] = s.pos, s.Z, s.lat
+        for k in ("pos", "Z", "lat", "periodic"):
+            out[f"{case}_{k}"] = np.asarray(st[k])
+        print(case, n, len(st["Z"]))
+    np.savez_compressed(os.path.join(GOLD, "repeat.npz"), **out)
+
+
+def make_stage_goldens():
+    """tests/golden/stages_<case>_<T|F>.npz: the reference's stage routines
+    (hdnnpCalcNeighborLists, hdnnpCalcSymfunctions with calcSfsTogether T
+    and F, hdnnpCalcNNs) on three golden structures: a triclinic cell with
+    self-images, a cluster and a 5-element perovskite cell (dsf of its first
+    3 atoms)."""
+    for case, pot, ndsf in STAGE_CASES:
+        d = np.load(os.path.join(GOLD, f"eval_{case}.npz"))
+        s = wl.Structure(np.array(d["pos"]), np.array(d["Z"], np.int32), np.array(d["lat"]), bool(d["periodic"]))
+        pdir = MA_DIR if pot == "methylammonium" else PEROV_DIR
+        for flag in ("T", "F"):
+            with tempfile.TemporaryDirectory() as td:
+                sf = os.path.join(td, "s.txt")
+                of = os.path.join(td, "o.txt")
+                wl.write_structure(sf, s)
+                run("stages", os.path.join(pdir, ""), sf, flag, str(ndsf), of)
+                r = parse_stages(of)
+            r.update(pos=s.pos, Z=s.Z, lat=s.lat, periodic=np.array(int(s.periodic)), potential=np.array(pot))
+            np.savez_compressed(os.path.join(GOLD, f"stages_{case}_{flag}.npz"), **r)
+            print(case, flag, s.nat, r["energy"], int(r["nneis"].sum()), r["dsf"].size)
+
+
 READER_CASES = (("fhiaims", "geometry.in"), ("fhiaims", "molecule.in"), ("ascii", "cell_ang.ascii"),
                 ("ascii", "cell_bohr.ascii"), ("asciiang", "cell_bohr.ascii"), ("poscar", "POSCAR"),
                 ("data", "input.data"), ("data", "big.data"))
@@ -380,5 +472,9 @@ if __name__ == "__main__":
         make_reader_goldens()
     elif sys.argv[1:] == ["cos-potential"]:   # only data/perovskite_cos
         make_cosine_potential()
+    elif sys.argv[1:] == ["stages"]:          # only tests/golden/stages_*.npz
+        make_stage_goldens()
+    elif sys.argv[1:] == ["repeat"]:          # only tests/golden/repeat.npz
+        make_repeat_goldens()
     else:
         main()

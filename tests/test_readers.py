@@ -107,3 +107,24 @@ def test_fortran_reader_matches_reference(fmt, fn, tmp_path):
             assert s["energy"] == w["energy"] and s["qtot"] == w["qtot"]
             np.testing.assert_array_equal(np.array(s["aux"])[:, 0], w["charges"])
             np.testing.assert_array_equal(np.array(s["aux"])[:, 1:], w["forces"])
+
+
+@pytest.mark.parametrize("case", ["perov1", "ma_tric2"])
+def test_fortran_repeat_unit_cell_matches_reference(wl, case, tmp_path):
+    """as_repeatUnitCell (atomicStructure.f90:636-671) of the Fortran
+    atomicStructure module (driver mode `hdnnp_md repeat`) against the
+    reference's own routine (tests/golden/repeat.npz, written by refdump
+    repeat): atom order (x, y, z repetitions, atoms innermost), positions and
+    lattice bit for bit."""
+    d = np.load(os.path.join(GOLD, "repeat.npz"))
+    n = [int(k) for k in d[f"{case}_n"]]
+    s = wl.Structure(d[f"{case}_in_pos"], d[f"{case}_in_Z"].astype(np.int32), d[f"{case}_in_lat"], True)
+    sf, out = tmp_path / "s.txt", tmp_path / "o.txt"
+    wl.write_structure(str(sf), s)
+    exe = os.path.join(ROOT, "high-dimensional-neural-network-potential_amd", "lib", "hdnnp_md")
+    subprocess.run([exe, "repeat", str(sf), *map(str, n), str(out)], check=True, capture_output=True)
+    got = parse_dump(str(out))[0]
+    assert np.array_equal(np.array(got["Z"]), d[f"{case}_Z"])
+    np.testing.assert_array_equal(np.array(got["pos"]).reshape(-1, 3), d[f"{case}_pos"])
+    np.testing.assert_array_equal(np.array(got["lat"]), d[f"{case}_lat"])
+    assert len(got["Z"]) == s.nat * n[0] * n[1] * n[2]
