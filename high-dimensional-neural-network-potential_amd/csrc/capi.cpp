@@ -376,12 +376,12 @@ int hdnnp_get_sf_table(hdnnp_ctx *ctx, int32_t e, int32_t cap_rows, double *rows
         double *r = rows + 11 * s;
         r[0] = f.type;
         r[1] = f.e1;
-        r[2] = f.type == 3 ? f.e2 : 0;
+        r[2] = f.type != 2 ? f.e2 : 0;
         r[3] = f.rcut;
         r[4] = f.eta;
         r[5] = f.type == 2 ? f.rs : 0.0;
-        r[6] = f.type == 3 ? f.lambda : 0.0;
-        r[7] = f.type == 3 ? f.zeta : 0.0;
+        r[6] = f.type != 2 ? f.lambda : 0.0;
+        r[7] = f.type != 2 ? f.zeta : 0.0;
         r[8] = f.sfmin;
         r[9] = f.sfmax;
         r[10] = f.sfmean;

@@ -20,6 +20,14 @@ struct ElemInfo {
     double rc, atom_energy;
 };
 
+// One symmetry function of the per-function path and of the G5 kernels:
+// type 2 (G2, neighbour element rank f), 3 (G4) or 9 (G5, element pair
+// {f, g}); s its index in its element's sorted list; pre = 2^(1 - zeta).
+struct SfParam {
+    int type, f, g, s;
+    double eta, rs, lambda, zeta, pre;
+};
+
 // Model tables in device memory; the struct itself is a kernel argument.
 // G2 entries of centre element e with neighbour element f live at
 // g2_tab[e*kMaxEl+f] = (start, count); G4 entries of centre e with the
@@ -62,6 +70,11 @@ struct DevModel {
     const int *zlut;             // Z (0..127) -> element rank; -1 absent, -2-rank unusable
     int fallback_el;             // usable element given to flagged atoms (results discarded)
     int cutoff_type;             // RuNNer cutoff_type 0..8 (2: tanh^3, the reference's)
+    // G5 (RuNNer type 9) functions of centre element e: g5[g5_start[e] ..
+    // + g5_count[e]); g5_total = 0 when the model has none
+    const SfParam *g5;
+    int g5_start[kMaxEl], g5_count[kMaxEl];
+    int g5_total;
 };
 
 // Cell grid of one structure of an evaluation (host-computed, reference
@@ -184,19 +197,19 @@ template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, const R *dEdG, R *gent,
                     hipStream_t s);
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad);
-// One symmetry function of the per-function path, in the sorted order of its
-// element's list: type 2 (G2, neighbour element rank f) or 3 (G4, element
-// pair {f, g}); pre = 2^(1 - zeta).
-struct SfParam {
-    int type, f, g, pad;
-    double eta, rs, lambda, zeta, pre;
-};
 // The per-function path (symmetryFunction2 / symmetryFunction3 of the
 // reference): for every atom and symmetry function, the centred / scaled
 // value G[i * ldG + s] and, when dsf is not null, its gradient rows
 // dsf[dsf_off[i] + 3 (s (count_i + 1) + slot)], slot 0 the atom, slot 1 + n
 // its list entry n (derivatives with respect to those positions, scaled).
 // par[par_off[e] ..] lists element e's functions.
+// G5 (RuNNer type 9, no fc(r_jk)): values into G (scaled, row stride
+// m.nsfmax), and the gradients of sum_s dE/dG_s G_s added to the per-entry
+// rows gent and the atoms' virials (after launch_sf_grad)
+template <typename R>
+void launch_g5_values(const DevModel &m, const Geo &g, const Work &w, R *G, hipStream_t s);
+template <typename R>
+void launch_g5_grad(const DevModel &m, const Geo &g, const Work &w, const R *dEdG, R *gent, hipStream_t s);
 template <typename R>
 void launch_sf_single(const DevModel &m, const Geo &g, const Work &w, const SfParam *par, const int *par_off,
                       const long long *dsf_off, int ldG, double *G, double *dsf, hipStream_t s);

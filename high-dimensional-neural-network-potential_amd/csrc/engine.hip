@@ -422,6 +422,29 @@ void Engine::init_device()
     d.g4_ipow = upload(model_mem_, g4ipow);
     d.g4_sq = upload(model_mem_, g4sq);
     d.g4_neweta = upload(model_mem_, g4new);
+    {   // G5 (type 9) functions per centre element
+        std::vector<SfParam> g5;
+        for (int e = 0; e < d.nel; ++e) {
+            d.g5_start[e] = (int)g5.size();
+            for (int s = 0; s < d.nsf[e]; ++s) {
+                const SymFn &sf = m.el[e].sfs[s];
+                if (sf.type != 9) continue;
+                SfParam p{};
+                p.type = 9;
+                p.f = m.rank_of_Z(sf.e1);
+                p.g = m.rank_of_Z(sf.e2);
+                p.s = s;
+                p.eta = sf.eta;
+                p.lambda = sf.lambda;
+                p.zeta = sf.zeta;
+                p.pre = std::pow(2.0, 1.0 - sf.zeta);
+                g5.push_back(p);
+            }
+            d.g5_count[e] = (int)g5.size() - d.g5_start[e];
+        }
+        d.g5_total = (int)g5.size();
+        d.g5 = upload(model_mem_, g5);
+    }
     d.nlayers = m.el.empty() ? 0 : (int)m.el[0].layers.size();
     mlp_maxw_ = d.nsfmax;
     mlp_sumc_ = 0;
@@ -512,6 +535,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     }
     if (!skip_values_) {
         launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
+        if (dm_.g5_total) launch_g5_values<R>(dm_, g, w, G, s);
         HIPCHK(hipGetLastError());
         trip_valid_ = w.trip != nullptr;
     } else if (!trip_valid_) {   // G came from elsewhere: pass B compacts its own triplets
@@ -535,6 +559,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     if (timed) HIPCHK(hipEventRecord(ev_[4], s));
 
     launch_sf_grad<R>(dm_, g, w, w.lim, ga, dEdG, gent, s);
+    if (dm_.g5_total) launch_g5_grad<R>(dm_, g, w, dEdG, gent, s);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[5], s));
 
@@ -1406,10 +1431,13 @@ void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
             w.trip = trip_.as<int>();
             w.tcount = tcount_.as<int>();
         }
-        if (fp64)
+        if (fp64) {
             launch_sf_values<double>(dm_, g, w, w.lim, ga, G_.as<double>(), s);
-        else
+            if (dm_.g5_total) launch_g5_values<double>(dm_, g, w, G_.as<double>(), s);
+        } else {
             launch_sf_values<float>(dm_, g, w, w.lim, ga, G_.as<float>(), s);
+            if (dm_.g5_total) launch_g5_values<float>(dm_, g, w, G_.as<float>(), s);
+        }
         HIPCHK(hipGetLastError());
         trip_valid_ = w.trip != nullptr;
         std::vector<double> tmp((size_t)nat * dm_.nsfmax);
@@ -1433,13 +1461,14 @@ void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
             for (const SymFn &f : model_.el[e].sfs) {
                 SfParam p{};
                 p.type = f.type;
+                p.s = (int)par.size() - off[e];
                 p.f = model_.rank_of_Z(f.e1);
-                p.g = f.type == 3 ? model_.rank_of_Z(f.e2) : -1;
+                p.g = f.type != 2 ? model_.rank_of_Z(f.e2) : -1;
                 p.eta = f.eta;
                 p.rs = f.type == 2 ? f.rs : 0.0;
-                p.lambda = f.type == 3 ? f.lambda : 0.0;
-                p.zeta = f.type == 3 ? f.zeta : 1.0;
-                p.pre = f.type == 3 ? std::pow(2.0, 1.0 - f.zeta) : 1.0;
+                p.lambda = f.type != 2 ? f.lambda : 0.0;
+                p.zeta = f.type != 2 ? f.zeta : 1.0;
+                p.pre = f.type != 2 ? std::pow(2.0, 1.0 - f.zeta) : 1.0;
                 par.push_back(p);
             }
         }

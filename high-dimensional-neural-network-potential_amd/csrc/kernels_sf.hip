@@ -1252,6 +1252,15 @@ int sf_bucket(int nmax)
     return b;
 }
 
+// base^y for the angular terms with base >= 0: 0^0 = 1 and 0^y = 0 otherwise
+// (the FP32 pow, exp2(y log2 x), gives NaN for 0^0; opposite images of an
+// atom in a cell smaller than the cutoff make cos theta = -1 exactly)
+template <typename R>
+__device__ __forceinline__ R powz(R base, R y)
+{
+    return base > R(0) ? M<R>::pow(base, y) : (y == R(0) ? R(1) : R(0));
+}
+
 // ---------------------------------------------------------------- per-function path
 // symmetryFunction2 / symmetryFunction3 (symmetryFunctions.f90:41-152), which
 // hdnnpCalcSymfunctions runs with calcSfsTogether = .false.
@@ -1315,6 +1324,31 @@ __global__ __launch_bounds__(64) void k_sf_single(DevModel m, Geo g, Work w, con
                     gy = k * dy[q];
                     gz = k * dz[q];
                 }
+            } else if (p.type == 9) {
+                if ((fq == p.f || fq == p.g) && rq < rc) {
+                    // G5: no fc(r_jk), no r_jk^2 in the exponential
+                    const int want = fq == p.f ? p.g : p.f;
+                    const R ax = dx[q], ay = dy[q], az = dz[q], fij = fcs[q], dfij = dfs[q];
+                    for (int k = 0; k < n; ++k) {
+                        if (k == q || el[k] != want || !(rr[k] < rc)) continue;
+                        const R bx = dx[k], by = dy[k], bz = dz[k], rk = rr[k], fk = fcs[k];
+                        const R ab = ax * bx + ay * by + az * bz;
+                        const R inv = R(1) / (rq * rk);
+                        const R cost = ab * inv;
+                        R base = R(1) + lam * cost;
+                        if (base < R(0)) base = R(0);
+                        const R P = powz<R>(base, zeta), dP = zeta * lam * powz<R>(base, zeta - R(1));
+                        const R E = M<R>::exp(-eta * (rq * rq + rk * rk));
+                        const R F = fij * fk;
+                        val += pre * P * E * F;
+                        const R a3 = cost / (rq * rq);
+                        const R cq = P * E * (-fk * dfij) + P * R(2) * eta * E * F + dP * a3 * E * F;
+                        const R ck = -dP * inv * E * F;
+                        gx += pre * (cq * ax + ck * bx);
+                        gy += pre * (cq * ay + ck * by);
+                        gz += pre * (cq * az + ck * bz);
+                    }
+                }
             } else if ((fq == p.f || fq == p.g) && rq < rc) {
                 const int want = fq == p.f ? p.g : p.f;   // the partner's element
                 const R ax = dx[q], ay = dy[q], az = dz[q], fij = fcs[q], dfij = dfs[q];
@@ -1333,8 +1367,8 @@ __global__ __launch_bounds__(64) void k_sf_single(DevModel m, Geo g, Work w, con
                     const R cost = ab * inv;
                     R base = R(1) + lam * cost;
                     if (base < R(0)) base = R(0);
-                    const R P = M<R>::pow(base, zeta);
-                    const R dP = zeta * lam * M<R>::pow(base, zeta - R(1));
+                    const R P = powz<R>(base, zeta);
+                    const R dP = zeta * lam * powz<R>(base, zeta - R(1));
                     const R E = M<R>::exp(-eta * (rq * rq + rk * rk + rjk2));
                     const R F = fij * fcs[k] * fjk;
                     val += pre * P * E * F;
@@ -1368,7 +1402,7 @@ __global__ __launch_bounds__(64) void k_sf_single(DevModel m, Geo g, Work w, con
         sy = wave_total(sy);
         sz = wave_total(sz);
         if (lane == 0) {
-            const double G = p.type == 2 ? (double)val : 0.5 * (double)val;
+            const double G = p.type == 2 ? (double)val : 0.5 * (double)val;   // pairs seen from both ends
             Gout[(size_t)i * ldG + s] = (G - c) / den;
             if (dsf) {
                 const long long o = row0 + 3 * ((long long)s * (n + 1));
@@ -1379,6 +1413,231 @@ __global__ __launch_bounds__(64) void k_sf_single(DevModel m, Geo g, Work w, con
         }
     }
 }
+
+// ---------------------------------------------------------------- G5
+// RuNNer's wide angular function (type 9): G5 = 2^(1-zeta) sum_{j<k}
+// (1 + lambda cos theta_jik)^zeta exp(-eta (r_ij^2 + r_ik^2)) fc(r_ij) fc(r_ik),
+// with no cutoff of r_jk, so the compacted triplet lists of the G4 passes
+// (r_jk < rc) do not serve it. One wavefront per atom; the neighbours within
+// the cutoff are staged in LDS grouped by element, in list order within an
+// element (ballot ranks, so the order never depends on timing), as
+// d = R_i - R_n, r, fc, fc'/r. For a function of element pair {f, g} the
+// lanes take the entries q of group f and sum over the partners k of group g
+// (k != q): each unordered pair once for f != g, twice for f = g.
+template <typename R>
+struct G5Lds {
+    R *dx, *dy, *dz, *r, *fc, *dfc;
+    unsigned char *el;
+    unsigned short *ent;
+    int seg[kMaxEl + 1];
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ size_t g5_lds_bytes(int lim)
+{
+    return (size_t)lim * (6 * sizeof(R) + 3) + 16;
+}
+
+// stage atom i's entries within rc; returns how many
+template <typename R>
+__device__ __forceinline__ int g5_stage(const DevModel &m, const Geo &g, const Work &w, int i, R rc, char *base,
+                                        G5Lds<R> &L)
+{
+    const int lane = threadIdx.x, n = w.count[i], lim = w.lim;
+    L.dx = (R *)base;
+    L.dy = L.dx + lim;
+    L.dz = L.dy + lim;
+    L.r = L.dz + lim;
+    L.fc = L.r + lim;
+    L.dfc = L.fc + lim;
+    L.el = (unsigned char *)(L.dfc + lim);
+    L.ent = (unsigned short *)(((size_t)(L.el + lim) + 1) & ~(size_t)1);
+    const double4 pi = w.wpos[i];
+    const Lat9 lt = atom_lattice(g, w, i);
+    const R rrc = R(1) / rc;
+    const unsigned long long below = (1ull << lane) - 1;
+    int cnt[kMaxEl];
+#pragma unroll
+    for (int f = 0; f < kMaxEl; ++f) cnt[f] = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        int run[kMaxEl];
+        if (pass == 1) {   // group starts
+            int o = 0;
+#pragma unroll
+            for (int f = 0; f < kMaxEl; ++f) {
+                L.seg[f] = o;
+                run[f] = o;
+                o += cnt[f];
+            }
+            L.seg[kMaxEl] = o;
+        }
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int q = c0 + lane;
+            int f = -1;
+            double d0 = 0, d1 = 0, d2 = 0, r = 0;
+            if (q < n) {
+                int j;
+                entry_geometry(lt, w, pi, i * w.cap + q, d0, d1, d2, r, j);
+                if ((R)r < rc) f = w.eidx[j];
+            }
+#pragma unroll
+            for (int h = 0; h < kMaxEl; ++h) {
+                if (h >= m.nel) break;
+                const unsigned long long mk = __ballot(f == h);
+                if (pass == 1 && f == h) {
+                    const int at = run[h] + __popcll(mk & below);
+                    R fc, dfc;
+                    cutoff_in<R>(m.cutoff_type, (R)r, rrc, fc, dfc);
+                    L.dx[at] = (R)d0;
+                    L.dy[at] = (R)d1;
+                    L.dz[at] = (R)d2;
+                    L.r[at] = (R)r;
+                    L.fc[at] = fc;
+                    L.dfc[at] = dfc;
+                    L.el[at] = (unsigned char)h;
+                    L.ent[at] = (unsigned short)q;
+                }
+                if (pass == 0) cnt[h] += __popcll(mk);
+                else run[h] += __popcll(mk);
+            }
+        }
+    }
+    __syncthreads();
+    return L.seg[kMaxEl];
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_g5_values(DevModel m, Geo g, Work w, R *G)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int i = blockIdx.x, lane = threadIdx.x, e = w.eidx[i];
+    const int t0 = m.g5_start[e], nt = m.g5_count[e];
+    if (nt == 0) return;
+    const R rc = (R)m.rc[e];
+    __shared__ G5Lds<R> L;
+    G5Lds<R> Lr;
+    g5_stage<R>(m, g, w, i, rc, smem, Lr);
+    if (lane == 0) L = Lr;
+    __syncthreads();
+    const G5Lds<R> &S = L;
+    for (int t = 0; t < nt; ++t) {
+        const SfParam p = m.g5[t0 + t];
+        const R eta = (R)p.eta, lam = (R)p.lambda, zeta = (R)p.zeta, pre = (R)p.pre;
+        const int fa = S.seg[p.f], fb = S.seg[p.f + 1], ga = S.seg[p.g], gb = S.seg[p.g + 1];
+        R val = 0;
+        for (int q = fa + lane; q < fb; q += 64) {
+            const R ax = S.dx[q], ay = S.dy[q], az = S.dz[q], rq = S.r[q], fq = S.fc[q];
+            for (int k = ga; k < gb; ++k) {
+                if (k == q) continue;
+                const R rk = S.r[k];
+                const R cost = (ax * S.dx[k] + ay * S.dy[k] + az * S.dz[k]) / (rq * rk);
+                R base = R(1) + lam * cost;
+                if (base < R(0)) base = R(0);
+                val += powz<R>(base, zeta) * M<R>::exp(-eta * (rq * rq + rk * rk)) * fq * S.fc[k];
+            }
+        }
+        val = wave_total(val);
+        if (lane == 0) {
+            const int s = m.sf_off[e] + p.s;
+            const double v = (double)pre * (p.f == p.g ? 0.5 * (double)val : (double)val);
+            G[(size_t)i * m.nsfmax + p.s] = (R)((v - m.sf_mean[s]) / m.sf_den[s]);
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_g5_grad(DevModel m, Geo g, Work w, const R *dEdG, R *gent)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int i = blockIdx.x, lane = threadIdx.x, e = w.eidx[i];
+    const int t0 = m.g5_start[e], nt = m.g5_count[e];
+    if (nt == 0) return;
+    const R rc = (R)m.rc[e];
+    __shared__ G5Lds<R> L;
+    G5Lds<R> Lr;
+    const int n = g5_stage<R>(m, g, w, i, rc, smem, Lr);
+    if (lane == 0) L = Lr;
+    double *gx = (double *)(smem + ((g5_lds_bytes<R>(w.lim) + 15) / 16 * 16)), *gy = gx + w.lim, *gz = gy + w.lim;
+    for (int q = lane; q < n; q += 64) gx[q] = gy[q] = gz[q] = 0.0;
+    __syncthreads();
+    const G5Lds<R> &S = L;
+    for (int t = 0; t < nt; ++t) {
+        const SfParam p = m.g5[t0 + t];
+        const int s = m.sf_off[e] + p.s;
+        const R wgt = (R)((double)dEdG[(size_t)i * m.nsfmax + p.s] / m.sf_den[s]);
+        const R eta = (R)p.eta, lam = (R)p.lambda, zeta = (R)p.zeta, pre = (R)p.pre;
+        // lanes over group f with partners in g, then (f != g) over g with partners in f
+        for (int side = 0; side < (p.f == p.g ? 1 : 2); ++side) {
+            const int own = side == 0 ? p.f : p.g, oth = side == 0 ? p.g : p.f;
+            const int fa = S.seg[own], fb = S.seg[own + 1], ga = S.seg[oth], gb = S.seg[oth + 1];
+            for (int q = fa + lane; q < fb; q += 64) {
+                const R ax = S.dx[q], ay = S.dy[q], az = S.dz[q], rq = S.r[q], fq = S.fc[q], dfq = S.dfc[q];
+                R sx = 0, sy = 0, sz = 0;
+                for (int k = ga; k < gb; ++k) {
+                    if (k == q) continue;
+                    const R bx = S.dx[k], by = S.dy[k], bz = S.dz[k], rk = S.r[k], fk = S.fc[k];
+                    const R ab = ax * bx + ay * by + az * bz;
+                    const R inv = R(1) / (rq * rk);
+                    const R cost = ab * inv;
+                    R base = R(1) + lam * cost;
+                    if (base < R(0)) base = R(0);
+                    const R P = powz<R>(base, zeta), dP = zeta * lam * powz<R>(base, zeta - R(1));
+                    const R E = M<R>::exp(-eta * (rq * rq + rk * rk));
+                    const R F = fq * fk;
+                    // d cos/dR_q = -d_k/(r_q r_k) + d_q cos / r_q^2; dF/dR_q = -fc_k fc'_q/r_q d_q;
+                    // dE/dR_q = 2 eta E d_q (d = R_i - R_n)
+                    const R a3 = cost / (rq * rq);
+                    const R cq = P * E * (-fk * dfq) + P * R(2) * eta * E * F + dP * a3 * E * F;
+                    const R ck = -dP * inv * E * F;
+                    sx += cq * ax + ck * bx;
+                    sy += cq * ay + ck * by;
+                    sz += cq * az + ck * bz;
+                }
+                gx[q] += (double)(wgt * pre * sx);
+                gy[q] += (double)(wgt * pre * sy);
+                gz[q] += (double)(wgt * pre * sz);
+            }
+            __syncthreads();
+        }
+    }
+    // the rows of the entries, and the atom's virial -sum_n g_n (x) d_in
+    double v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) v[t] = 0;
+    for (int q = lane; q < n; q += 64) {
+        const size_t en = (size_t)i * w.cap + S.ent[q];
+        gent[3 * en + 0] += (R)gx[q];
+        gent[3 * en + 1] += (R)gy[q];
+        gent[3 * en + 2] += (R)gz[q];
+        const double dxq = S.dx[q], dyq = S.dy[q], dzq = S.dz[q];
+        v[0] -= gx[q] * dxq; v[1] -= gx[q] * dyq; v[2] -= gx[q] * dzq;
+        v[3] -= gy[q] * dxq; v[4] -= gy[q] * dyq; v[5] -= gy[q] * dzq;
+        v[6] -= gz[q] * dxq; v[7] -= gz[q] * dyq; v[8] -= gz[q] * dzq;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const double sv = wave_total(v[t]);
+        if (lane == 0) w.vir[9 * (size_t)i + t] += sv;
+    }
+}
+
+template <typename R>
+void launch_g5_values(const DevModel &m, const Geo &g, const Work &w, R *G, hipStream_t s)
+{
+    if (w.nat == 0 || m.g5_total == 0) return;
+    hipLaunchKernelGGL(k_g5_values<R>, dim3(w.nat), dim3(64), g5_lds_bytes<R>(w.lim), s, m, g, w, G);
+}
+template <typename R>
+void launch_g5_grad(const DevModel &m, const Geo &g, const Work &w, const R *dEdG, R *gent, hipStream_t s)
+{
+    if (w.nat == 0 || m.g5_total == 0) return;
+    const size_t lds = (g5_lds_bytes<R>(w.lim) + 15) / 16 * 16 + 24 * (size_t)w.lim;
+    hipLaunchKernelGGL(k_g5_grad<R>, dim3(w.nat), dim3(64), lds, s, m, g, w, dEdG, gent);
+}
+template void launch_g5_values<double>(const DevModel &, const Geo &, const Work &, double *, hipStream_t);
+template void launch_g5_values<float>(const DevModel &, const Geo &, const Work &, float *, hipStream_t);
+template void launch_g5_grad<double>(const DevModel &, const Geo &, const Work &, const double *, double *, hipStream_t);
+template void launch_g5_grad<float>(const DevModel &, const Geo &, const Work &, const float *, float *, hipStream_t);
 
 template <typename R>
 void launch_sf_single(const DevModel &m, const Geo &g, const Work &w, const SfParam *par, const int *par_off,

@@ -90,8 +90,8 @@ static std::string sf_key(const SymFn &s)
     std::string k;
     if (s.type == 2)
         k = fmt_i(2, 2) + fmt_f(s.rcut, 9, 5) + fmt_f(s.eta, 9, 5) + fmt_f(s.rs, 9, 5) + fmt_i(s.e1, 3);
-    else
-        k = fmt_i(3, 2) + fmt_f(s.rcut, 9, 5) + fmt_f(s.eta, 9, 5) + fmt_f(s.zeta, 9, 5) +
+    else   // types 3 and 9 (RuNNer's wide angular function, in type 3's form)
+        k = fmt_i(s.type, 2) + fmt_f(s.rcut, 9, 5) + fmt_f(s.eta, 9, 5) + fmt_f(s.zeta, 9, 5) +
             fmt_f(s.lambda + 10, 9, 5) + fmt_i(s.e1, 3) + fmt_i(s.e2, 3);
     k.resize(256, ' ');
     return k;
@@ -240,9 +240,9 @@ HostModel read_runner(const std::string &dir_in)
                 s.eta = read_real(w[4], "symfunction_short");
                 s.rs = read_real(w[5], "symfunction_short");
                 s.rcut = read_real(w[6], "symfunction_short");
-            } else if (name == 3) {
+            } else if (name == 3 || name == 9) {   // 9: RuNNer's wide angular G5 (the reference reads 2 and 3)
                 need(9);
-                s.type = 3;
+                s.type = name;
                 s.e1 = elem_sym_to_num(w[3].substr(0, 2));
                 s.e2 = elem_sym_to_num(w[4].substr(0, 2));
                 s.eta = read_real(w[5], "symfunction_short");
@@ -252,12 +252,12 @@ HostModel read_runner(const std::string &dir_in)
             } else {
                 // the reference skips other types silently, and its weights
                 // would then be read with the wrong layout: refused here
-                throw Error(3, "symmetry-function type " + w[2] + " is not supported (types 2 and 3 are)");
+                throw Error(3, "symmetry-function type " + w[2] + " is not supported (types 2, 3 and 9 are)");
             }
             if (s.rcut > maxrc) maxrc = s.rcut;
             if (centre < 0 || !allocated[centre])
                 throw Error(2, "symmetry function for element '" + w[1] + "' that is not in 'elements'");
-            if (s.e1 < 0 || (s.type == 3 && s.e2 < 0))
+            if (s.e1 < 0 || (s.type != 2 && s.e2 < 0))
                 throw Error(2, "unknown neighbour element in: " + line);
             if ((int)lists[centre].size() + 1 > kMaxSF)
                 throw Error(2, "Error: more symmetry functions in input file than nmaxSFs allows");

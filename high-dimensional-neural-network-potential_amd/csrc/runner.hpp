@@ -18,7 +18,7 @@ struct Error : std::runtime_error {
 };
 
 struct SymFn {
-    int type = 0;          // 2 (G2, radial) or 3 (G4, angular)
+    int type = 0;          // 2 (G2, radial), 3 (G4, angular) or 9 (G5, wide angular: no fc(r_jk))
     int e1 = 0, e2 = 0;    // neighbour element(s) as Z, as written in input.nn
     double rcut = 0, eta = 0, rs = 0, lambda = 0, zeta = 0;
     double sfmin = 0, sfmax = 0, sfmean = 0;

@@ -411,8 +411,8 @@ static int calc_all(const orc_model *m, int e, int nn, const double *P, const in
     const int nsf = m->nsf[e], o = m->sf_off[e];
     int sf3start = -1;
     const double rcut = m->sf_rcut[o];
-    for (int s = 0; s < nsf; ++s) {                   /* :183-199 */
-        if (m->sf_type[o + s] == 3 && sf3start < 0) sf3start = s;
+    for (int s = 0; s < nsf; ++s) {                   /* :183-199 (type 9 sorts after 3) */
+        if (m->sf_type[o + s] != 2 && sf3start < 0) sf3start = s;
         else if (m->sf_type[o + s] == 2 && sf3start >= 0) return 3;   /* not sorted */
         const double dr = m->sf_rcut[o + s] - rcut;
         if (dr * dr > 1.e-10) return 4;                                /* mixed rc */
@@ -467,6 +467,35 @@ static int calc_all(const orc_model *m, int e, int nn, const double *P, const in
             for (int s = sf3start; s < nsf; ++s) {
                 const int e1 = m->sf_e1[o + s], e2 = m->sf_e2[o + s];
                 if (!((els[j] == e1 && els[kk] == e2) || (els[j] == e2 && els[kk] == e1))) continue;
+                if (m->sf_type[o + s] == 9) {
+                    /* RuNNer's wide angular function G5, which the reference does
+                     * not have: 2^(1-zeta) (1 + lambda cos)^zeta
+                     * exp(-eta (r_ij^2 + r_ik^2)) fc(r_ij) fc(r_ik), restated from
+                     * that definition with the G4 terms above: no fc(r_jk), no
+                     * r_jk^2 in the exponential */
+                    const double zeta = m->sf_zeta[o + s], lam = m->sf_lambda[o + s];
+                    const double eta = m->sf_eta[o + s];
+                    const double b = 1.0 + lam * cost;
+                    const double pre = pow(2.0, 1.0 - zeta);
+                    const double tp = b > 0.0 ? pow(b, zeta - 1.0) : 0.0, cosp = tp * b;
+                    const double ex5 = exp(-(eta * (rij * rij + rik * rik)));
+                    const double fc5 = fcij * fcik;
+                    double *dj = dsf + ((size_t)s * nn + j) * 3, *dk = dsf + ((size_t)s * nn + kk) * 3;
+                    double *di = dsf + (size_t)s * nn * 3;
+                    for (int k = 0; k < 3; ++k) {
+                        const double dfj5 = -(fcik * dfcij * dij[k]), dfk5 = -(fcij * dfcik * dik[k]);
+                        const double dej5 = ex5 * eta * 2.0 * dij[k], dek5 = ex5 * eta * 2.0 * dik[k];
+                        const double gj = pre * (cosp * ex5 * dfj5 + cosp * dej5 * fc5 +
+                                                 tp * lam * zeta * dcj[k] * ex5 * fc5);
+                        const double gk = pre * (cosp * ex5 * dfk5 + cosp * dek5 * fc5 +
+                                                 tp * lam * zeta * dck[k] * ex5 * fc5);
+                        dj[k] = dj[k] + gj;
+                        dk[k] = dk[k] + gk;
+                        di[k] = di[k] - (gj + gk);
+                    }
+                    sf[s] = sf[s] + pre * cosp * ex5 * fc5;
+                    continue;
+                }
                 const double zeta = m->sf_zeta[o + s], lam = m->sf_lambda[o + s];
                 const double eta = m->sf_eta[o + s];
                 const int iz = (int)zeta;

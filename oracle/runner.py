@@ -62,7 +62,7 @@ def _fmt_i(n: int, w: int) -> str:
 
 @dataclass
 class SymFunction:
-    name: int                 # 2 or 3
+    name: int                 # 2, 3, or 9 (RuNNer's wide angular G5, beyond the reference)
     els: tuple                # (Z1,) or (Z1, Z2) as written in the file
     rcut: float
     eta: float
@@ -78,8 +78,8 @@ class SymFunction:
         if self.name == 2:
             k = (_fmt_i(2, 2) + _fmt_f(self.rcut, 9, 5) + _fmt_f(self.eta, 9, 5)
                  + _fmt_f(self.rs, 9, 5) + _fmt_i(self.els[0], 3))
-        else:
-            k = (_fmt_i(3, 2) + _fmt_f(self.rcut, 9, 5) + _fmt_f(self.eta, 9, 5)
+        else:                 # type 3, and 9 in type 3's form
+            k = (_fmt_i(self.name, 2) + _fmt_f(self.rcut, 9, 5) + _fmt_f(self.eta, 9, 5)
                  + _fmt_f(self.zeta, 9, 5) + _fmt_f(self.lam + 10, 9, 5)
                  + _fmt_i(self.els[0], 3) + _fmt_i(self.els[1], 3))
         return k.ljust(256)
@@ -168,8 +168,8 @@ def read_potential(path: str) -> Potential:
             if name == 2:
                 sf = SymFunction(2, (elem_sym_to_num(w[3][:2]),), rcut=_freal(w[6]),
                                  eta=_freal(w[4]), rs=_freal(w[5]))
-            elif name == 3:
-                sf = SymFunction(3, (elem_sym_to_num(w[3][:2]), elem_sym_to_num(w[4][:2])),
+            elif name in (3, 9):
+                sf = SymFunction(name, (elem_sym_to_num(w[3][:2]), elem_sym_to_num(w[4][:2])),
                                  rcut=_freal(w[8]), eta=_freal(w[5]), lam=_freal(w[6]),
                                  zeta=_freal(w[7]))
             else:
@@ -315,7 +315,7 @@ class Oracle:
             m.atom_energy[e] = pot.atom_energy.get(z, 0.0)
         m.sf_type = _ptr(arr([s.name for s in allsf], np.int32), _ip)
         m.sf_e1 = _ptr(arr([s.els[0] for s in allsf], np.int32), _ip)
-        m.sf_e2 = _ptr(arr([s.els[1] if s.name == 3 else 0 for s in allsf], np.int32), _ip)
+        m.sf_e2 = _ptr(arr([s.els[1] if s.name != 2 else 0 for s in allsf], np.int32), _ip)
         for name, attr in (("sf_rcut", "rcut"), ("sf_eta", "eta"), ("sf_rs", "rs"),
                            ("sf_lambda", "lam"), ("sf_zeta", "zeta"), ("sf_min", "sfmin"),
                            ("sf_max", "sfmax"), ("sf_mean", "sfmean")):

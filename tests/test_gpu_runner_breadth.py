@@ -37,3 +37,56 @@ def test_cutoff_types_against_oracle(pkg, wl, oracle, tmp_path, t):
 @pytest.mark.parametrize("center", [False, True])
 def test_scale_range_against_oracle(pkg, wl, oracle, tmp_path, center):
     _compare(pkg, oracle, wl, potential_copy(tmp_path, center=center, scale_range=(-1.0, 1.0)))
+
+
+def test_g5_against_oracle(pkg, wl, oracle, tmp_path):
+    """Symmetry-function type 9 (G5, no fc(r_jk)) on the GPU against the
+    oracle (pinned by tests/test_runner_breadth.py), FP64 and mixed."""
+    from test_runner_breadth import g5_potential
+    _compare(pkg, oracle, wl, g5_potential(tmp_path, oracle, wl))
+
+
+def test_g5_batch_equals_single(pkg, wl, oracle, tmp_path):
+    from test_runner_breadth import g5_potential
+    d = g5_potential(tmp_path, oracle, wl)
+    ss = [wl.perovskite(2, 1, 1, seed=5), wl.cluster(30, seed=6), wl.perovskite(1, 2, 2, seed=7)]
+    for prec in ("fp64", "mixed"):
+        h = pkg.parseRuNNerInput(d, prec)
+        E, F, L = h.energy_and_forces_batch(ss)
+        for k, s in enumerate(ss):
+            e, f, l = h.energy_and_forces(s)
+            assert E[k] == e and np.array_equal(F[k], f), (prec, k)
+
+
+def test_g5_per_function_path(pkg, wl, oracle, tmp_path):
+    """hdnnpCalcSymfunctions with calcSfsTogether = .false. takes G5 through
+    the per-function kernel: its values equal the fused G5 kernel's, and its
+    dG/dR rows the oracle's G under finite differences."""
+    from test_runner_breadth import g5_potential
+    d = g5_potential(tmp_path, oracle, wl)
+    s = wl.cluster(16, seed=61)
+    h = pkg.parseRuNNerInput(d, "fp64")
+    nl = pkg.hdnnpCalcNeighborLists(h, s)
+    fused = pkg.hdnnpCalcSymfunctions(h, s, nl, True)
+    single = pkg.hdnnpCalcSymfunctions(h, s, nl, False)
+    for a, b in zip(fused, single):
+        np.testing.assert_allclose(a.sf, b.sf, rtol=0, atol=1e-11)
+    o = oracle.Oracle(oracle.read_potential(d))
+    hh = 1e-5
+    for i in (0, 3):                             # atoms with 9 and 10 neighbours
+        dsf = single[i].dsf                      # [s, slot, 3]
+        for slot in (1, 4):
+            j = int(nl[i].ineis[slot])
+            fd = np.zeros((dsf.shape[0], 3))
+            for c in range(3):
+                sp, sm = s.copy(), s.copy()
+                sp.pos = s.pos.copy()
+                sm.pos = s.pos.copy()
+                sp.pos[j, c] += hh
+                sm.pos[j, c] -= hh
+                fd[:, c] = (o.evaluate(sp, want_sf=True)["G"][i, :dsf.shape[0]] -
+                            o.evaluate(sm, want_sf=True)["G"][i, :dsf.shape[0]]) / (2 * hh)
+            np.testing.assert_allclose(dsf[:, slot, :], fd, rtol=0, atol=1e-6 * max(1.0, np.abs(fd).max()))
+    E0, F0, _ = h.energy_and_forces(s)
+    E1, F1, _ = pkg.hdnnpCalcNNs(h, s, nl, single)
+    assert abs(E1 - E0) <= 1e-10 * abs(E0) and np.abs(F1 - F0).max() <= 1e-9

@@ -1,9 +1,12 @@
-"""RuNNer keywords beyond the reference's (cutoff_type 0..8 and the
-scale_min_short / scale_max_short min-max scaling), in the oracle and the
-parser. The reference evaluates none of them (it always uses cutoff type 2
-and ignores the scaling range), so these are pinned by the mathematics:
-closed forms at special points, G2 of two atoms, F = -grad E, and the
-extrema of a scaling set landing on [scale_min, scale_max]. No GPU."""
+"""RuNNer keywords beyond the reference's (cutoff_type 0..8, the
+scale_min_short / scale_max_short min-max scaling, and symmetry-function
+type 9, the wide angular G5), in the oracle and the parser. The reference
+evaluates none of them (it always uses cutoff type 2, ignores the scaling
+range and skips other function types), so these are pinned by the
+mathematics: closed forms at special points, G2 of two atoms, G5 of three
+atoms (also where r_jk lies beyond the cutoff, which G4 cuts and G5 does
+not), F = -grad E, and the extrema of a scaling set landing on
+[scale_min, scale_max]. No GPU."""
 from __future__ import annotations
 
 import os
@@ -202,12 +205,12 @@ def test_parser_reads_runner_keywords(pkg, oracle, tmp_path):
     assert (p.cutoff_type, p.scale_range, p.scale_min, p.scale_max) == (6, True, -0.5, 0.5)
     ref = pkg.parseRuNNerInput(PEROV_PARAMS)
     assert (ref.info.cutoff_type, ref.info.scale_range) == (2, 0)
-    e = os.path.join(str(tmp_path), "g5")
+    e = os.path.join(str(tmp_path), "g6")
     shutil.copytree(PEROV_PARAMS, e, copy_function=shutil.copyfile)
-    open(os.path.join(e, "input.nn"), "a").write("symfunction_short H 9 H H 0.01 1.0 1.0 12.0\n")
+    open(os.path.join(e, "input.nn"), "a").write("symfunction_short H 6 H 0.01 1.0 12.0\n")
     with pytest.raises(pkg.HdnnpError) as ei:          # the reference would skip it silently
         pkg.parseRuNNerInput(e + "/")
-    assert ei.value.code == 3 and "type 9" in str(ei.value)
+    assert ei.value.code == 3 and "type 6" in str(ei.value)
     for bad in ("cutoff_type 9", "cutoff_type 2 0.1", "scale_min_short 1.0\nscale_max_short 0.5"):
         e = os.path.join(str(tmp_path), "bad" + str(abs(hash(bad))))
         shutil.copytree(PEROV_PARAMS, e, copy_function=shutil.copyfile)
@@ -217,3 +220,93 @@ def test_parser_reads_runner_keywords(pkg, oracle, tmp_path):
         assert ei.value.code == 3
         with pytest.raises(oracle.RunnerError):
             oracle.read_potential(e + "/")
+
+
+def g5_potential(tmp, oracle, wl, scaled=True):
+    """The perovskite potential with its 20 G4 functions of lambda = +1 made
+    G5 (type 9, same parameters; they sort after the G4 ones), its scaling
+    recomputed by the oracle for the new functions (or none: scaled=False)."""
+    d = os.path.join(str(tmp), f"pot_g5_{scaled}")
+    shutil.copytree(PEROV_PARAMS, d, copy_function=shutil.copyfile)
+    out = []
+    for ln in open(os.path.join(d, "input.nn")).read().splitlines():
+        w = ln.split()
+        if len(w) == 9 and w[0] == "symfunction_short" and w[2] == "3" and float(w[6]) == 1.0:
+            w[2] = "9"
+            ln = " ".join(w)
+        if not scaled and w and w[0] in ("scale_symmetry_functions", "center_symmetry_functions"):
+            continue
+        out.append(ln)
+    open(os.path.join(d, "input.nn"), "w").write("\n".join(out) + "\n")
+    return rescale(d + "/", oracle, wl) if scaled else d + "/"
+
+
+def _g5_raw(r, theta, eta, lam, zeta, rc=RC):
+    """G5 of a centre with two neighbours at distance r and angle theta:
+    2^(1-zeta) (1 + lambda cos theta)^zeta exp(-2 eta r^2) fc(r)^2."""
+    b = max(0.0, 1.0 + lam * np.cos(theta))
+    return 2.0 ** (1 - zeta) * b ** zeta * np.exp(-2 * eta * r * r) * closed_form(2, r, rc) ** 2
+
+
+@pytest.mark.parametrize("r, deg", [(2.5, 60.0), (3.1, 90.0), (8.0, 100.0), (6.5, 150.0)])
+def test_g5_three_atoms(oracle, wl, tmp_path, r, deg):
+    """An H atom with two H neighbours at distance r and angle theta; at
+    8 bohr and 100 degrees the neighbours are 12.26 bohr apart, beyond the
+    cutoff: the G4 of the same parameters vanish there, G5 does not."""
+    d = g5_potential(tmp_path, oracle, wl, scaled=False)
+    p = oracle.read_potential(d)
+    o = oracle.Oracle(p)
+    th = np.radians(deg)
+    s = wl.Structure(np.array([[0.0, 0, 0], [r, 0, 0], [r * np.cos(th), r * np.sin(th), 0]]),
+                     np.array([1, 1, 1], np.int32))
+    G = o.evaluate(s, want_sf=True)["G"][0]
+    rjk = 2 * r * np.sin(th / 2)
+    n5 = n4 = 0
+    for k, sf in enumerate(p.sfs[1]):
+        if sf.name == 9 and sf.els == (1, 1):
+            assert abs(G[k] - _g5_raw(r, th, sf.eta, sf.lam, sf.zeta)) <= 1e-14 + 1e-12 * abs(G[k]), sf
+            n5 += 1
+        if sf.name == 3 and sf.els == (1, 1) and rjk >= RC:
+            assert G[k] == 0.0
+            n4 += 1
+    assert n5 > 0 and (n4 > 0 or rjk < RC)
+
+
+def test_g5_sorts_after_g4_in_both_parsers(pkg, oracle, wl, tmp_path):
+    d = g5_potential(tmp_path, oracle, wl, scaled=False)
+    h = pkg.parseRuNNerInput(d)
+    p = oracle.read_potential(d)
+    for e, z in enumerate(h.elements):
+        rows = h.sf_table(e)
+        types = rows[:, 0].astype(int)
+        assert list(types) == [sf.name for sf in p.sfs[z]]
+        assert (types == 9).sum() == 20 and types[-20:].tolist() == [9] * 20
+        np.testing.assert_array_equal(rows[:, 4], [sf.eta for sf in p.sfs[z]])
+        np.testing.assert_array_equal(rows[:, 6], [sf.lam if sf.name != 2 else 0.0 for sf in p.sfs[z]])
+
+
+def test_g5_forces_are_minus_gradient(oracle, wl, tmp_path):
+    o = oracle.Oracle(oracle.read_potential(g5_potential(tmp_path, oracle, wl)))
+    for s in (wl.cluster(14, seed=51), wl.perovskite(1, 1, 1, seed=52)):
+        r = o.evaluate(s)
+        h = 1e-5
+        for i, a in ((0, 0), (3, 1), (7, 2), (11, 0)):
+            sp, sm = s.copy(), s.copy()
+            sp.pos = s.pos.copy()
+            sm.pos = s.pos.copy()
+            sp.pos[i, a] += h
+            sm.pos[i, a] -= h
+            fd = -(o.evaluate(sp)["energy"] - o.evaluate(sm)["energy"]) / (2 * h)
+            assert abs(fd - r["force"][i, a]) < 1e-6 * max(1.0, abs(fd)), (i, a)
+        if s.periodic:   # dEdLat under a strain of the lattice at fixed fractional coordinates
+            eps = 1e-6
+            M = np.array([[0.3, 0.1, -0.2], [0.1, -0.4, 0.25], [0.05, 0.2, 0.1]])
+            lat = np.asarray(s.lat).reshape(3, 3)
+            want = float(np.sum(r["dEdLat"] * (lat @ M.T)))
+            vals = []
+            for sg in (1, -1):
+                t = s.copy()
+                t.lat = lat @ (np.eye(3) + sg * eps * M.T)
+                t.pos = s.pos @ (np.eye(3) + sg * eps * M.T)
+                vals.append(o.evaluate(t)["energy"])
+            assert abs((vals[0] - vals[1]) / (2 * eps) - want) < 1e-6 * max(1.0, abs(want))
