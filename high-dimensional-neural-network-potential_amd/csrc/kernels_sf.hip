@@ -35,6 +35,9 @@
 // library's sqrt, pow and cospi.
 #include <rocprim/intrinsics/warp_shuffle.hpp>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "device.hpp"
 #include "fmath.hpp"
 
@@ -317,16 +320,24 @@ struct Lds {
     LDS R *sp1;                   // q3 = w P lambda zeta
     LDS int *smeta;               // zeta = 2^k: k, else 0xFF; | new-eta flag << 8
     LDS I2 *btab;                 // block f*kMaxEl+h (f <= h): (first u, count)
+    // values over all pairs (FP32, one atom per wavefront): the function
+    // sums in FP64, added to with LDS atomics; sp0 = (exp scale, lambda,
+    // 2^(1-zeta), zeta - 1), smeta also holds the function's slot << 16, btab
+    LDS double *accd;
 };
 
 // Everything sized by the staging capacity nmax comes first, so that with
 // nmax a compile-time constant these arrays sit at constant offsets.
 struct LdsOffsets {
-    size_t a, b, el, ent, pairs, seg, g, acc, sp0, sp1, smeta, btab, total;
+    size_t a, b, el, ent, pairs, seg, g, acc, accd, sp0, sp1, smeta, btab, total;
 };
 
+// triplets per compacted chunk of the all-pairs values pass
+constexpr int kAllPairsChunk = 512;
+
 template <typename R>
-__host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, int nel, bool grad)
+__host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax, int g4max, int nel, bool grad,
+                                                          bool allp = false)
 {
     LdsOffsets o{};
     size_t off = 0;
@@ -339,13 +350,14 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
     o.b = take(sizeof(typename Vec4<R>::T) * (size_t)nmax);
     o.el = take((size_t)nmax);
     o.ent = take(sizeof(unsigned short) * (size_t)nmax);
-    o.pairs = take(sizeof(int) * kPairChunk<R>);
+    o.pairs = take(sizeof(int) * (allp ? kAllPairsChunk : kPairChunk<R>));
     o.seg = take(sizeof(int) * (kMaxEl + 1));
     o.g = grad ? take(sizeof(double) * 3 * (size_t)nmax) : 0;
     o.acc = take(sizeof(R) * (size_t)nsfmax);
-    if (grad) {
+    if (allp && sizeof(R) == 4) o.accd = take(sizeof(double) * (size_t)nsfmax);   // FP64: acc
+    if (grad || allp) {
         o.sp0 = take(sizeof(typename Vec4<R>::T) * (size_t)g4max);
-        o.sp1 = take(sizeof(R) * (size_t)g4max);
+        o.sp1 = grad ? take(sizeof(R) * (size_t)g4max) : 0;
         o.smeta = take(sizeof(int) * (size_t)g4max);
         o.btab = take(sizeof(I2) * (size_t)nel * (size_t)nel);   // block f*nel+h, f <= h
     }
@@ -355,15 +367,17 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(int nmax, int nsfmax,
 
 // bytes of dynamic LDS of a wavefront (GA atom regions)
 template <typename R>
-__host__ __device__ inline size_t lds_bytes(int nmax, int ga, int nsfmax, int g4max, int nel, bool grad)
+__host__ __device__ inline size_t lds_bytes(int nmax, int ga, int nsfmax, int g4max, int nel, bool grad,
+                                            bool allp = false)
 {
-    return ga * lds_offsets<R>(nmax, nsfmax, g4max, nel, grad).total;
+    return ga * lds_offsets<R>(nmax, nsfmax, g4max, nel, grad, allp).total;
 }
 
 template <typename R>
-__device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int nel, bool grad, LDS char *base)
+__device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int nel, bool grad, LDS char *base,
+                                           bool allp = false)
 {
-    const LdsOffsets o = lds_offsets<R>(nmax, nsfmax, g4max, nel, grad);
+    const LdsOffsets o = lds_offsets<R>(nmax, nsfmax, g4max, nel, grad, allp);
     Lds<R> L{};
     L.a = (LDS typename Lds<R>::V *)(base + o.a);
     L.b = (LDS typename Lds<R>::V *)(base + o.b);
@@ -378,9 +392,10 @@ __device__ __forceinline__ Lds<R> make_lds(int nmax, int nsfmax, int g4max, int 
         L.gz = pd + 2 * nmax;
     }
     L.acc = (LDS R *)(base + o.acc);
-    if (grad) {
+    if (allp) L.accd = (LDS double *)(base + (sizeof(R) == 4 ? o.accd : o.acc));
+    if (grad || allp) {
         L.sp0 = (LDS typename Lds<R>::V *)(base + o.sp0);
-        L.sp1 = (LDS R *)(base + o.sp1);
+        if (grad) L.sp1 = (LDS R *)(base + o.sp1);
         L.smeta = (LDS int *)(base + o.smeta);
         L.btab = (LDS I2 *)(base + o.btab);
     }
@@ -914,6 +929,21 @@ __device__ __forceinline__ void circulant_pair(int p, int n, float inv_n, int &j
     if (k >= n) k -= n;
 }
 
+// block f*nel+h (f <= h) of the centre element e: (first function - g4s, count)
+template <typename R>
+__device__ __forceinline__ void block_table(const DevModel &m, const Lds<R> &L, int e, int g4s)
+{
+    for (int t = lane_id(); t < m.nel * m.nel; t += 64) {
+        const int f = t / m.nel, h = t - f * m.nel;
+        int sx = 0, cnt = 0;
+        if (f <= h) {
+            cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, sx, cnt);
+            sx -= g4s;
+        }
+        L.btab[t] = I2{sx, cnt};
+    }
+}
+
 // G4 tables of the centre element for the all-pairs pass, with the weights
 // w = dE/dG of the raw G (in acc) folded in.
 template <typename R>
@@ -934,24 +964,38 @@ __device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, 
         const int sq = cld(m.g4_sq, x);
         L.smeta[u] = (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0);
     }
-    for (int t = lane; t < m.nel * m.nel; t += 64) {
-        const int f = t / m.nel, h = t - f * m.nel;
-        int sx = 0, cnt = 0;
-        if (f <= h) {
-            cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, sx, cnt);
-            sx -= g4s;
-        }
-        L.btab[t] = I2{sx, cnt};
+    block_table<R>(m, L, e, g4s);
+}
+
+// G4 tables of the centre element for the all-pairs values pass: sp0 =
+// (exp scale, lambda, 2^(1-zeta), zeta - 1), smeta = sq | new-eta << 8 |
+// the function's slot in the atom's G << 16
+template <typename R>
+__device__ __forceinline__ void values_tables(const DevModel &m, const Lds<R> &L, int e, int g4s, int g4n)
+{
+    const int lane = lane_id();
+    for (int u = lane; u < g4n; u += 64) {
+        const int x = g4s + u;
+        typename Lds<R>::V p;
+        p.x = g4_ea<R>(m, x);
+        p.y = tabc<R>(m.g4_lam, m.g4f_lam, x);
+        p.z = tabc<R>(m.g4_pre, m.g4f_pre, x);
+        p.w = tabc<R>(m.g4_zeta, m.g4f_zeta, x) - R(1);
+        L.sp0[u] = p;
+        const int sq = cld(m.g4_sq, x);
+        L.smeta[u] = (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0) | (cld(m.g4_s, x) << 16);
     }
+    block_table<R>(m, L, e, g4s);
 }
 
 // candidates [cb, ce) of the circulant enumeration, compacted to the
 // triplets whose block has G4 functions and r_jk < rc
 template <typename R>
-__device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int n, float inv_n, R rc2, int nel)
+__device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int n, float inv_n, R rc2, int nel,
+                                           int out = 0)
 {
     const int lane = lane_id();
-    int nv = 0;
+    int nv = out;
     for (int p0 = cb; p0 < ce; p0 += 64) {
         const int p = p0 + lane;
         bool ok = false;
@@ -970,7 +1014,52 @@ __device__ __forceinline__ int compact_circ(const Lds<R> &L, int cb, int ce, int
         if (ok) L.pairs[nv + mbcnt(mk)] = pk;
         nv += __popcll(mk);
     }
+    return nv - out;
+}
+
+// candidates from p0 on, compacted as compact_circ, until the next round
+// could overflow cap (p0 advances past what was taken)
+template <typename R>
+__device__ __forceinline__ int compact_circ_fill(const Lds<R> &L, int &p0, int np, int cap, int n, float inv_n, R rc2,
+                                                 int nel)
+{
+    int nv = 0;
+    for (; p0 < np && nv <= cap - 64; p0 += 64) nv += compact_circ<R>(L, p0, min(np, p0 + 64), n, inv_n, rc2, nel, nv);
     return nv;
+}
+
+// Pass A over a compacted chunk of the all-pairs enumeration: each lane
+// evaluates its triplet for the functions of the triplet's block and adds
+// 2^(1-zeta) (1 + lambda cos)^zeta exp(-eta sum r^2) fc fc fc to the FP64
+// function sums with LDS atomics, whose order within a wavefront is fixed
+template <typename R>
+__device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
+{
+    const int lane = lane_id();
+    for (int q = lane; q < nv; q += 64) {
+        const int pk = L.pairs[q];
+        const int j = pk & 0x3FF, k = (pk >> 10) & 0x3FF;
+        const I2 bt = L.btab[pk >> 20];
+        const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+        const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
+        const R rjk2 = ex * ex + ey * ey + ez * ez;
+        const R fcjk = cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc);
+        const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * aj.w * ak.w;
+        const R F = bj.y * bk.y * fcjk;
+        const R r2 = bj.w + bk.w + rjk2;
+        R EF = 0;
+        for (int t = 0; t < bt.y; ++t) {
+            const int u = bt.x + t;
+            const typename Lds<R>::V p = L.sp0[u];
+            const int mt = L.smeta[u];
+            if (t == 0 || (mt & (1 << 8))) EF = M<R>::expsc(p.x, r2) * F;
+            const R b = R(1) + p.y * cs;
+            const int sq = mt & 0xFF;
+            const R P = sq != 0xFF ? pow2z<R>(b, sq) : b * angpow<R>(b, cld(m.g4_ipow, g4s + u), p.w);
+            __hip_atomic_fetch_add(&L.accd[mt >> 16], (double)(p.z * (P * EF)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
 }
 
 template <typename R>
@@ -1044,7 +1133,12 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
 template <typename R, int NMAX, int GA, bool GRAD>
 constexpr int kSfWaves = sizeof(R) == 8 && NMAX <= 96 && GA == 1 ? (GRAD ? 3 : 4) : 1;
 
-template <typename R, int NMAX, int GA>
+// ALLP (one atom per wavefront): G4 over all pairs of the atom, in pass B's
+// order, with the function sums in FP64 (g4_values_list); the compacted
+// triplets go to the atom's list for pass B (up to tcap; the count is
+// stored in full, and pass B compacts again when it is larger). Else per
+// element-pair block (g4_blocks).
+template <typename R, int NMAX, int GA, bool ALLP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, NMAX, GA, false>))) void k_sf_values(DevModel m, Geo g, Work w, R *G)
 {
     constexpr int GL = 64 / GA;
@@ -1056,9 +1150,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
     const int i = tn < 0 ? t0 : (grp < tn ? w.perm[t0 + grp] : -1);
     const ElemInfo ei = cld_einfo(m.einfo, e);
     const int nsf = ei.nsf;
-    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, 0, m.nel, false,
-                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, 0, m.nel, false).total);
-    for (int s = gl; s < nsf; s += GL) L.acc[s] = 0;
+    const int g4x = ALLP ? m.g4_max : 0;
+    const Lds<R> L = make_lds<R>(NMAX, m.nsfmax, g4x, m.nel, false,
+                                 (LDS char *)smem + grp * lds_offsets<R>(NMAX, m.nsfmax, g4x, m.nel, false, ALLP).total,
+                                 ALLP);
+    for (int s = gl; s < nsf; s += GL) {
+        if (ALLP) L.accd[s] = 0;
+        else L.acc[s] = 0;
+    }
     stage<R, GL, NMAX, false>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, (R *)nullptr);
     __syncthreads();
     const R rce = (R)ei.rc;
@@ -1074,19 +1173,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
             const R dr = b.x - rs;
             sum += M<R>::expsc(ea, dr * dr) * b.y;
         }
-        L.acc[cld(m.g2_s, x)] += sum;
+        if (ALLP) L.accd[cld(m.g2_s, x)] += (double)sum;
+        else L.acc[cld(m.g2_s, x)] += sum;
     }
     // G4, symmetryFunctions.f90:231-305
-    int *tl = (sizeof(R) == 8 && GA == 1 && w.trip) ? w.trip + (size_t)i * w.tcap : nullptr;
-    const int ntrip = g4_blocks<R, GL, false>(m, L, e, rce, tl, w.tcap);
-    if (sizeof(R) == 8 && tl && gl == 0) w.tcount[i] = ntrip;
+    if constexpr (ALLP) {
+        values_tables<R>(m, L, e, ei.g4_start, ei.g4_count);
+        __syncthreads();
+        const int n = uni(L.seg[kMaxEl]), np = n * (n - 1) / 2;
+        const float inv_n = 1.0f / (float)max(n, 1);
+        const R rrc = R(1) / rce;
+        int *tl = (GA == 1 && w.trip && i >= 0) ? w.trip + (size_t)i * w.tcap : nullptr;
+        int tbase = 0;
+        for (int p0 = 0; p0 < np;) {
+            const int nv = uni(compact_circ_fill<R>(L, p0, np, kAllPairsChunk, n, inv_n, rce * rce, m.nel));
+            __syncthreads();
+            if (tl)
+                for (int q = gl; q < nv && tbase + q < w.tcap; q += 64) tl[tbase + q] = L.pairs[q];
+            tbase += nv;
+            g4_values_list<R>(m, L, nv, ei.g4_start, rrc);
+            __syncthreads();
+        }
+        if (tl && gl == 0) w.tcount[i] = tbase;
+    } else {
+        int *tl = (sizeof(R) == 8 && GA == 1 && w.trip) ? w.trip + (size_t)i * w.tcap : nullptr;
+        const int ntrip = g4_blocks<R, GL, false>(m, L, e, rce, tl, w.tcap);
+        if (sizeof(R) == 8 && tl && gl == 0) w.tcount[i] = ntrip;
+    }
     __syncthreads();
     // centre and scale, hdnnps.f90:471-481
     if (i < 0) return;
     const int so = ei.sf_off;
     for (int s = gl; s < m.nsfmax; s += GL)
         G[(size_t)i * m.nsfmax + s] =
-            s < nsf ? (R)(((double)L.acc[s] - m.sf_mean[so + s]) / m.sf_den[so + s]) : R(0);
+            s < nsf ? (R)(((ALLP ? L.accd[s] : (double)L.acc[s]) - m.sf_mean[so + s]) / m.sf_den[so + s]) : R(0);
 }
 
 // ---------------------------------------------------------------- pass B
@@ -1153,8 +1273,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         } else {
             const int np = n * (n - 1) / 2;
             const float inv_n = 1.0f / (float)max(n, 1);
-            for (int cb = 0; cb < np; cb += kPairChunk<R>) {
-                const int nv = uni(compact_circ<R>(L, cb, min(np, cb + kPairChunk<R>), n, inv_n, rce2, m.nel));
+            for (int p0 = 0; p0 < np;) {
+                const int nv = uni(compact_circ_fill<R>(L, p0, np, kPairChunk<R>, n, inv_n, rce2, m.nel));
                 __syncthreads();
                 g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
                 __syncthreads();
@@ -1189,12 +1309,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
 // wavefronts of a launch: tiles of GA atoms of one element
 static int sf_blocks(const DevModel &m, const Work &w, int ga) { return ga == 1 ? w.nat : (w.nat + ga - 1) / ga + m.nel; }
 
+// pass A over all pairs with one atom per wavefront, unless
+// HDNNP_SF_ALLPAIRS=0 in the environment selects the per-block pass
+bool sf_allpairs_enabled()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("HDNNP_SF_ALLPAIRS");
+        return !(v && std::strcmp(v, "0") == 0);
+    }();
+    return on;
+}
+
 template <typename R>
 void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, R *G, hipStream_t s)
 {
+    const bool allp = ga == 1 && sf_allpairs_enabled();
 #define HDNNP_SFV(N, A)                                                                            \
-    hipLaunchKernelGGL((k_sf_values<R, N, A>), dim3(sf_blocks(m, w, A)), dim3(64),                  \
-                       lds_bytes<R>(N, A, m.nsfmax, 0, m.nel, false), s, m, g, w, G)
+    do {                                                                                           \
+        if (allp && A == 1)                                                                        \
+            hipLaunchKernelGGL((k_sf_values<R, N, A, A == 1>), dim3(sf_blocks(m, w, A)),            \
+                               dim3(64), lds_bytes<R>(N, A, m.nsfmax, m.g4_max, m.nel, false, true), s, m, g, w, G); \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_sf_values<R, N, A, false>), dim3(sf_blocks(m, w, A)), dim3(64),  \
+                               lds_bytes<R>(N, A, m.nsfmax, 0, m.nel, false), s, m, g, w, G);       \
+    } while (0)
     if (nmax <= 64) {
         if (ga == 4) HDNNP_SFV(64, 4);
         else if (ga == 2) HDNNP_SFV(64, 2);
@@ -1657,6 +1795,9 @@ template void launch_sf_single<float>(const DevModel &, const Geo &, const Work 
 size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad)
 {
     nmax = sf_bucket(nmax);
+    if (!grad && sf_allpairs_enabled())
+        return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, nel, false, true)
+                    : lds_bytes<float>(nmax, 1, nsfmax, g4max, nel, false, true);
     return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, nel, grad)
                 : lds_bytes<float>(nmax, 1, nsfmax, g4max, nel, grad);
 }
