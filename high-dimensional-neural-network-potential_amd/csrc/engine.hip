@@ -521,10 +521,8 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     R *G = G_.as<R>(), *dEdG = dEdG_.as<R>(), *gent = gent_.as<R>();
 
     const int ga = sf_tile(sizeof(R) == 8, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
-    // FP64 with one atom per wavefront: pass A hands its compacted triplets to
-    // pass B. (In FP32 the list's stores cost pass A about what they save
-    // pass B: its barriers wait for them.)
-    if (sizeof(R) == 8 && ga == 1) {
+    // one atom per wavefront: pass A hands its compacted triplets to pass B
+    if (ga == 1) {
         trip_.ensure(4 * (size_t)nat * w.tcap);
         tcount_.ensure(4 * (size_t)nat);
         w.trip = trip_.as<int>();
@@ -663,7 +661,8 @@ static int entry_capacity(double est, int hint, int lim)
 
 // Triplets per atom the value pass keeps for the gradient pass: the pairs of
 // lim neighbours, at most kTripletSlots. An atom with more falls back to the
-// gradient pass's own compaction, which adds its triplets in another order.
+// gradient pass's own compaction: the same triplets in the same order, but
+// in other chunks, so that its sums round differently.
 // Whether an atom falls back therefore must not depend on the evaluation it
 // is part of (a batch result equals the structure's result alone, bit for
 // bit): an atom has at most n(n-1)/2 <= lim(lim-1)/2 triplets, so the test
@@ -704,7 +703,7 @@ size_t Engine::footprint_bytes(const Footprint &f, bool slack) const
     const size_t nat = (size_t)f.nat;
     size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
     b += nat * cap * (16 + 3 * R);                                                           // list slots
-    if (R == 8) b += nat * (4 * (size_t)triplet_capacity(entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
+    b += nat * (4 * (size_t)triplet_capacity(entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
     b += (size_t)f.ncell * 12;                                                               // cells
     if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
     b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure
@@ -1425,7 +1424,7 @@ void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
         const int ga = sf_tile(fp64, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
         w.trip = nullptr;
         w.tcount = nullptr;
-        if (fp64 && ga == 1) {
+        if (ga == 1) {
             trip_.ensure(4 * (size_t)nat * w.tcap);
             tcount_.ensure(4 * (size_t)nat);
             w.trip = trip_.as<int>();

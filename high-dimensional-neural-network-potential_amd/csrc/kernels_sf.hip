@@ -310,7 +310,8 @@ struct Lds {
     LDS V *b;                     // (r, fc, fc'/r, r^2)
     LDS unsigned char *el;        // element rank
     LDS unsigned short *ent;      // list slot within the atom's list (< cap <= 1024)
-    LDS int *pairs;               // compacted triplets (j | k << 16)
+    LDS int *pairs;               // compacted triplets: j | k << 16 (per block), j | k << 10 | block << 20
+                                  // (all pairs)
     LDS int *seg;                 // [kMaxEl+1] element segments of the staged neighbours
     LDS double *gx, *gy, *gz;     // gradients: per-neighbour accumulators (FP64: ds_add_f64
                                   // is much faster than ds_add_f32 here)
@@ -477,7 +478,9 @@ __device__ __forceinline__ Lat9 atom_lattice(const Geo &g, const Work &w, int i)
     return L;
 }
 
-// Geometry of one list entry: d = R_i - (R_j + L.h) (FP64), r = |d|.
+// Geometry of one list entry: d = R_i - (R_j + L.h) (FP64), r = |d| in
+// the precision R of the symmetry functions (FP32: from the rounded d).
+template <typename R = double>
 __device__ __forceinline__ void entry_geometry(const Lat9 &lt, const Work &w, const double4 &pi, int e,
                                                double &d0, double &d1, double &d2, double &r, int &j)
 {
@@ -492,7 +495,12 @@ __device__ __forceinline__ void entry_geometry(const Lat9 &lt, const Work &w, co
     d0 = pi.x - (pj.x + l0);
     d1 = pi.y - (pj.y + l1);
     d2 = pi.z - (pj.z + l2);
-    r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if constexpr (sizeof(R) == 4) {
+        const float x = (float)d0, y = (float)d1, z = (float)d2;
+        r = (double)M<float>::sqrt(x * x + y * y + z * z);
+    } else {
+        r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
 }
 
 // Stage the neighbours of atom i (i < 0: idle group) with r < rc(e) into its
@@ -505,8 +513,9 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
                                       const Lds<R> &L, R *gent)
 {
     // KEEP: up to 4 chunks of GL entries the counting pass keeps its geometry
-    // in registers for the placing pass (pays in pass B; in pass A the extra
-    // registers cost more occupancy than the recomputation)
+    // in registers for the placing pass (pays in pass B and in the FP32
+    // all-pairs pass A; elsewhere in pass A the extra registers cost more
+    // occupancy than the recomputation)
     constexpr int NC = (NMAX + GL - 1) / GL;
     constexpr bool kKeep = KEEP && NC <= 4;
     constexpr int NK = kKeep ? NC : 1;
@@ -530,7 +539,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
         double d0 = 0, d1 = 0, d2 = 0, r = 1e300;
         if (q < n) {
             int j;
-            entry_geometry(lt, w, pi, off + q, d0, d1, d2, r, j);
+            entry_geometry<R>(lt, w, pi, off + q, d0, d1, d2, r, j);
             f = r < rce ? w.eidx[j] : -1;
         }
         if (kKeep) {
@@ -574,7 +583,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
             r = 1e300;
             if (q < n) {
                 int j;
-                entry_geometry(lt, w, pi, off + q, d0, d1, d2, r, j);
+                entry_geometry<R>(lt, w, pi, off + q, d0, d1, d2, r, j);
                 f = r < rce ? w.eidx[j] : -1;
             }
         }
@@ -595,7 +604,7 @@ __device__ __forceinline__ void stage(const DevModel &m, const Geo &g, const Wor
             va.x = (R)d0;
             va.y = (R)d1;
             va.z = (R)d2;
-            va.w = (R)(1.0 / r);
+            va.w = sizeof(R) == 4 ? M<R>::rcp(rr) : (R)(1.0 / r);
             vb.x = rr;
             vb.y = fc;
             vb.z = dfc;
@@ -1032,9 +1041,10 @@ __device__ __forceinline__ int compact_circ_fill(const Lds<R> &L, int &p0, int n
 // evaluates its triplet for the functions of the triplet's block and adds
 // 2^(1-zeta) (1 + lambda cos)^zeta exp(-eta sum r^2) fc fc fc to the FP64
 // function sums with LDS atomics, whose order within a wavefront is fixed
-template <typename R>
+template <typename R, int CT>
 __device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
 {
+    const int ct = CT >= 0 ? CT : m.cutoff_type;
     const int lane = lane_id();
     for (int q = lane; q < nv; q += 64) {
         const int pk = L.pairs[q];
@@ -1043,7 +1053,7 @@ __device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &
         const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
         const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
         const R rjk2 = ex * ex + ey * ey + ez * ez;
-        const R fcjk = cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc);
+        const R fcjk = cutoff_in<R>(ct, M<R>::sqrt(rjk2), rrc);
         const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * aj.w * ak.w;
         const R F = bj.y * bk.y * fcjk;
         const R r2 = bj.w + bk.w + rjk2;
@@ -1062,9 +1072,10 @@ __device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &
     }
 }
 
-template <typename R>
+template <typename R, int CT>
 __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
 {
+    const int ct = CT >= 0 ? CT : m.cutoff_type;
     const int lane = lane_id();
     for (int q = lane; q < nv; q += 64) {
         const int pk = L.pairs[q];
@@ -1074,7 +1085,7 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
         const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
         const R rjk2 = ex * ex + ey * ey + ez * ez;
         R fcjk, dfcjk;
-        cutoff_in<R>(m.cutoff_type, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+        cutoff_in<R>(ct, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
         const R irj = aj.w, irk = ak.w;
         const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
         const R inv = irj * irk;
@@ -1124,6 +1135,14 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
     }
 }
 
+// the tanh^3 cutoff (the reference's) inlined, other types through the switch
+template <typename R>
+__device__ __forceinline__ void g4_grad_list_ct(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
+{
+    if (m.cutoff_type == 2) g4_grad_list<R, 2>(m, L, nv, g4s, rrc);
+    else g4_grad_list<R, -1>(m, L, nv, g4s, rrc);
+}
+
 // ---------------------------------------------------------------- pass A
 // Waves per SIMD the FP64 passes at the 96-neighbour staging, one atom per
 // wavefront, are compiled for: their LDS allows 4 (values) and 3
@@ -1158,7 +1177,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         if (ALLP) L.accd[s] = 0;
         else L.acc[s] = 0;
     }
-    stage<R, GL, NMAX, false>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, (R *)nullptr);
+    stage<R, GL, NMAX, ALLP && sizeof(R) == 4>(m, g, w, i, wave_max(i >= 0 ? w.count[i] : 0), ei.rc, L, (R *)nullptr);
     __syncthreads();
     const R rce = (R)ei.rc;
 
@@ -1191,7 +1210,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
             if (tl)
                 for (int q = gl; q < nv && tbase + q < w.tcap; q += 64) tl[tbase + q] = L.pairs[q];
             tbase += nv;
-            g4_values_list<R>(m, L, nv, ei.g4_start, rrc);
+            if (m.cutoff_type == 2) g4_values_list<R, 2>(m, L, nv, ei.g4_start, rrc);   // tanh^3 inlined
+            else g4_values_list<R, -1>(m, L, nv, ei.g4_start, rrc);
             __syncthreads();
         }
         if (tl && gl == 0) w.tcount[i] = tbase;
@@ -1259,7 +1279,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         grad_tables<R>(m, L, e, ei.g4_start, ei.g4_count);
         __syncthreads();
         const R rce2 = rce * rce, rrc = R(1) / rce;
-        const int tc = (sizeof(R) == 8 && w.trip) ? uni(w.tcount[i]) : -1;
+        const int tc = w.trip ? uni(w.tcount[i]) : -1;
         if (tc >= 0 && tc <= w.tcap) {
             // the triplets pass A compacted
             const int *tl = w.trip + (size_t)i * w.tcap;
@@ -1267,7 +1287,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
                 const int nv = min(kPairChunk<R>, tc - cb);
                 for (int q = gl; q < nv; q += 64) L.pairs[q] = tl[cb + q];
                 __syncthreads();
-                g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
+                g4_grad_list_ct<R>(m, L, nv, ei.g4_start, rrc);
                 __syncthreads();
             }
         } else {
@@ -1276,7 +1296,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
             for (int p0 = 0; p0 < np;) {
                 const int nv = uni(compact_circ_fill<R>(L, p0, np, kPairChunk<R>, n, inv_n, rce2, m.nel));
                 __syncthreads();
-                g4_grad_list<R>(m, L, nv, ei.g4_start, rrc);
+                g4_grad_list_ct<R>(m, L, nv, ei.g4_start, rrc);
                 __syncthreads();
             }
         }
