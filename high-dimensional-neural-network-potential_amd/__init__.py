@@ -38,7 +38,8 @@ EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_eval
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
-           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_destroy", "hdnnp_set_tile", "hdnnp_get_tile", "hdnnp_autotune",
+           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_set_graphs", "hdnnp_graph_counts", "hdnnp_destroy",
+           "hdnnp_set_tile", "hdnnp_get_tile", "hdnnp_autotune",
            "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_md_open",
            "hdnnp_server_md_step", "hdnnp_server_md_wait", "hdnnp_server_md_get", "hdnnp_server_md_close",
            "hdnnp_server_stats", "hdnnp_server_last_error", "hdnnp_server_destroy")
@@ -117,6 +118,9 @@ def load_library():
         lib.hdnnp_last_error.argtypes = [vp]
         lib.hdnnp_last_error.restype = ctypes.c_char_p
         lib.hdnnp_last_timings.argtypes = [vp, _dp]
+        lib.hdnnp_set_graphs.argtypes = [vp, ctypes.c_int32]
+        lib.hdnnp_graph_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64)]
         lib.hdnnp_destroy.argtypes = [vp]
         lib.hdnnp_destroy.restype = None
         lib.hdnnp_set_tile.argtypes = [vp, ctypes.c_int32]
@@ -271,6 +275,23 @@ class Hdnnp:
     @tile.setter
     def tile(self, atoms_per_wave: int):
         self._check(self.lib.hdnnp_set_tile(self.ctx, int(atoms_per_wave)))
+
+    @property
+    def graphs(self) -> bool:
+        """graph replay of small evaluations (hdnnp_set_graphs)"""
+        on = ctypes.c_int32()
+        self._check(self.lib.hdnnp_graph_counts(self.ctx, ctypes.byref(on), None, None))
+        return bool(on.value)
+
+    @graphs.setter
+    def graphs(self, on: bool):
+        self._check(self.lib.hdnnp_set_graphs(self.ctx, 1 if on else 0))
+
+    def graph_counts(self):
+        """(captured, replayed) evaluations of this context"""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.hdnnp_graph_counts(self.ctx, None, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def autotune(self, structures):
         """hdnnp_autotune on a sample: (chosen tile, {tile: ms of the SF passes})."""

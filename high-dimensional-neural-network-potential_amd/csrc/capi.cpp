@@ -441,6 +441,20 @@ int hdnnp_last_timings(hdnnp_ctx *ctx, double *ms6)
     return HDNNP_OK;
 }
 
+int hdnnp_set_graphs(hdnnp_ctx *ctx, int32_t on)
+{
+    if (!ctx || (on != 0 && on != 1)) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->set_eval_graphs(on != 0); });
+}
+
+int hdnnp_graph_counts(const hdnnp_ctx *ctx, int32_t *on, int64_t *captures, int64_t *replays)
+{
+    if (!ctx) return HDNNP_ERR_ARG;
+    if (on) *on = ctx->eng->eval_graphs() ? 1 : 0;
+    ctx->eng->eval_graph_counts(captures, replays);
+    return HDNNP_OK;
+}
+
 void hdnnp_destroy(hdnnp_ctx *ctx)
 {
     if (!ctx) return;

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace hdnnp {
@@ -191,6 +192,7 @@ Engine::Engine(std::vector<HostModel> members, int precision) : precision_(preci
     members_.assign(std::make_move_iterator(members.begin() + 1), std::make_move_iterator(members.end()));
     if (precision_ != 0 && precision_ != 1) throw Error(1, "precision must be HDNNP_FP64 or HDNNP_MIXED");
     if ((int)model_.el.size() > kMaxEl) throw Error(6, "too many elements for this build");
+    if (const char *v = std::getenv("HDNNP_EVAL_GRAPHS")) eval_graphs_on_ = std::strcmp(v, "1") == 0;
     // committee members (globalHdnnps.f90:31-70) reuse the first potential's
     // symmetry functions and scaling and differ in their networks and atom
     // energies: every element of the first needs a network with as many inputs
@@ -234,6 +236,7 @@ size_t Engine::device_bytes() const
 void Engine::release_work()
 {
     if (stream_) HIPCHK(hipStreamSynchronize(stream_));
+    drop_eval_graphs();
     for (DevMem *d : work_buffers()) d->release();
     uploaded_.clear();
     prep_ = Prepared{};
@@ -245,6 +248,8 @@ void Engine::release_work()
 Engine::~Engine()
 {
     HipUnsafe guard;
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_eval_graphs();
     for (auto &d : model_mem_) d.release();
     for (DevMem *d : work_buffers()) d->release();
     misc_.release();
@@ -713,7 +718,7 @@ size_t Engine::footprint_bytes(const Footprint &f, bool slack) const
 }
 
 void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
-                   double rc_list)
+                   double rc_list, bool launch)
 {
     const int nstruct = (int)st.size();
     const double rc = rc_list;
@@ -762,7 +767,6 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     }
     const int ncell = (int)cell_off;
 
-    HIPCHK(hipEventRecord(ev_[0], s));
     // per-atom buffers
     eidx_.ensure(4 * (size_t)nat);
     wpos_.ensure(sizeof(double4) * (size_t)nat);
@@ -840,10 +844,6 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.force = nullptr;              // set by finish
     w.result = res_.as<double>();
 
-    // flags and element counts start at zero (a fill kernel: no memset node
-    // when a caller captures the launches)
-    launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
-
     Geo g{};
     g.sg = geo_.as<StructGeo>();
     g.rc2 = rc * rc;
@@ -858,28 +858,13 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
     const bool small_sort = nat <= kSmallSortAtoms && ncell <= kSmallSortCells;
-    if (small_sort) {
-        // one workgroup does the set-up and sorts by cell and by element; the
-        // radix sort's input and output keys serve as scratch
-        launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, s);
-    } else {
-        launch_prep(dm_, g, w, iota_.as<int>(), s);
+    if (!small_sort) {   // the radix sort's temporaries and the element sort's chunks
         size_t b1 = 0;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                                   w.cell_atoms, nat, 0, bits, s));
         cub_tmp_.ensure(b1);
-        size_t bt = cub_tmp_.bytes;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
-                                                  w.cell_atoms, nat, 0, bits, s));
-        launch_cell_bounds(w, ncell, s);
-    }
-    // atoms sorted by element (MLP tiles); needs no host round trip
-    if (!small_sort) {
         elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
-        launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[1], s));
 
     // Entry capacity per atom: the most neighbours seen by this context, or an
     // estimate from the mean density, rounded up to a staging size of the
@@ -905,18 +890,49 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
         pos_start_.ensure((size_t)ps_total * 2);
         w.pos_start = pos_start_.as<unsigned short>();
     }
-    launch_nl_build(g, w, ncell, s);
-    launch_nl_rev(g, w, s);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev_[2], s));
     prep_.w = w;
     prep_.g = g;
     prep_.nchunk = nchunk;
     prep_.lds_lim = lds_lim;
+    prep_.ncell = ncell;
+    prep_.bits = bits;
+    prep_.small_sort = small_sort;
     prep_.o_res = o_res;
     prep_.o_flags = o_flags;
     prep_.res_b = res_b;
     prep_.valid = true;
+    if (launch) build_launch(s, true);
+}
+
+void Engine::build_launch(hipStream_t s, bool timed)
+{
+    if (prep_.nat == 0) return;
+    if (!prep_.valid) throw Error(5, "no prepared evaluation");
+    Work w = prep_.w;
+    const Geo g = prep_.g;
+    if (timed) HIPCHK(hipEventRecord(ev_[0], s));
+    // flags and element counts start at zero (a fill kernel: no memset node
+    // when the launches are captured)
+    launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
+    if (prep_.small_sort) {
+        // one workgroup does the set-up and sorts by cell and by element; the
+        // radix sort's input and output keys serve as scratch
+        launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, s);
+    } else {
+        launch_prep(dm_, g, w, iota_.as<int>(), s);
+        size_t bt = cub_tmp_.bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                                  w.cell_atoms, prep_.nat, 0, prep_.bits, s));
+        launch_cell_bounds(w, prep_.ncell, s);
+        // atoms sorted by element (MLP tiles); needs no host round trip
+        launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
+    }
+    HIPCHK(hipGetLastError());
+    if (timed) HIPCHK(hipEventRecord(ev_[1], s));
+    launch_nl_build(g, w, prep_.ncell, s);
+    launch_nl_rev(g, w, s);
+    HIPCHK(hipGetLastError());
+    if (timed) HIPCHK(hipEventRecord(ev_[2], s));
 }
 
 bool Engine::finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat)
@@ -1048,10 +1064,127 @@ void Engine::run(std::vector<StructIn> &st, const int *Z_dev, const double *pos_
 {
     const double rc = model_.maxrc;
     for (;;) {
-        build(st, Z_dev, pos_dev, s, rc);
-        if (finish(force_dev, s, energy, dEdLat)) return;
+        build(st, Z_dev, pos_dev, s, rc, false);
+        launch_evaluation(force_dev, s);
+        if (finish_collect(s, energy, dEdLat)) return;
         if (retry-- <= 0) throw Error(6, "neighbour capacity did not converge");
     }
+}
+
+// Everything the kernels of build_launch and finish_launch take (launch
+// parameters, buffer addresses, the device models): a graph captured from
+// them replays the same launches while this stays the same.
+std::vector<char> Engine::eval_key() const
+{
+    std::vector<char> k = launch_key();
+    auto add = [&k](const void *p, size_t n) {
+        const char *c = (const char *)p;
+        k.insert(k.end(), c, c + n);
+    };
+    const int ints[] = {prep_.ncell, prep_.bits, prep_.small_sort ? 1 : 0, skip_values_ ? 1 : 0,
+                        (int)dmm_.size(), mlp_maxw_, mlp_sumc_};
+    add(ints, sizeof ints);
+    for (const DevModel &d : dmm_) add(&d, sizeof d);
+    add(&dm_, sizeof dm_);
+    add(maxw_m_.data(), sizeof(int) * maxw_m_.size());
+    std::vector<const void *> ptrs;
+    for (DevMem *b : const_cast<Engine *>(this)->work_buffers()) ptrs.push_back(b->p);
+    ptrs.push_back(misc_.p);
+    ptrs.push_back(stream_);
+    add(ptrs.data(), sizeof(void *) * ptrs.size());
+    return k;
+}
+
+void Engine::launch_evaluation(double *force_dev, hipStream_t s)
+{
+    const bool graphable = eval_graphs_on_ && prep_.valid && prep_.nat > 0 && prep_.small_sort && s == stream_ &&
+                           !t_arena && !skip_values_;
+    if (!graphable) {
+        build_launch(s, true);
+        finish_launch(force_dev, s, true);
+        return;
+    }
+    prep_.w.force = force_dev;
+    std::vector<char> key = eval_key();
+    EvalGraph *hit = nullptr;
+    for (EvalGraph &e : eval_graphs_)
+        if (e.key == key) hit = &e;
+    if (!hit && key == eval_seen_) hit = capture_evaluation(force_dev, s, key);
+    if (!hit) {
+        build_launch(s, true);
+        finish_launch(force_dev, s, true);
+        eval_seen_ = eval_key();   // with the buffers this launch sized
+        return;
+    }
+    hit->used = ++eval_tick_;
+    // the replay's whole device time is the first stage's
+    HIPCHK(hipEventRecord(ev_[0], s));
+    HIPCHK(hipGraphLaunch(hit->exec, s));
+    for (int k = 1; k < 7; ++k) HIPCHK(hipEventRecord(ev_[k], s));
+    ++eval_replays_;
+}
+
+Engine::EvalGraph *Engine::capture_evaluation(double *force_dev, hipStream_t s, std::vector<char> key)
+{
+    // no other thread allocates, frees or copies synchronously through this
+    // library while the capture runs (HipUnsafe)
+    HipUnsafe::Capture excl;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        eval_graphs_on_ = false;
+        return nullptr;
+    }
+    bool ok = true;
+    try {
+        build_launch(s, false);
+        finish_launch(force_dev, s, false);
+    } catch (const std::exception &) {
+        ok = false;
+    }
+    hipGraph_t g = nullptr;
+    ok = hipStreamEndCapture(s, &g) == hipSuccess && ok && g;
+    hipGraphExec_t x = nullptr;
+    if (ok) ok = hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {   // launch directly from now on
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        eval_graphs_on_ = false;
+        return nullptr;
+    }
+    if ((int)eval_graphs_.size() >= kEvalGraphs) {   // the least recently used goes
+        size_t lru = 0;
+        for (size_t k = 1; k < eval_graphs_.size(); ++k)
+            if (eval_graphs_[k].used < eval_graphs_[lru].used) lru = k;
+        (void)hipGraphExecDestroy(eval_graphs_[lru].exec);
+        (void)hipGraphDestroy(eval_graphs_[lru].graph);
+        eval_graphs_.erase(eval_graphs_.begin() + lru);
+    }
+    eval_graphs_.push_back(EvalGraph{std::move(key), g, x, ++eval_tick_});
+    ++eval_captures_;
+    return &eval_graphs_.back();
+}
+
+void Engine::drop_eval_graphs()
+{
+    for (EvalGraph &e : eval_graphs_) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
+    eval_graphs_.clear();
+    eval_seen_.clear();
+}
+
+void Engine::set_eval_graphs(bool on)
+{
+    eval_graphs_on_ = on;
+    if (!on) drop_eval_graphs();
+}
+
+void Engine::eval_graph_counts(int64_t *captures, int64_t *replays) const
+{
+    if (captures) *captures = eval_captures_;
+    if (replays) *replays = eval_replays_;
 }
 
 void Engine::evaluate_batch_host(int nstruct, const int *nat, const int *Z, const double *pos, const int *periodic,
@@ -1135,6 +1268,8 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
     std::vector<double> E(nstruct), F(3 * (size_t)std::max<long>(total, 1));
     const int cand[3] = {1, 2, 4};
     const int keep = tile_;
+    const bool graphs = eval_graphs_on_;
+    eval_graphs_on_ = false;   // the stage timings of direct launches
     int best = 1;
     double best_ms = 0;
     try {
@@ -1157,9 +1292,11 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
         }
     } catch (...) {
         tile_ = keep;
+        eval_graphs_on_ = graphs;
         throw;
     }
     tile_ = best;
+    eval_graphs_on_ = graphs;
     return best;
 }
 

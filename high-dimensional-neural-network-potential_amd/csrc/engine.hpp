@@ -132,7 +132,36 @@ class Engine {
     void get_symfunctions(int ld, double *G, double *dEdG, double *Ei);
     void timings(double *ms6) const;
 
+    // With graphs on, evaluations of up to kSmallSortAtoms atoms on the
+    // context's own stream (build and finish: a dozen small kernels) are
+    // replayed from captured graphs when the same launch repeats: the second
+    // evaluation with the launch parameters and buffers of the one before it
+    // is captured, and later ones with any of the last kEvalGraphs captured
+    // launches replay it. Off by default (a replay measured slower than the
+    // direct launches, whose host side overlaps the kernels); on with
+    // HDNNP_EVAL_GRAPHS=1 or set_eval_graphs. Counts of captures and replays.
+    void set_eval_graphs(bool on);
+    bool eval_graphs() const { return eval_graphs_on_; }
+    void eval_graph_counts(int64_t *captures, int64_t *replays) const;
+
   private:
+    static constexpr int kEvalGraphs = 4;
+    struct EvalGraph {
+        std::vector<char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    std::vector<EvalGraph> eval_graphs_;
+    std::vector<char> eval_seen_;        // launch of the last direct evaluation
+    bool eval_graphs_on_ = false;
+    unsigned long long eval_tick_ = 0;
+    int64_t eval_captures_ = 0, eval_replays_ = 0;
+    // the prepared evaluation's kernels: replayed, captured or launched
+    void launch_evaluation(double *force_dev, hipStream_t s);
+    std::vector<char> eval_key() const;
+    EvalGraph *capture_evaluation(double *force_dev, hipStream_t s, std::vector<char> key);
+    void drop_eval_graphs();
     void init_device();
     void d2h(void *dst, const void *src, size_t bytes);
     // The structures st (atoms concatenated in Z_dev / pos_dev / force_dev).
@@ -145,8 +174,12 @@ class Engine {
     // (wrapped copies in wpos()). finish: symmetry functions of the current
     // wrapped positions, networks, forces and sums; false (nothing written)
     // when an atom had more neighbours than the list capacity -- build again.
+    // launch = false: the host part only (grids, tables and their upload,
+    // buffer sizes); build_launch then queues the kernels (timed = false:
+    // no timing events, the form a captured graph holds)
     void build(const std::vector<StructIn> &st, const int *Z_dev, const double *pos_dev, hipStream_t s,
-               double rc_list);
+               double rc_list, bool launch = true);
+    void build_launch(hipStream_t s, bool timed);
     bool finish(double *force_dev, hipStream_t s, double *energy, double *dEdLat);
     // finish in two parts: the kernels (queued on s, nothing waited for),
     // then the read-back, checks and sums (waits for s)
@@ -233,8 +266,9 @@ class Engine {
         Work w{};
         Geo g{};
         std::vector<StructIn> st;
-        int nat = 0, nchunk = 0, lds_lim = 0;
+        int nat = 0, nchunk = 0, lds_lim = 0, ncell = 0, bits = 1;
         size_t o_res = 0, o_flags = 0, res_b = 0;
+        bool small_sort = false;
         bool valid = false;
     } prep_;
     unsigned long long build_gen_ = 0;

@@ -326,8 +326,29 @@ const char *hdnnp_last_error(const hdnnp_ctx *ctx);
 
 /* Timing of the last evaluation's stages in milliseconds (HIP events on the
  * evaluation stream): [0] wrap+cells, [1] neighbour list, [2] symmetry
- * functions, [3] MLP, [4] gradients, [5] force gather + reductions. */
+ * functions, [3] MLP, [4] gradients, [5] force gather + reductions. After an
+ * evaluation replayed from a graph (hdnnp_set_graphs), [0] holds its whole
+ * device time and the others 0. */
 int hdnnp_last_timings(hdnnp_ctx *ctx, double *ms6);
+
+/* Graph replay of small evaluations (no counterpart in the reference, whose
+ * hdnnpEnergyAndForces, hdnnps.f90:333-371, runs its loops anew on every
+ * call). An evaluation of up to 8,192 atoms on the context's own stream
+ * (hdnnp_evaluate, hdnnp_evaluate_batch, and the device entry points with
+ * stream NULL) is a dozen small kernels. When one has the launch parameters
+ * and buffers of the one before it (same atom counts and cells, same
+ * neighbour capacity, same input and output arrays for the device entry
+ * points), its kernels are captured into a graph; later evaluations with the
+ * launch of any of the last four captures replay it: one launch in place of
+ * a dozen. Results equal those of direct launches bit for bit. on = 1 or 0
+ * (direct launches only, the default unless the environment sets
+ * HDNNP_EVAL_GRAPHS=1; drops the captured graphs). On MI355X with ROCm 7 a
+ * replay measured slower than the direct launches, whose host side overlaps
+ * the kernels: the small evaluations are bound by the kernels' own latency.
+ * hdnnp_graph_counts: the mode, and how many evaluations were captured and
+ * replayed (each may be NULL). */
+int hdnnp_set_graphs(hdnnp_ctx *ctx, int32_t on);
+int hdnnp_graph_counts(const hdnnp_ctx *ctx, int32_t *on, int64_t *captures, int64_t *replays);
 
 void hdnnp_destroy(hdnnp_ctx *ctx);
 
