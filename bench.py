@@ -271,26 +271,35 @@ def bench_batch(args):
                      "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy(), "E": E, "L": L}
         print(f"{mode}: {t * 1e3:.3f} ms per batch -> {nat / t:.4g} atoms/s, {len(ss) / t:.4g} structures/s; "
               f"stage ms {res[mode]['stage_ms']}", file=sys.stderr, flush=True)
-    rmse = None
-    per_kind = None
-    if "fp64" in res and "mixed" in res:
-        rmse = float(np.sqrt(np.mean((res["mixed"]["F"] - res["fp64"]["F"]) ** 2)))
-        per_kind = {}
+    def kind_errors(mode):
+        out = {}
         off = np.concatenate([[0], np.cumsum(nat_s)])
         for kind, periodic in (("periodic", True), ("cluster", False)):
             ks = [k for k in range(len(ss)) if bool(ss[k].periodic) == periodic]
             rows = np.concatenate([np.arange(off[k], off[k + 1]) for k in ks])
-            dF = res["mixed"]["F"][rows] - res["fp64"]["F"][rows]
+            dF = res[mode]["F"][rows] - res["fp64"]["F"][rows]
             e = {"structures": len(ks), "force_rmse": float(np.sqrt(np.mean(dF ** 2))),
                  "force_max": float(np.abs(dF).max()),
-                 "energy_err_per_atom_max": float(max(abs(res["mixed"]["E"][k] - res["fp64"]["E"][k]) / ss[k].nat
+                 "energy_err_per_atom_max": float(max(abs(res[mode]["E"][k] - res["fp64"]["E"][k]) / ss[k].nat
                                                       for k in ks))}
             if periodic:
-                ds = [np.abs(pkg.dEdLat2Stress(ss[k].lat, res["mixed"]["L"][k]) -
+                ds = [np.abs(pkg.dEdLat2Stress(ss[k].lat, res[mode]["L"][k]) -
                              pkg.dEdLat2Stress(ss[k].lat, res["fp64"]["L"][k])).max() for k in ks]
                 e["stress_err_max"] = float(max(ds))
-            per_kind[kind] = e
+            out[kind] = e
+        return out
+
+    rmse = None
+    per_kind = None
+    if "fp64" in res and "mixed" in res:
+        rmse = float(np.sqrt(np.mean((res["mixed"]["F"] - res["fp64"]["F"]) ** 2)))
+        per_kind = kind_errors("mixed")
         print(f"mixed vs fp64 per kind: {per_kind}", file=sys.stderr, flush=True)
+    bf16 = None
+    if "fp64" in res and "mixed-bf16" in res:
+        bf16 = {"force_rmse": float(np.sqrt(np.mean((res["mixed-bf16"]["F"] - res["fp64"]["F"]) ** 2))),
+                "per_kind": kind_errors("mixed-bf16")}
+        print(f"mixed-bf16 vs fp64: {bf16}", file=sys.stderr, flush=True)
     head = "mixed" if ("mixed" in res and rmse is not None and rmse <= FORCE_RMSE_BOUND) else next(iter(res))
     out = {"metric": "atoms/s (E+F+stress), batched independent structures", "value": res[head]["atoms_per_s"],
            "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -299,7 +308,7 @@ def bench_batch(args):
            "data": "synthetic (seeded structures, seeded random-weight potential)",
            "config": {"workload": f"batch of {len(ss)} structures ({nat} atoms): perovskite n1xn2xn3 (n in 1..4) "
                                   f"and 8-64 atom clusters, atoms resident in device memory", "precision": head},
-           "force_rmse_mixed_vs_fp64": rmse, "mixed_vs_fp64": per_kind,
+           "force_rmse_mixed_vs_fp64": rmse, "mixed_vs_fp64": per_kind, "bf16_mlp_vs_fp64": bf16,
            "modes": {m: {k: v for k, v in r.items() if k not in ("F", "E", "L")} for m, r in res.items()}}
     print(json.dumps(out), flush=True)
 
@@ -466,6 +475,7 @@ def main():
     ap.add_argument("--cells", type=int, default=16, help="supercell edge (16 -> 49,152 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cosine", action="store_true", help="skip the cosine-cutoff variant")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-MLP variant (mixed-bf16)")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--workload", default="cell", choices=["cell", "batch", "serve"],
@@ -529,6 +539,16 @@ def main():
             print(f"cosine cutoff, {mode}: {cosine[mode]['atoms_per_s']:.4g} atoms/s", file=sys.stderr, flush=True)
         if "fp64" in cres and "mixed" in cres:
             cosine["mixed_vs_fp64"] = mixed_errors(pkg, s, cres["mixed"], cres["fp64"])
+    # the MLP's forward layers in BF16 (FP32 accumulation), measured against
+    # FP64 beside the FP32 form; it is not the headline unless within bound
+    bf16 = None
+    if not args.no_bf16 and "fp64" in res:
+        _, rb = run_mode(pkg, torch, s, "mixed-bf16", args.steps, args.warmup)
+        eb = mixed_errors(pkg, s, rb, res["fp64"])
+        bf16 = {"atoms_per_s": s.nat * args.steps / rb["wall"], "ms_per_step": rb["wall"] / args.steps * 1e3,
+                "stage_ms": np.round(rb["stage"].mean(axis=0), 4).tolist(), "vs_fp64": eb,
+                "within_force_bound": eb["force_rmse"] <= FORCE_RMSE_BOUND}
+        print(f"mixed-bf16: {bf16['atoms_per_s']:.4g} atoms/s; vs fp64 {eb}", file=sys.stderr, flush=True)
     head = "mixed" if ("mixed" in res and rmse is not None and rmse <= FORCE_RMSE_BOUND) else \
         ("fp64" if "fp64" in res else "mixed")
     r = res[head]
@@ -553,6 +573,7 @@ def main():
         "force_rmse_mixed_vs_fp64": rmse,
         "mixed_vs_fp64": errors,
         "cosine_cutoff": cosine or None,
+        "bf16_mlp": bf16,
         "modes": {m: {"atoms_per_s": s.nat * args.steps / v["wall"],
                       "ms_per_step": v["wall"] / args.steps * 1e3,
                       "stage_ms": np.round(v["stage"].mean(axis=0), 4).tolist()} for m, v in res.items()},

@@ -25,8 +25,8 @@ from . import readers  # noqa: F401  (structure file formats of atomicStructure.
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libhdnnp_amd.so")
 
-FP64, MIXED = 0, 1
-_PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+FP64, MIXED, MIXED_BF16 = 0, 1, 2
+_PREC = {"fp64": FP64, "mixed": MIXED, "mixed-bf16": MIXED_BF16, FP64: FP64, MIXED: MIXED, MIXED_BF16: MIXED_BF16}
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 

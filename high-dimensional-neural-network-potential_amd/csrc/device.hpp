@@ -218,7 +218,8 @@ int sf_tile(bool fp64, int nmax, int ga, int nsfmax, int g4max, int nel);
 // ---- kernels_mlp.hip: 16 atoms per wavefront, maxw = widest layer/input ----
 template <typename R>
 void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, double *Ei, R scale,
-                  bool accumulate, hipStream_t s);
+                  bool accumulate, hipStream_t s,
+                  bool bf16 = false);
 size_t mlp_lds_bytes(bool fp64, const DevModel &m, int maxw);
 // ---- kernels_misc.hip ----
 template <typename R>

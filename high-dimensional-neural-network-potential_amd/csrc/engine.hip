@@ -190,7 +190,8 @@ Engine::Engine(std::vector<HostModel> members, int precision) : precision_(preci
     if (members.empty()) throw Error(1, "no potential given");
     model_ = std::move(members[0]);
     members_.assign(std::make_move_iterator(members.begin() + 1), std::make_move_iterator(members.end()));
-    if (precision_ != 0 && precision_ != 1) throw Error(1, "precision must be HDNNP_FP64 or HDNNP_MIXED");
+    if (precision_ < 0 || precision_ > 2)
+        throw Error(1, "precision must be HDNNP_FP64, HDNNP_MIXED or HDNNP_MIXED_BF16");
     if ((int)model_.el.size() > kMaxEl) throw Error(6, "too many elements for this build");
     if (const char *v = std::getenv("HDNNP_EVAL_GRAPHS")) eval_graphs_on_ = std::strcmp(v, "1") == 0;
     // committee members (globalHdnnps.f90:31-70) reuse the first potential's
@@ -556,7 +557,7 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     for (int k = 0; k < nm; ++k) {
         if (mlp_lds_bytes(sizeof(R) == 8, dmm_[k], maxw_m_[k]) > 65536) throw Error(6, "MLP too wide for this build");
         launch_mlp_t<R>(dmm_[k], w, maxw_m_[k], (nat + 15) / 16 + dm_.nel, G, dEdG, w.Ei + (size_t)k * nat,
-                        R(1) / R(nm), k > 0, s);
+                        R(1) / R(nm), k > 0, s, precision_ == 2);
     }
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[4], s));

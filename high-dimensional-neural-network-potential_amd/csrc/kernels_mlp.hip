@@ -75,6 +75,17 @@ __device__ __forceinline__ float act_d(char a, float x)
     }
 }
 
+// BF16 operands for the forward layers (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation): lane (ri, kq) supplies X[ri][k0 + 4 kq + i] and
+// W[k0 + 4 kq + i][j0 + ri], i = 0..3, and holds D[4 kq + r][j0 + ri] as the
+// FP32 16x16x4 form does. Rounding to nearest even.
+typedef short Bf4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ short bf16_rne(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 template <typename R> __device__ __forceinline__ const R *weights(const DevModel &m, int e);
 template <> __device__ __forceinline__ const double *weights<double>(const DevModel &m, int e) { return m.W64[e]; }
 template <> __device__ __forceinline__ const float *weights<float>(const DevModel &m, int e) { return m.W32[e]; }
@@ -94,8 +105,9 @@ struct MlpShape {
 constexpr int kKU = 8;   // matrix-core steps whose operands are loaded together
 
 // Ei_out[i] = E_i of this network; dEdG = scale * dE_i/dG, or added to dEdG
-// when accumulate is set (committee mean over several networks).
-template <typename R>
+// when accumulate is set (committee mean over several networks). BF16 (FP32
+// only): the forward layers take BF16 operands; the backward pass stays FP32.
+template <typename R, bool BF16>
 __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, const R *G, R *dEdG, double *Ei_out,
                                             R scale, int accumulate)
 {
@@ -158,18 +170,37 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         for (int j0 = 0; j0 < nout; j0 += 16) {
             V acc = {0, 0, 0, 0};
             const int j = j0 + ri;
-            // kKU steps at a time, operands loaded first; steps past nin
-            // multiply zeros, which leaves the accumulator as it is
-            for (int k0 = 0; k0 < nin; k0 += 4 * kKU) {
-                R a[kKU], b[kKU];
+            if constexpr (BF16 && sizeof(R) == 4) {
+                // 16 k per instruction, two instructions per step
+                for (int k0 = 0; k0 < nin; k0 += 32) {
+                    Bf4 a[2], b[2];
 #pragma unroll
-                for (int u = 0; u < kKU; ++u) {
-                    const int k = k0 + 4 * u + kq;
-                    a[u] = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
-                    b[u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int k = k0 + 16 * u + 4 * kq + i;
+                            const R x = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
+                            const R y = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                            a[u][i] = bf16_rne(x);
+                            b[u][i] = bf16_rne(y);
+                        }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
                 }
+            } else {
+                // kKU steps at a time, operands loaded first; steps past nin
+                // multiply zeros, which leaves the accumulator as it is
+                for (int k0 = 0; k0 < nin; k0 += 4 * kKU) {
+                    R a[kKU], b[kKU];
 #pragma unroll
-                for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
+                    for (int u = 0; u < kKU; ++u) {
+                        const int k = k0 + 4 * u + kq;
+                        a[u] = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
+                        b[u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
+                }
             }
             if (j < nout) {
                 const R bj = bias[j];
@@ -273,7 +304,7 @@ size_t mlp_lds_bytes(bool fp64, const DevModel &m, int)
 
 template <typename R>
 void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G, R *dEdG, double *Ei, R scale,
-                  bool accumulate, hipStream_t s)
+                  bool accumulate, hipStream_t s, bool bf16)
 {
     MlpShape sh{};
     sh.nlayers = m.nlayers;
@@ -288,13 +319,18 @@ void launch_mlp_t(const DevModel &m, Work &w, int maxw, int nblocks, const R *G,
         off += 16 * sh.ldc[l];
     }
     sh.total = 2 * 16 * sh.ld + off;
-    hipLaunchKernelGGL(k_mlp<R>, dim3(nblocks), dim3(64), sizeof(R) * (size_t)sh.total + 16 * sizeof(int), s, m, w,
-                       sh, G, dEdG, Ei, scale, accumulate ? 1 : 0);
+    const size_t lds = sizeof(R) * (size_t)sh.total + 16 * sizeof(int);
+    if (bf16 && sizeof(R) == 4)
+        hipLaunchKernelGGL((k_mlp<R, true>), dim3(nblocks), dim3(64), lds, s, m, w, sh, G, dEdG, Ei, scale,
+                           accumulate ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_mlp<R, false>), dim3(nblocks), dim3(64), lds, s, m, w, sh, G, dEdG, Ei, scale,
+                           accumulate ? 1 : 0);
 }
 
 template void launch_mlp_t<double>(const DevModel &, Work &, int, int, const double *, double *, double *, double,
-                                   bool, hipStream_t);
+                                   bool, hipStream_t, bool);
 template void launch_mlp_t<float>(const DevModel &, Work &, int, int, const float *, float *, double *, float, bool,
-                                  hipStream_t);
+                                  hipStream_t, bool);
 
 }  // namespace hdnnp

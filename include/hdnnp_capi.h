@@ -28,9 +28,12 @@ typedef struct hdnnp_ctx hdnnp_ctx;
 
 /* arithmetic of an evaluation */
 enum {
-    HDNNP_FP64 = 0,   /* everything in FP64                                      */
-    HDNNP_MIXED = 1   /* wrap/list/energy/virial FP64, symmetry functions FP32,  */
-                      /* MLP FP32                                                */
+    HDNNP_FP64 = 0,        /* everything in FP64                                 */
+    HDNNP_MIXED = 1,       /* wrap/list/energy/virial FP64, symmetry functions   */
+                           /* FP32, MLP FP32                                     */
+    HDNNP_MIXED_BF16 = 2   /* as HDNNP_MIXED, with BF16 operands (FP32           */
+                           /* accumulation) in the MLP's forward layers; its     */
+                           /* backward pass stays FP32                           */
 };
 
 /* status codes */
