@@ -57,6 +57,7 @@ struct DevModel {
     int g2_start[kMaxEl], g2_count[kMaxEl];
     int g4_start[kMaxEl], g4_count[kMaxEl];
     int g4_max;              // largest g4_count
+    int g4_lampm;            // 1 when every G4 lambda is +1 or -1 and every zeta >= 1
     int powbits;             // bits of the largest |g4_ipow| of integral zeta
     const ElemInfo *einfo;   // [nel]
     const double *sf_mean;   // [sf_off[e] + s]: scaled G = (G - sf_mean) / sf_den
@@ -196,7 +197,7 @@ void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga
 template <typename R>
 void launch_sf_grad(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, const R *dEdG, R *gent,
                     hipStream_t s);
-size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad);
+size_t sf_lds_bytes(bool fp64, int nmax, const DevModel &m, bool grad);
 // The per-function path (symmetryFunction2 / symmetryFunction3 of the
 // reference): for every atom and symmetry function, the centred / scaled
 // value G[i * ldG + s] and, when dsf is not null, its gradient rows

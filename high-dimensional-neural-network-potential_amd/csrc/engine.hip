@@ -384,6 +384,9 @@ void Engine::init_device()
             if (ip != kNonIntegralZeta) maxp = std::max(maxp, std::abs(ip));
         d.powbits = 0;
         while ((1 << d.powbits) <= maxp) ++d.powbits;
+        d.g4_lampm = 1;
+        for (size_t k = 0; k < g4lam.size(); ++k)
+            if (!(g4lam[k] == 1.0 || g4lam[k] == -1.0) || !(g4zeta[k] >= 1.0)) d.g4_lampm = 0;
         auto tof = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         d.g2f_eta = upload(model_mem_, tof(g2eta));
         d.g2f_rs = upload(model_mem_, tof(g2rs));
@@ -747,8 +750,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     const bool fp64 = precision_ == 0;
     int lds_lim = 0;
     for (int c : kStage)
-        if (sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, true) <= 65536 &&
-            sf_lds_bytes(fp64, c, dm_.nsfmax, dm_.g4_max, dm_.nel, false) <= 65536)
+        if (sf_lds_bytes(fp64, c, dm_, true) <= 65536 && sf_lds_bytes(fp64, c, dm_, false) <= 65536)
             lds_lim = c;
     if (lds_lim == 0) throw Error(6, "symmetry-function tables too large for this build");
     long cell_off = 0, ps_total = 0;

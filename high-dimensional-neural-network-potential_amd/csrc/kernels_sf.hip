@@ -294,6 +294,7 @@ __device__ __forceinline__ int mbcnt(unsigned long long m)
 
 template <typename R> struct Vec4 { typedef R T __attribute__((ext_vector_type(4))); };
 typedef int I2 __attribute__((ext_vector_type(2)));
+typedef float F2 __attribute__((ext_vector_type(2)));
 
 // Atoms per wavefront (GA, the tile): several atoms of one element can share
 // a wavefront (64/GA lanes each; the SF parameters stay wave-uniform), which
@@ -954,9 +955,12 @@ __device__ __forceinline__ void block_table(const DevModel &m, const Lds<R> &L, 
 }
 
 // G4 tables of the centre element for the all-pairs pass, with the weights
-// w = dE/dG of the raw G (in acc) folded in.
+// w = dE/dG of the raw G (in acc) folded in: sp0 = (exp scale, lambda, q1 =
+// w P, q2 = w P 2 eta), sp1 = q3 = w P lambda zeta, smeta = sq | new-eta << 8,
+// or (logf) the bits of zeta - 1
 template <typename R>
-__device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, int e, int g4s, int g4n)
+__device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, int e, int g4s, int g4n,
+                                            bool logf = false)
 {
     const int lane = lane_id();
     for (int u = lane; u < g4n; u += 64) {
@@ -971,7 +975,7 @@ __device__ __forceinline__ void grad_tables(const DevModel &m, const Lds<R> &L, 
         L.sp0[u] = p;
         L.sp1[u] = q1 * lam * zeta;
         const int sq = cld(m.g4_sq, x);
-        L.smeta[u] = (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0);
+        L.smeta[u] = logf ? __float_as_int((float)(zeta - R(1))) : (sq >= 0 ? sq : 0xFF) | (cld(m.g4_neweta, x) ? 1 << 8 : 0);
     }
     block_table<R>(m, L, e, g4s);
 }
@@ -1072,7 +1076,12 @@ __device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &
     }
 }
 
-template <typename R, int CT>
+__device__ __forceinline__ float log2_base(float b);
+
+// LOGF (FP32, every lambda = +-1): b^(zeta-1) exp(-eta R2) as one
+// exponential of (zeta - 1) log2(b) + ea R2, with log2(1 +- cos) once per
+// triplet (see the row-sweep values pass), no branch on zeta
+template <typename R, int CT, bool LOGF = false>
 __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
 {
     const int ct = CT >= 0 ? CT : m.cutoff_type;
@@ -1093,6 +1102,22 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
         const R F = fcij * fcik * fcjk;
         const R r2 = bj.w + bk.w + rjk2;
         R S1 = 0, S2 = 0, S3 = 0, E = 0;
+        if constexpr (LOGF) {
+#pragma clang fp contract(off)
+            const float bp = 1.0f + cs, bm = 1.0f - cs;
+            const float Lp = log2_base(bp), Lm = log2_base(bm);
+            for (int t = 0; t < bt.y; ++t) {
+                const int u = bt.x + t;
+                const typename Lds<R>::V p = L.sp0[u];
+                const float zm = __int_as_float(L.smeta[u]);
+                const bool pos = p.y > 0.0f;
+                const float EB = __builtin_amdgcn_exp2f(fmaR(zm, pos ? Lp : Lm, p.x * r2));   // b^(zeta-1) E
+                const float C = EB * (pos ? bp : bm);                                        // b^zeta E
+                S1 = fmaR(C, p.z, S1);
+                S2 = fmaR(C, p.w, S2);
+                S3 = fmaR(EB, L.sp1[u], S3);
+            }
+        } else
         for (int t = 0; t < bt.y; ++t) {
             const int u = bt.x + t;
             const typename Lds<R>::V p = L.sp0[u];
@@ -1139,6 +1164,13 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
 template <typename R>
 __device__ __forceinline__ void g4_grad_list_ct(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
 {
+    if constexpr (sizeof(R) == 4) {
+        if (m.g4_lampm) {
+            if (m.cutoff_type == 2) g4_grad_list<R, 2, true>(m, L, nv, g4s, rrc);
+            else g4_grad_list<R, -1, true>(m, L, nv, g4s, rrc);
+            return;
+        }
+    }
     if (m.cutoff_type == 2) g4_grad_list<R, 2>(m, L, nv, g4s, rrc);
     else g4_grad_list<R, -1>(m, L, nv, g4s, rrc);
 }
@@ -1276,7 +1308,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
     __syncthreads();
     // G4
     if (GA == 1) {
-        grad_tables<R>(m, L, e, ei.g4_start, ei.g4_count);
+        grad_tables<R>(m, L, e, ei.g4_start, ei.g4_count, sizeof(R) == 4 && m.g4_lampm);
         __syncthreads();
         const R rce2 = rce * rce, rrc = R(1) / rce;
         const int tc = w.trip ? uni(w.tcount[i]) : -1;
@@ -1326,6 +1358,269 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
     }
 }
 
+// ---------------------------------------------------------------- FP32 row-sweep values pass
+// Pass A in FP32 with one atom per wavefront when every G4 lambda is +-1 (so
+// 1 + lambda cos >= 0) and every zeta >= 1. It takes the unordered pairs of
+// the staged neighbours in circulant order, as rows: the pair (j, j + d mod
+// n) belongs to neighbour j for d = 1..(n-1)/2, and for even n also (j, j +
+// n/2) for j < n/2. A lane owns a neighbour j and sweeps its d (for more
+// than 64 lanes' worth of pairs the d-range is cut into chunks, item c*n +
+// j); at step d the lanes read rows k = j + d, distinct and consecutive, so
+// the LDS reads have no bank conflicts, and j's own row stays in registers.
+// Pairs with r_jk >= rc, or of a block without G4 functions, add nothing.
+// The staged neighbours are grouped by element, so along a sweep the block
+// (el j, el k) changes at most nel + 1 times: the lane keeps the block's
+// parameters (up to kRowFns functions) and its term sums in registers, and
+// adds the sums to the FP64 function sums (LDS atomics) when the block
+// changes -- not one LDS update per term. The angular and radial factors of
+// a function are one exponential:
+//   2^(1-zeta) (1 + lambda cos)^zeta exp(-eta R2) = exp2(zeta L + (1 - zeta) + ea R2)
+// with L = log2(1 + lambda cos) (two logarithms per triplet, one per sign of
+// lambda; L >= -126, so b = 0 gives 0 for zeta > 1), R2 = r_ij^2 + r_ik^2 +
+// r_jk^2 and ea = -eta log2(e): no branch on zeta, and any real zeta >= 1.
+// Functions of a block beyond kRowFns read their parameters from LDS per
+// triplet. The triplets with r_jk < rc go to the atom's list for pass B (in
+// the sweep's order). The wavefront of atom b runs on XCD b % 8: xcd_block
+// gives each XCD a contiguous range of atoms, whose neighbours overlap in
+// its L2.
+constexpr int kRowFns = 4;
+
+struct DirectOffsets {
+    size_t a, b, el, ent, seg, accd, tab0, btab, total;
+};
+
+__host__ __device__ __forceinline__ DirectOffsets direct_offsets(int nmax, int nsfmax, int g4max, int nel)
+{
+    DirectOffsets o{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 15) / 16 * 16;
+        return at;
+    };
+    o.a = take(16 * (size_t)nmax);
+    o.b = take(16 * (size_t)nmax);
+    o.el = take((size_t)nmax);
+    o.ent = take(2 * (size_t)nmax);
+    o.seg = take(4 * (kMaxEl + 1));
+    o.accd = take(8 * (size_t)nsfmax);   // G sums (FP64)
+    o.tab0 = take(16 * (size_t)g4max);
+    o.btab = take(8 * (size_t)nel * (size_t)nel);
+    o.total = off;
+    return o;
+}
+
+// the stage() view of the row-sweep pass's LDS
+__device__ __forceinline__ Lds<float> direct_lds(const DirectOffsets &o, LDS char *base)
+{
+    Lds<float> L{};
+    L.a = (LDS Vec4<float>::T *)(base + o.a);
+    L.b = (LDS Vec4<float>::T *)(base + o.b);
+    L.el = (LDS unsigned char *)(base + o.el);
+    L.ent = (LDS unsigned short *)(base + o.ent);
+    L.seg = (LDS int *)(base + o.seg);
+    L.btab = (LDS I2 *)(base + o.btab);
+    L.accd = (LDS double *)(base + o.accd);
+    return L;
+}
+
+// log2(b) of b = 1 + lambda cos, b clamped to [0, ...) against rounding and
+// the logarithm to -126 (so that 0 * L = 0)
+__device__ __forceinline__ float log2_base(float b)
+{
+    return __builtin_fmaxf(__builtin_amdgcn_logf(__builtin_fmaxf(b, 0.0f)), -126.0f);
+}
+
+// The sweep of n staged neighbours: Rt rows of d (the last, for even n, only
+// for j < n/2), cut into C chunks of len rows, items = C n, in passes of 64.
+// C (1..8) gives the fewest passes * len.
+struct Sweep {
+    int n, Rt, C, len, items;
+    float inv_n;
+};
+__device__ __forceinline__ Sweep make_sweep(int n)
+{
+    Sweep s;
+    s.n = n;
+    s.Rt = n / 2;   // (n-1)/2 rows, plus the n/2 row for even n
+    s.C = 1;
+    int best = 1 << 30;
+    for (int c = 1; c <= 8; ++c) {
+        const int len = (s.Rt + c - 1) / c;
+        const int cost = ((n * c + 63) / 64) * len;
+        if (cost < best) {
+            best = cost;
+            s.C = c;
+        }
+    }
+    s.len = (s.Rt + s.C - 1) / s.C;
+    s.items = n * s.C;
+    s.inv_n = 1.0f / (float)max(n, 1);
+    return s;
+}
+// item -> (j, first d, last d)
+__device__ __forceinline__ void sweep_item(const Sweep &s, int item, int &j, int &d0, int &d1)
+{
+    const int c = row_of(item, s.inv_n);
+    j = item - __mul24(c, s.n);
+    d0 = 1 + c * s.len;
+    d1 = min(s.Rt, d0 + s.len - 1);
+    // the row d = n/2 of even n holds (j, j + n/2) for j < n/2 only
+    if ((s.n & 1) == 0 && d1 == s.Rt && j >= s.n / 2) --d1;
+}
+
+// block (el j, el k): (first local function, count)
+__device__ __forceinline__ I2 row_block(const Lds<float> &L, int ej, int ek, int nel)
+{
+    return L.btab[min(ej, ek) * nel + max(ej, ek)];
+}
+
+template <int NMAX, int CT>
+__global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work w, float *G)
+{
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) char smem[];
+    const int i = xcd_block(blockIdx.x, w.nat);
+    const int lane = lane_id();
+    const int e = uni(w.eidx[i]);
+    const ElemInfo ei = cld_einfo(m.einfo, e);
+    const int nsf = ei.nsf, g4s = ei.g4_start, g4n = ei.g4_count;
+    const int ct = CT >= 0 ? CT : m.cutoff_type;
+    const DirectOffsets o = direct_offsets(NMAX, m.nsfmax, m.g4_max, m.nel);
+    const Lds<float> L = direct_lds(o, (LDS char *)smem);
+    LDS Vec4<float>::T *tab = (LDS Vec4<float>::T *)((LDS char *)smem + o.tab0);
+    for (int s = lane; s < nsf; s += 64) L.accd[s] = 0;
+    // (ea, zeta, lambda, G slot) of each local G4 function
+    for (int u = lane; u < g4n; u += 64) {
+        const int x = g4s + u;
+        tab[u] = Vec4<float>::T{cld(m.g4f_ea, x), cld(m.g4f_zeta, x), cld(m.g4f_lam, x),
+                                __int_as_float(cld(m.g4_s, x))};
+    }
+    block_table<float>(m, L, e, g4s);
+    stage<float, 64, NMAX, true>(m, g, w, i, wave_max(w.count[i]), ei.rc, L, (float *)nullptr);
+    __syncthreads();
+    const float rce = (float)ei.rc;
+
+    // G2, symmetryFunctions.f90:220-230: one lane per SF of the centre element
+    for (int t = lane; t < ei.g2_count; t += 64) {
+        const int x = ei.g2_start + t;
+        const int f = cld(m.g2_f, x);
+        const float ea = g2_ea<float>(m, x), rs = cld(m.g2f_rs, x);
+        float sum = 0;
+        for (int q = L.seg[f]; q < L.seg[f + 1]; ++q) {
+            const Vec4<float>::T b = L.b[q];
+            const float dr = b.x - rs;
+            sum = fmaR(M<float>::expsc(ea, dr * dr), b.y, sum);
+        }
+        L.accd[cld(m.g2_s, x)] += (double)sum;
+    }
+    __syncthreads();
+    // G4, symmetryFunctions.f90:231-305, over the rows of the staged neighbours
+    const Sweep sw = make_sweep(uni(L.seg[kMaxEl]));
+    const float rrc = 1.0f / rce, rc2 = rce * rce;
+    int *tl = w.trip ? w.trip + (size_t)i * w.tcap : nullptr;   // the triplets, for pass B
+    int tbase = 0;
+    for (int p0 = 0; p0 < sw.items; p0 += 64) {
+        const int item = p0 + lane;
+        int j = 0, d0 = 1, d1 = 0;
+        if (item < sw.items) sweep_item(sw, item, j, d0, d1);
+        const Vec4<float>::T aj = L.a[j], bj = L.b[j];
+        const int ej = L.el[j];
+        int hcur = -1, cnt = 0, u0 = 0, blk = 0;
+        float ea[kRowFns], ze[kRowFns], cz[kRowFns], lam[kRowFns], acc[kRowFns];
+        int slot[kRowFns];
+#pragma unroll
+        for (int t = 0; t < kRowFns; ++t) ea[t] = ze[t] = cz[t] = lam[t] = acc[t] = 0, slot[t] = 0;
+        for (int d = d0; d <= d1; ++d) {
+            int k = j + d;
+            if (k >= sw.n) k -= sw.n;
+            const int ek = L.el[k];
+            if (ek != hcur) {   // a new block: flush the sums, load its functions
+#pragma unroll
+                for (int t = 0; t < kRowFns; ++t)
+                    if (t < cnt) __hip_atomic_fetch_add(&L.accd[slot[t]], (double)acc[t], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                hcur = ek;
+                blk = min(ej, ek) * m.nel + max(ej, ek);
+                const I2 bt = L.btab[blk];
+                u0 = bt.x;
+                cnt = bt.y;
+#pragma unroll
+                for (int t = 0; t < kRowFns; ++t) {
+                    const Vec4<float>::T P = tab[min(u0 + t, g4n - 1)];
+                    ea[t] = P.x;
+                    ze[t] = P.y;
+                    cz[t] = 1.0f - P.y;
+                    lam[t] = P.z;
+                    slot[t] = __float_as_int(P.w);
+                    acc[t] = 0;
+                }
+            }
+            Vec4<float>::T ak{}, bk{};
+            float rjk2 = 0;
+            bool valid = false;
+            if (cnt > 0) {
+                ak = L.a[k];
+                bk = L.b[k];
+                const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
+                rjk2 = fmaR(ez, ez, fmaR(ey, ey, ex * ex));
+                valid = rjk2 < rc2;
+            }
+            if (tl) {   // in the order of the sweep's steps; lanes active here have all seen the same steps
+                const unsigned long long mk = __ballot(valid);
+                const int pos = tbase + mbcnt(mk);
+                if (valid && pos < w.tcap) tl[pos] = j | (k << 10) | (blk << 20);
+                tbase += __popcll(mk);
+            }
+            if (!valid) continue;
+            const float fcjk = cutoff_in<float>(ct, M<float>::sqrt(rjk2), rrc);
+            const float cs = (fmaR(aj.z, ak.z, fmaR(aj.y, ak.y, aj.x * ak.x)) * aj.w) * ak.w;
+            const float F = (bj.y * bk.y) * fcjk;
+            const float r2 = (bj.w + bk.w) + rjk2;
+            const float Lp = log2_base(1.0f + cs), Lm = log2_base(1.0f - cs);
+#pragma unroll
+            for (int t = 0; t < kRowFns; ++t) {
+                if (t >= cnt) break;
+                const float Ls = lam[t] > 0.0f ? Lp : Lm;
+                acc[t] = fmaR(__builtin_amdgcn_exp2f(fmaR(ze[t], Ls, fmaR(ea[t], r2, cz[t]))), F, acc[t]);
+            }
+            for (int t = kRowFns; t < cnt; ++t) {   // functions beyond the registers' (not in common models)
+                const Vec4<float>::T P = tab[u0 + t];
+                const float Ls = P.z > 0.0f ? Lp : Lm;
+                const float T = __builtin_amdgcn_exp2f(fmaR(P.y, Ls, fmaR(P.x, r2, 1.0f - P.y))) * F;
+                __hip_atomic_fetch_add(&L.accd[__float_as_int(P.w)], (double)T, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kRowFns; ++t)
+            if (t < cnt) __hip_atomic_fetch_add(&L.accd[slot[t]], (double)acc[t], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+        tbase = wave_max(tbase);   // the lanes that stepped longest saw every step
+    }
+    if (tl && lane == 0) w.tcount[i] = tbase;
+    __syncthreads();
+    // centre and scale, hdnnps.f90:471-481
+    const int so = ei.sf_off;
+    for (int s = lane; s < m.nsfmax; s += 64)
+        G[(size_t)i * m.nsfmax + s] = s < nsf ? (float)((L.accd[s] - m.sf_mean[so + s]) / m.sf_den[so + s]) : 0.0f;
+}
+
+bool sf_allpairs_enabled();
+int sf_bucket(int nmax);
+
+// the direct FP32 passes serve this model, tile and staging size (both fit
+// 64 KiB of LDS)
+static bool sf_direct(const DevModel &m, int ga, int nmax)
+{
+    static const bool on = [] {   // HDNNP_SF_DIRECT=0 in the environment selects the compacting passes
+        const char *v = std::getenv("HDNNP_SF_DIRECT");
+        return !(v && std::strcmp(v, "0") == 0);
+    }();
+    return on && ga == 1 && m.g4_lampm && sf_allpairs_enabled() &&
+           direct_offsets(nmax, m.nsfmax, m.g4_max, m.nel).total <= 65536;
+}
+
 // wavefronts of a launch: tiles of GA atoms of one element
 static int sf_blocks(const DevModel &m, const Work &w, int ga) { return ga == 1 ? w.nat : (w.nat + ga - 1) / ga + m.nel; }
 
@@ -1343,6 +1638,28 @@ bool sf_allpairs_enabled()
 template <typename R>
 void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, R *G, hipStream_t s)
 {
+    if constexpr (sizeof(R) == 4) {
+        const int nb = sf_bucket(nmax);
+        if (sf_direct(m, ga, nb)) {
+            const size_t lds = direct_offsets(nb, m.nsfmax, m.g4_max, m.nel).total;
+#define HDNNP_SFD(N)                                                                                     \
+    do {                                                                                                 \
+        if (m.cutoff_type == 2)                                                                          \
+            hipLaunchKernelGGL((k_sf_values_direct<N, 2>), dim3(w.nat), dim3(64), lds, s, m, g, w, G);   \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_sf_values_direct<N, -1>), dim3(w.nat), dim3(64), lds, s, m, g, w, G);  \
+    } while (0)
+            if (w.nat == 0) return;
+            if (nb == 64) HDNNP_SFD(64);
+            else if (nb == 96) HDNNP_SFD(96);
+            else if (nb == 128) HDNNP_SFD(128);
+            else if (nb == 256) HDNNP_SFD(256);
+            else if (nb == 512) HDNNP_SFD(512);
+            else HDNNP_SFD(1024);
+#undef HDNNP_SFD
+            return;
+        }
+    }
     const bool allp = ga == 1 && sf_allpairs_enabled();
 #define HDNNP_SFV(N, A)                                                                            \
     do {                                                                                           \
@@ -1812,9 +2129,11 @@ template void launch_sf_single<float>(const DevModel &, const Geo &, const Work 
 
 // LDS of one wavefront of the kernels serving nmax neighbours with one atom
 // per wavefront (this decides the staging capacity limit)
-size_t sf_lds_bytes(bool fp64, int nmax, int nsfmax, int g4max, int nel, bool grad)
+size_t sf_lds_bytes(bool fp64, int nmax, const DevModel &m, bool grad)
 {
     nmax = sf_bucket(nmax);
+    const int nsfmax = m.nsfmax, g4max = m.g4_max, nel = m.nel;
+    if (!fp64 && !grad && sf_direct(m, 1, nmax)) return direct_offsets(nmax, nsfmax, g4max, nel).total;
     if (!grad && sf_allpairs_enabled())
         return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, nel, false, true)
                     : lds_bytes<float>(nmax, 1, nsfmax, g4max, nel, false, true);
