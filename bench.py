@@ -574,6 +574,10 @@ def main():
         "mixed_vs_fp64": errors,
         "cosine_cutoff": cosine or None,
         "bf16_mlp": bf16,
+        # the mixed form's rate over the FP64 form's on this GPU (both the same
+        # fused two-pass design; the 20x aim assumed a plain FP64 port)
+        "mixed_over_fp64": (s.nat * args.steps / res["mixed"]["wall"]) / (s.nat * args.steps / res["fp64"]["wall"])
+        if ("mixed" in res and "fp64" in res) else None,
         "modes": {m: {"atoms_per_s": s.nat * args.steps / v["wall"],
                       "ms_per_step": v["wall"] / args.steps * 1e3,
                       "stage_ms": np.round(v["stage"].mean(axis=0), 4).tolist()} for m, v in res.items()},
