@@ -14,6 +14,7 @@ force RMSE against the FP64 path is <= 1e-5 Ha/bohr, else the FP64 rate.
     python bench.py --calibrate-copy   # plain 512 MiB device copy, for HBM counter calibration
     python bench.py --workload batch   # 4096 independent structures per evaluation
     python bench.py --workload serve   # request server: Poisson stream + 64 MD sessions
+    python bench.py --workload molecule   # the example's 8-atom molecule: call latency, MD step
 
 `roofline.traffic` (HBM bytes per evaluation of the three symmetry-function /
 MLP kernels) is read from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
@@ -313,6 +314,53 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_molecule(args):
+    """The reference example's 8-atom methylammonium molecule with its own
+    potential (the example's RuNNer files): latency of one host-array
+    evaluation (hdnnp_evaluate) and time per velocity-Verlet step on the
+    device (dt 0.5 fs, 140 K), without and with a 1-bohr Verlet skin."""
+    import torch
+    torch.cuda.is_available()
+    pkg = load_package()
+    s = pkg.workloads.methylammonium()
+    params = os.path.join(ROOT, "tests", "golden", "methylammonium")
+    rng = np.random.default_rng(3)
+    kT = 140.0 * 3.166811563e-6
+    mom = np.sqrt(pkg.atomic_masses(s.Z) * kT)[:, None] * rng.standard_normal((s.nat, 3))
+    res = {}
+    for mode in args.modes.split(","):
+        h = pkg.parseRuNNerInput(params, mode)
+        for _ in range(max(args.warmup, 20)):
+            h.energy_and_forces(s)
+        lat = []
+        for _ in range(max(args.steps, 200)):
+            a = time.perf_counter()
+            h.energy_and_forces(s)
+            lat.append((time.perf_counter() - a) * 1e3)
+        r = {"call_p50_ms": float(np.percentile(lat, 50)), "call_p99_ms": float(np.percentile(lat, 99))}
+        for skin in (0.0, 1.0):
+            m = pkg.MdSystems(h, [s], [mom], 0.5 * 41.34137333519408, skin=skin)
+            m.step(50)
+            a = time.perf_counter()
+            m.step(1000)
+            r[f"md_step_ms_skin{skin:g}"] = (time.perf_counter() - a)
+            m.close()
+        res[mode] = r
+        print(f"{mode}: call p50 {r['call_p50_ms'] * 1e3:.1f} us, p99 {r['call_p99_ms'] * 1e3:.1f} us; MD step "
+              f"{r['md_step_ms_skin0'] * 1e3:.1f} us (no skin), {r['md_step_ms_skin1'] * 1e3:.1f} us (1-bohr skin)",
+              file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in res else next(iter(res))
+    out = {"metric": "latency of one E+F evaluation (host arrays)", "value": res[head]["call_p50_ms"], "unit": "ms",
+           "n_gpus": 1, "steps": max(args.steps, 200), "warmup": max(args.warmup, 20),
+           "ms_per_step": res[head]["call_p50_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if head == "fp64" else "f32 (symmetry functions, MLP); f64 (wrap, list, sums)",
+           "data": "the reference example's molecule and potential",
+           "config": {"workload": "methylammonium CH3NH3+ (8 atoms, nonperiodic), the example's RuNNer potential; "
+                                  "1000 MD steps of dt 0.5 fs per MD figure", "precision": head},
+           "modes": res}
+    print(json.dumps(out), flush=True)
+
+
 def serve_window(pkg, mode, pool, md_cells, args):
     """One serving run: a Poisson stream of evaluation requests drawn from
     `pool`, and MD sessions (one client thread each, args.md_steps steps per
@@ -478,9 +526,10 @@ def main():
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-MLP variant (mixed-bf16)")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--workload", default="cell", choices=["cell", "batch", "serve"],
+    ap.add_argument("--workload", default="cell", choices=["cell", "batch", "serve", "molecule"],
                     help="cell: one perovskite supercell (headline); batch: many independent structures; "
-                         "serve: request server under a Poisson stream with MD sessions")
+                         "serve: request server under a Poisson stream with MD sessions; molecule: the "
+                         "reference example's 8-atom molecule (call latency, MD step)")
     ap.add_argument("--nstruct", type=int, default=4096, help="structures of the batch workload")
     ap.add_argument("--pool", type=int, default=128, help="serve: distinct request structures")
     ap.add_argument("--sessions", type=int, default=64, help="serve: MD sessions")
@@ -493,11 +542,11 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload in ("batch", "serve"):
+    if args.workload in ("batch", "serve", "molecule"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        (bench_batch if args.workload == "batch" else bench_serve)(args)
+        {"batch": bench_batch, "serve": bench_serve, "molecule": bench_molecule}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of this "
