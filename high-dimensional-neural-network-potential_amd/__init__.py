@@ -40,7 +40,8 @@ EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_eval
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
            "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_set_graphs", "hdnnp_graph_counts", "hdnnp_destroy",
            "hdnnp_set_tile", "hdnnp_get_tile", "hdnnp_autotune",
-           "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_md_open",
+           "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_cancel",
+           "hdnnp_server_md_open",
            "hdnnp_server_md_step", "hdnnp_server_md_wait", "hdnnp_server_md_get", "hdnnp_server_md_close",
            "hdnnp_server_stats", "hdnnp_server_last_error", "hdnnp_server_destroy")
 
@@ -48,6 +49,7 @@ SERVER_STATS = ("evaluations", "eval_atoms", "md_requests", "md_atom_steps", "ev
                 "md_p50_ms", "md_p99_ms", "batches", "mean_batch", "peak_reserved_bytes", "budget_bytes",
                 "seconds", "max_busy_lanes", "refused", "md_steps")
 HDNNP_ERR_TIMEOUT = 7
+HDNNP_ERR_CANCELLED = 8
 
 
 class HdnnpError(RuntimeError):
@@ -131,6 +133,7 @@ def load_library():
                                             ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(vp)]
         lib.hdnnp_server_submit.argtypes = [vp, ctypes.c_int32, _ip, _dp, ctypes.c_int32, _dp, i64p]
         lib.hdnnp_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+        lib.hdnnp_server_cancel.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
         lib.hdnnp_server_md_open.argtypes = [vp, ctypes.c_int32, _ip, _dp, _dp, _dp, ctypes.c_int32, _dp,
                                              ctypes.c_double, ctypes.c_double, i64p]
         lib.hdnnp_server_md_step.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, i64p]
@@ -638,6 +641,14 @@ class Server:
             self._nat.pop(ticket, None)
         self._check(rc)
         return e.value, f, d, sd.value, lat.value
+
+    def cancel(self, ticket: int) -> bool:
+        """Cancel a queued evaluation or MD-step request: True when it had not
+        started (waiting for it then raises HdnnpError with code
+        HDNNP_ERR_CANCELLED), False when it runs or is done."""
+        c = ctypes.c_int32()
+        self._check(self.lib.hdnnp_server_cancel(self.sv, int(ticket), ctypes.byref(c)))
+        return bool(c.value)
 
     def md_open(self, s, momenta, dt: float, skin: float = 1.0, masses=None) -> int:
         """Open an MD session of structure s with momenta (nat, 3); its id."""

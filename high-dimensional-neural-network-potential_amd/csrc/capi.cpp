@@ -530,6 +530,12 @@ int hdnnp_server_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double
     return rc;
 }
 
+int hdnnp_server_cancel(hdnnp_server *sv, int64_t ticket, int32_t *cancelled)
+{
+    if (!sv || !cancelled) return HDNNP_ERR_ARG;
+    return sguarded([&] { *cancelled = sv->sv->cancel(ticket) ? 1 : 0; });
+}
+
 int hdnnp_server_md_open(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, const double *mom,
                          const double *mass, int32_t periodic, const double *lat, double dt, double skin,
                          int64_t *session)

@@ -497,6 +497,21 @@ bool Server::wait(long long ticket, double timeout_s, double *energy, double *fo
     return true;
 }
 
+bool Server::cancel(long long ticket)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(ticket);
+    if (it == jobs_.end()) throw Error(1, "unknown ticket " + std::to_string(ticket));
+    const std::shared_ptr<Job> jp = it->second;
+    if (jp->done) return false;
+    auto q = std::find(queue_.begin(), queue_.end(), jp);
+    if (q == queue_.end()) return false;   // a lane runs it
+    queue_.erase(q);
+    finish_job(*jp, kCancelled, "request cancelled");
+    cv_work_.notify_all();                 // work queued behind it may fit now
+    return true;
+}
+
 long long Server::md_open(int nat, const int *Z, const double *pos, const double *mom, const double *mass,
                           int periodic, const double *lat, double dt, double skin)
 {

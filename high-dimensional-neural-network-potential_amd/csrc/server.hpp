@@ -61,6 +61,10 @@ class Server {
     // committee's standard deviation of the energy (0 for one potential)
     bool wait(long long ticket, double timeout_s, double *energy, double *force, double *dEdLat, double *energy_std,
               double *latency_ms);
+    // cancel a queued evaluation or MD-step request: true when it had not
+    // started (its wait then throws kCancelled), false when it runs or is done
+    bool cancel(long long ticket);
+    static constexpr int kCancelled = 8;
 
     // open an MD session (opened on a lane once it fits the budget); its id
     long long md_open(int nat, const int *Z, const double *pos, const double *mom, const double *mass, int periodic,

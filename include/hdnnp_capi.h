@@ -45,7 +45,8 @@ enum {
     HDNNP_ERR_ELEMENT = 4,  /* structure holds an element the model lacks  */
     HDNNP_ERR_DEVICE = 5,   /* HIP runtime error                           */
     HDNNP_ERR_CAPACITY = 6, /* a size limit of this build was exceeded     */
-    HDNNP_ERR_TIMEOUT = 7   /* a server wait ran out of time               */
+    HDNNP_ERR_TIMEOUT = 7,  /* a server wait ran out of time               */
+    HDNNP_ERR_CANCELLED = 8 /* the server request was cancelled            */
 };
 
 /* Replaces parseRuNNerInput(handle, path) (src/potentials/hdnnps.f90:34-233):
@@ -299,6 +300,13 @@ int hdnnp_server_submit(hdnnp_server *sv, int32_t nat, const int32_t *Z, const d
  * latency_ms is the time from submission to completion. */
 int hdnnp_server_wait(hdnnp_server *sv, int64_t ticket, double timeout_s, double *energy, double *force,
                       double *dEdLat, double *energy_std, double *latency_ms);
+/* Cancel a queued evaluation or MD-step request (no counterpart in the
+ * reference: the server is new). *cancelled = 1 when it was still waiting
+ * for a lane: it never runs, and its wait returns HDNNP_ERR_CANCELLED (an MD
+ * session's later requests still run, from the state without it); 0 when a
+ * lane already runs it or it is done, and its wait returns the result. The
+ * ticket stays valid until waited for once. */
+int hdnnp_server_cancel(hdnnp_server *sv, int64_t ticket, int32_t *cancelled);
 /* open an MD session of one system (opened on a lane once it fits) */
 int hdnnp_server_md_open(hdnnp_server *sv, int32_t nat, const int32_t *Z, const double *pos, const double *mom,
                          const double *mass, int32_t periodic, const double *lat, double dt, double skin,

@@ -136,6 +136,65 @@ def test_server_error_stays_with_its_request(pkg, wl):
     sv.close()
 
 
+def test_server_cancel(pkg, wl):
+    """Queued requests cancelled behind a long one never run: their waits
+    raise HDNNP_ERR_CANCELLED; every request not cancelled (cancel returned
+    False because a lane had it, or never asked) equals its evaluation
+    alone; a cancelled MD-step request leaves the session stepping from the
+    state without it; a finished ticket cannot be cancelled, an unknown one
+    is an argument error."""
+    big = [wl.perovskite(8, 8, 8, seed=31 + k) for k in range(3)]   # ~6,000 atoms each, one batch each
+    small = _mix(wl, 8, seed=19)
+    structs = big + small
+    want = _alone(pkg, "mixed", structs)
+    sv = pkg.Server(PEROV_PARAMS, "mixed", lanes=1, max_batch_atoms=4096)
+    ts = [sv.submit(s) for s in structs]
+    cancelled = {k: sv.cancel(ts[k]) for k in (1, 2, 4, 6, 8, 10)}
+    print("cancelled", cancelled, flush=True)
+    assert any(cancelled.values())
+    for k, t in enumerate(ts):
+        if cancelled.get(k):
+            with pytest.raises(pkg.HdnnpError) as ei:
+                sv.result(t, timeout=120)
+            assert ei.value.code == pkg.HDNNP_ERR_CANCELLED
+        else:
+            E, F, D, _, _ = sv.result(t, timeout=120)
+            assert E == want[k][0] and np.array_equal(F, want[k][1]) and np.array_equal(D, want[k][2]), k
+    t = sv.submit(small[0])
+    sv.result(t, timeout=120)
+    with pytest.raises(pkg.HdnnpError) as ei:
+        sv.cancel(t)                          # waited for: the ticket is gone
+    assert ei.value.code == 1
+    # MD: a session behind a long evaluation, three queued requests of 2 steps
+    cell = wl.perovskite(2, 2, 2, seed=40)
+    rng = np.random.default_rng(4)
+    mom = np.sqrt(pkg.atomic_masses(cell.Z) * 300.0 * 3.166811563e-6)[:, None] * rng.standard_normal((cell.nat, 3))
+    sid = sv.md_open(cell, mom, DT, skin=1.0)
+    sv.md_result(sv.md_step(sid, 0), timeout=120)
+    tb = sv.submit(big[0])
+    steps = [sv.md_step(sid, 2) for _ in range(3)]
+    gone = [sv.cancel(steps[1])]
+    print("MD step cancelled", gone, flush=True)
+    kept = [steps[0]] + ([] if gone[0] else [steps[1]]) + [steps[2]]
+    got = [sv.md_result(x, timeout=120) for x in kept]
+    if gone[0]:
+        with pytest.raises(pkg.HdnnpError) as ei:
+            sv.md_result(steps[1], timeout=120)
+        assert ei.value.code == pkg.HDNNP_ERR_CANCELLED
+    sv.result(tb, timeout=120)
+    pos, momf, f = sv.md_get(sid)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed")
+    m = pkg.MdSystems(h, [cell], [mom], DT, skin=1.0)
+    m.step(0)
+    alone = [m.step(2) for _ in kept]
+    assert [g[0] for g in got] == [a[0][0] for a in alone]
+    pos0, mom0, f0 = m.get()
+    assert np.array_equal(pos, pos0) and np.array_equal(momf, mom0) and np.array_equal(f, f0)
+    m.close()
+    sv.md_close(sid)
+    sv.close()
+
+
 def test_server_failed_session_stays_with_itself(pkg, wl):
     """A session whose structure has an element the potential lacks fails
     its own requests only; a sound session and evaluations beside it run as
