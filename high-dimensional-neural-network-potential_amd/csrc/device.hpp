@@ -7,6 +7,7 @@
 namespace hdnnp {
 
 constexpr int kMaxEl = 16;       // elements per model (= kMaxElements)
+constexpr int kTripBlk = 64;   // ints per atom of Work::tblk
 constexpr int kMaxLay = 8;       // weight layers
 constexpr int kNonIntegralZeta = -(1 << 30);   // g4_ipow marker
 
@@ -58,6 +59,7 @@ struct DevModel {
     int g4_start[kMaxEl], g4_count[kMaxEl];
     int g4_max;              // largest g4_count
     int g4_lampm;            // 1 when every G4 lambda is +1 or -1 and every zeta >= 1
+    int g4_nblk;             // most element-pair blocks with G4 functions of one centre element
     int powbits;             // bits of the largest |g4_ipow| of integral zeta
     const ElemInfo *einfo;   // [nel]
     const double *sf_mean;   // [sf_off[e] + s]: scaled G = (G - sf_mean) / sf_den
@@ -130,6 +132,10 @@ struct Work {
     // not written.
     int *trip, *tcount;
     int tcap;
+    // Per atom kTripBlk ints: the element-pair blocks of its triplet list
+    // (which the FP32 values pass writes in block order): the count, then
+    // start | count << 11 | block << 23 each. null: not written.
+    int *tblk;
     int *ent_j;          // [nat*cap] neighbour atom
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)
     int *ent_s;          // [nat*cap] stencil-cell index of the entry

@@ -387,6 +387,13 @@ void Engine::init_device()
         d.g4_lampm = 1;
         for (size_t k = 0; k < g4lam.size(); ++k)
             if (!(g4lam[k] == 1.0 || g4lam[k] == -1.0) || !(g4zeta[k] >= 1.0)) d.g4_lampm = 0;
+        d.g4_nblk = 0;
+        for (int e = 0; e < d.nel; ++e) {
+            int nb = 0;
+            for (int f = 0; f < d.nel; ++f)
+                for (int g = f; g < d.nel; ++g) nb += g4tab[(e * kMaxEl + f) * kMaxEl + g].y > 0;
+            d.g4_nblk = std::max(d.g4_nblk, nb);
+        }
         auto tof = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         d.g2f_eta = upload(model_mem_, tof(g2eta));
         d.g2f_rs = upload(model_mem_, tof(g2rs));
@@ -533,21 +540,27 @@ void Engine::compute(Work &w, const Geo &g, hipStream_t s, bool timed)
     // one atom per wavefront: pass A hands its compacted triplets to pass B
     if (ga == 1) {
         trip_.ensure(4 * (size_t)nat * w.tcap);
-        tcount_.ensure(4 * (size_t)nat);
+        tcount_.ensure(4 * (size_t)nat * (1 + kTripBlk));
         w.trip = trip_.as<int>();
         w.tcount = tcount_.as<int>();
+        w.tblk = w.tcount + nat;
     } else {
         w.trip = nullptr;
         w.tcount = nullptr;
+        w.tblk = nullptr;
     }
     if (!skip_values_) {
-        launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);
+        launch_sf_values<R>(dm_, g, w, w.lim, ga, G, s);   // clears w.tblk unless it writes the tables
         if (dm_.g5_total) launch_g5_values<R>(dm_, g, w, G, s);
         HIPCHK(hipGetLastError());
         trip_valid_ = w.trip != nullptr;
+        tblk_valid_ = w.tblk != nullptr;
     } else if (!trip_valid_) {   // G came from elsewhere: pass B compacts its own triplets
         w.trip = nullptr;
         w.tcount = nullptr;
+        w.tblk = nullptr;
+    } else if (!tblk_valid_) {
+        w.tblk = nullptr;
     }
     if (timed) HIPCHK(hipEventRecord(ev_[3], s));
 
@@ -730,6 +743,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     prep_.st = st;
     ++build_gen_;
     trip_valid_ = false;
+    tblk_valid_ = false;
     if (!(model_.maxrc > 0)) throw Error(3, "potential has no symmetry functions (cutoff 0)");
     if (dm_.fallback_el < 0) throw Error(3, elem_why_[0].empty() ? "no element of the potential can be evaluated"
                                                                  : elem_why_[0]);
@@ -878,6 +892,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.tcap = triplet_capacity(w.lim);
     w.trip = nullptr;               // set by compute when the passes share triplets
     w.tcount = nullptr;
+    w.tblk = nullptr;
     if ((long)nat * w.cap >= (1L << 31)) throw Error(6, "neighbour list exceeds 2^31 entry slots");
     const size_t slots = (size_t)nat * w.cap;
     ent_j_.ensure(4 * slots);
@@ -1564,11 +1579,13 @@ void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
         const int ga = sf_tile(fp64, w.lim, tile_, dm_.nsfmax, dm_.g4_max, dm_.nel);
         w.trip = nullptr;
         w.tcount = nullptr;
+        w.tblk = nullptr;
         if (ga == 1) {
             trip_.ensure(4 * (size_t)nat * w.tcap);
-            tcount_.ensure(4 * (size_t)nat);
+            tcount_.ensure(4 * (size_t)nat * (1 + kTripBlk));
             w.trip = trip_.as<int>();
             w.tcount = tcount_.as<int>();
+            w.tblk = w.tcount + nat;
         }
         if (fp64) {
             launch_sf_values<double>(dm_, g, w, w.lim, ga, G_.as<double>(), s);
@@ -1579,6 +1596,7 @@ void Engine::stage_symfunctions(int together, int ld, double *G, double *dsf)
         }
         HIPCHK(hipGetLastError());
         trip_valid_ = w.trip != nullptr;
+        tblk_valid_ = w.tblk != nullptr;
         std::vector<double> tmp((size_t)nat * dm_.nsfmax);
         if (fp64) {
             d2h(tmp.data(), G_.p, 8 * tmp.size());

@@ -298,6 +298,7 @@ class Engine {
     std::vector<double> stage_pos_;     // the stage's input positions
     bool skip_values_ = false;          // compute: G is already in G_ (stage_nns)
     bool trip_valid_ = false;           // trip_ holds the values pass's triplets of the current build
+    bool tblk_valid_ = false;           // ... and their block tables (the FP32 values pass in block order)
     DevMem sfpar_, sfpar_off_, stage_G_, stage_dsf_, stage_off_;   // per-function path
     int last_nat_ = 0, last_cap_ = 0, last_lim_ = 0;
     int64_t last_total_ = 0;   // -1: not yet summed

@@ -19,10 +19,14 @@
 //   (loaded through the constant address space, so uniform indices compile to
 //   scalar loads); each lane accumulates per function, one wave reduction per
 //   function and block.
+//   (The FP32 pass with lambda = +-1 and zeta >= 1 is the block sweep below:
+//   the blocks' triplets with r_jk < rc into a list in block order, then the
+//   lanes dealt out to the blocks with the functions and sums in registers.)
 //   Pass B (gradients) needs no per-function sums, so it runs over the
-//   triplets of all blocks at once, in circulant order, with every lane busy;
-//   each lane reads its block's parameters, with dE/dG folded in, from LDS
-//   tables, forms per triplet
+//   triplets of all blocks at once, in circulant order, with every lane busy
+//   (after the FP32 block sweep: over its list, lanes dealt out to the
+//   blocks, g4_grad_blocks); each lane reads its block's parameters, with
+//   dE/dG folded in, from LDS tables (or registers), forms per triplet
 //     S1 = sum w P C E, S2 = sum w P C E 2 eta, S3 = sum w P b^(zeta-1) lambda zeta E
 //   and adds g_j = S1 dF/dR_j + S2 F (d_ij - d_jk) + S3 F dcos/dR_j (same for
 //   k) to FP64 per-neighbour LDS accumulators.
@@ -294,6 +298,7 @@ __device__ __forceinline__ int mbcnt(unsigned long long m)
 
 template <typename R> struct Vec4 { typedef R T __attribute__((ext_vector_type(4))); };
 typedef int I2 __attribute__((ext_vector_type(2)));
+typedef int I4 __attribute__((ext_vector_type(4)));
 typedef float F2 __attribute__((ext_vector_type(2)));
 
 // Atoms per wavefront (GA, the tile): several atoms of one element can share
@@ -1078,85 +1083,243 @@ __device__ __forceinline__ void g4_values_list(const DevModel &m, const Lds<R> &
 
 __device__ __forceinline__ float log2_base(float b);
 
-// LOGF (FP32, every lambda = +-1): b^(zeta-1) exp(-eta R2) as one
-// exponential of (zeta - 1) log2(b) + ea R2, with log2(1 +- cos) once per
-// triplet (see the row-sweep values pass), no branch on zeta
+// 1 + cos and 1 - cos of the angle between d_ij and d_ik (a = (d, 1/r)) as
+// |u_j + u_k|^2 / 2 and |u_j - u_k|^2 / 2 of the unit vectors: accurate to a
+// few ulp also where the angle is near 0 or pi, where 1 -+ cos in FP32 loses
+// the digits that (1 -+ cos)^zeta of large zeta needs
+__device__ __forceinline__ void one_pm_cos(const Vec4<float>::T &aj, const Vec4<float>::T &ak, float &bp, float &bm)
+{
+#pragma clang fp contract(off)
+    const float ujx = aj.x * aj.w, ujy = aj.y * aj.w, ujz = aj.z * aj.w;
+    const float ukx = ak.x * ak.w, uky = ak.y * ak.w, ukz = ak.z * ak.w;
+    const float sx = ujx + ukx, sy = ujy + uky, sz = ujz + ukz;
+    const float dx = ujx - ukx, dy = ujy - uky, dz = ujz - ukz;
+    bp = 0.5f * fmaR(sz, sz, fmaR(sy, sy, sx * sx));
+    bm = 0.5f * fmaR(dz, dz, fmaR(dy, dy, dx * dx));
+}
+
+// G4 functions of one element-pair block kept in registers by the passes
+// that deal the lanes out to the blocks
+constexpr int kRowFns = 4;
+constexpr int kSumRun = 4;   // terms summed in FP32 before they go to the FP64 sums
+
+// ceil(c / T) for 0 <= c < 2^22 and T >= 1, with rT about 1 / T (the
+// quotient of the product is off by at most one, which the remainder fixes)
+__device__ __forceinline__ int ceil_div_f(int c, int T, float rT)
+{
+    int qq = (int)((float)c * rT);
+    int rr = c - qq * T;
+    if (rr >= T) {
+        ++qq;
+        rr -= T;
+    }
+    if (rr < 0) {
+        --qq;
+        rr += T;
+    }
+    return qq + (rr > 0);
+}
+
+// sum over the wavefront of v in [0, 128): one ballot per bit
+__device__ __forceinline__ int wave_sum7(int v)
+{
+    int sum = 0;
+#pragma unroll
+    for (int bit = 0; bit < 7; ++bit) sum += __popcll(__ballot((v >> bit) & 1)) << bit;
+    return sum;
+}
+
+// Steps T of a wavefront dealt out to nw (< 64) blocks of a list of V
+// entries, block x (this lane's, x = lane < nw) holding cx: each block gets
+// nlx = ceil(cx / T) lanes, and T is the fewest steps (from ceil(V / 64),
+// trying two more, else ceil(V / (64 - nw)), which always fits) with at most
+// 64 lanes in all
+__device__ __forceinline__ void deal_lanes(int V, int nw, int cx, int &T, int &nlx)
+{
+    T = max((V + 63) >> 6, 1);
+    nlx = ceil_div_f(cx, T, __builtin_amdgcn_rcpf((float)T));
+    for (int tr = 0; tr < 2 && wave_sum7(nlx) > 64; ++tr) {
+        ++T;
+        nlx = ceil_div_f(cx, T, __builtin_amdgcn_rcpf((float)T));
+    }
+    if (wave_sum7(nlx) > 64) {   // at most V / T + nw <= 64 lanes
+        T = ceil_div_f(V, 64 - nw, __builtin_amdgcn_rcpf((float)(64 - nw)));
+        nlx = ceil_div_f(cx, T, __builtin_amdgcn_rcpf((float)T));
+    }
+}
+
+// One triplet (j, k) of pass B: its geometry, the weighted function sums
+// S1 = sum w P b^zeta E, S2 = sum w P 2 eta b^zeta E, S3 = sum w P lambda
+// zeta b^(zeta-1) E by sums(aj, ak, cs, r2, S1, S2, S3), and the gradient
+// rows of j and k (FP64 LDS sums)
+template <typename R, int CT, typename Sums>
+__device__ __forceinline__ void g4_grad_triplet(const DevModel &m, const Lds<R> &L, int j, int k, R rrc, Sums sums)
+{
+    const int ct = CT >= 0 ? CT : m.cutoff_type;
+    const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+    const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
+    const R rjk2 = ex * ex + ey * ey + ez * ez;
+    R fcjk, dfcjk;
+    cutoff_in<R>(ct, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
+    const R irj = aj.w, irk = ak.w;
+    const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
+    const R inv = irj * irk;
+    const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * inv;
+    const R F = fcij * fcik * fcjk;
+    const R r2 = bj.w + bk.w + rjk2;
+    R S1 = 0, S2 = 0, S3 = 0;
+    sums(aj, ak, cs, r2, S1, S2, S3);
+    // dF/dR_j = a d_jk - bj d_ij, dF/dR_k = -(a d_jk + bk d_ik)
+    const R a = fcik * dfcjk * fcij;
+    const R bjj = fcik * dfcij * fcjk, bkk = fcij * dfcik * fcjk;
+    const R S2F = S2 * F, S3F = S3 * F;
+    const R beta = S1 * a - S2F;
+    const R cjk = beta - S3F * inv;
+    const R cjj = S2F - S1 * bjj + S3F * cs * irj * irj - beta;
+    const R ckk = S2F - S1 * bkk + S3F * cs * irk * irk - beta;
+    __hip_atomic_fetch_add(&L.gx[j], (double)(cjj * aj.x + cjk * ak.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&L.gy[j], (double)(cjj * aj.y + cjk * ak.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&L.gz[j], (double)(cjj * aj.z + cjk * ak.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&L.gx[k], (double)(ckk * ak.x + cjk * aj.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&L.gy[k], (double)(ckk * ak.y + cjk * aj.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&L.gz[k], (double)(ckk * ak.z + cjk * aj.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The LOGF terms of one function (FP32, lambda = +-1): b^(zeta-1) exp(-eta
+// R2) as one exponential of (zeta - 1) log2(b) + ea R2, with log2(1 +- cos)
+// once per triplet, no branch on zeta. p = (exp scale, lambda, q1, q2), zm =
+// zeta - 1, q3: the weights of S1, S2, S3.
+__device__ __forceinline__ void logf_terms(const Vec4<float>::T p, float zm, float q3, float r2, float bp, float bm,
+                                           float Lp, float Lm, float &S1, float &S2, float &S3)
+{
+#pragma clang fp contract(off)
+    const bool pos = p.y > 0.0f;
+    const float EB = __builtin_amdgcn_exp2f(fmaR(zm, pos ? Lp : Lm, p.x * r2));   // b^(zeta-1) E
+    const float C = EB * (pos ? bp : bm);                                        // b^zeta E
+    S1 = fmaR(C, p.z, S1);
+    S2 = fmaR(C, p.w, S2);
+    S3 = fmaR(EB, q3, S3);
+}
+
+// Pass B over a chunk of nv triplets in L.pairs, the functions of each
+// triplet's block from the LDS tables. LOGF: FP32 with every lambda = +-1
+// (logf_terms).
 template <typename R, int CT, bool LOGF = false>
 __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L, int nv, int g4s, R rrc)
 {
-    const int ct = CT >= 0 ? CT : m.cutoff_type;
     const int lane = lane_id();
-    for (int q = lane; q < nv; q += 64) {
+    // each lane takes a contiguous range of the chunk: for a list in block
+    // order the lanes of one step then hold different neighbours j and k,
+    // whose gradient rows they update
+    const int per = (nv + 63) >> 6;
+    for (int st = 0; st < per; ++st) {
+        const int q = lane * per + st;
+        if (q >= nv) break;
         const int pk = L.pairs[q];
         const int j = pk & 0x3FF, k = (pk >> 10) & 0x3FF;
         const I2 bt = L.btab[pk >> 20];
-        const typename Lds<R>::V aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
-        const R ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;   // d_jk
-        const R rjk2 = ex * ex + ey * ey + ez * ez;
-        R fcjk, dfcjk;
-        cutoff_in<R>(ct, M<R>::sqrt(rjk2), rrc, fcjk, dfcjk);
-        const R irj = aj.w, irk = ak.w;
-        const R fcij = bj.y, fcik = bk.y, dfcij = bj.z, dfcik = bk.z;
-        const R inv = irj * irk;
-        const R cs = (aj.x * ak.x + aj.y * ak.y + aj.z * ak.z) * inv;
-        const R F = fcij * fcik * fcjk;
-        const R r2 = bj.w + bk.w + rjk2;
-        R S1 = 0, S2 = 0, S3 = 0, E = 0;
-        if constexpr (LOGF) {
-#pragma clang fp contract(off)
-            const float bp = 1.0f + cs, bm = 1.0f - cs;
-            const float Lp = log2_base(bp), Lm = log2_base(bm);
-            for (int t = 0; t < bt.y; ++t) {
-                const int u = bt.x + t;
-                const typename Lds<R>::V p = L.sp0[u];
-                const float zm = __int_as_float(L.smeta[u]);
-                const bool pos = p.y > 0.0f;
-                const float EB = __builtin_amdgcn_exp2f(fmaR(zm, pos ? Lp : Lm, p.x * r2));   // b^(zeta-1) E
-                const float C = EB * (pos ? bp : bm);                                        // b^zeta E
-                S1 = fmaR(C, p.z, S1);
-                S2 = fmaR(C, p.w, S2);
-                S3 = fmaR(EB, L.sp1[u], S3);
-            }
-        } else
-        for (int t = 0; t < bt.y; ++t) {
-            const int u = bt.x + t;
-            const typename Lds<R>::V p = L.sp0[u];
-            const int mt = L.smeta[u];
-            if (t == 0 || (mt & (1 << 8))) E = M<R>::expsc(p.x, r2);
-            R b = R(1) + p.y * cs;
-            R bz, bm;
-            const int sq = mt & 0xFF;
-            if (sq != 0xFF) {
-                bm = R(1);
-                for (int kk = 0; kk < sq; ++kk) {
-                    bm *= b;
-                    b *= b;
+        g4_grad_triplet<R, CT>(m, L, j, k, rrc, [&](const typename Lds<R>::V &aj, const typename Lds<R>::V &ak, R cs,
+                                                    R r2, R &S1, R &S2, R &S3) {
+            if constexpr (LOGF) {
+                float bp, bm;
+                one_pm_cos(aj, ak, bp, bm);
+                const float Lp = log2_base(bp), Lm = log2_base(bm);
+                for (int t = 0; t < bt.y; ++t) {
+                    const int u = bt.x + t;
+                    logf_terms(L.sp0[u], __int_as_float(L.smeta[u]), L.sp1[u], r2, bp, bm, Lp, Lm, S1, S2, S3);
                 }
-                bz = b;
             } else {
-                const int x = g4s + u;
-                bm = angpow<R>(b, cld(m.g4_ipow, x), tabc<R>(m.g4_zeta, m.g4f_zeta, x) - R(1));
-                bz = bm * b;
+                R E = 0;
+                for (int t = 0; t < bt.y; ++t) {
+                    const int u = bt.x + t;
+                    const typename Lds<R>::V p = L.sp0[u];
+                    const int mt = L.smeta[u];
+                    if (t == 0 || (mt & (1 << 8))) E = M<R>::expsc(p.x, r2);
+                    R b = R(1) + p.y * cs;
+                    R bz, bm;
+                    const int sq = mt & 0xFF;
+                    if (sq != 0xFF) {
+                        bm = R(1);
+                        for (int kk = 0; kk < sq; ++kk) {
+                            bm *= b;
+                            b *= b;
+                        }
+                        bz = b;
+                    } else {
+                        const int x = g4s + u;
+                        bm = angpow<R>(b, cld(m.g4_ipow, x), tabc<R>(m.g4_zeta, m.g4f_zeta, x) - R(1));
+                        bz = bm * b;
+                    }
+                    const R C = E * bz;
+                    S1 += C * p.z;
+                    S2 += C * p.w;
+                    S3 += E * bm * L.sp1[u];
+                }
             }
-            const R C = E * bz;
-            S1 += C * p.z;
-            S2 += C * p.w;
-            S3 += E * bm * L.sp1[u];
+        });
+    }
+}
+
+// Pass B (FP32, LOGF) over the atom's triplet list in block order, as the
+// values pass left it: tb = (entries, then start | count << 11 | block << 23
+// each). The lanes are dealt out to the blocks in proportion to their
+// triplets; a lane keeps its block's function weights in registers.
+template <int CT>
+__device__ __forceinline__ void g4_grad_blocks(const DevModel &m, const Lds<float> &L, const int *tl, const int *tb,
+                                               int V, float rrc)
+{
+    const int lane = lane_id();
+    const int nw = uni(tb[0]);
+    const int ent = lane < nw ? tb[1 + lane] : 0;
+    const int cx = (ent >> 11) & 0xFFF;
+    int T, nlx;
+    deal_lanes(V, nw, cx, T, nlx);
+    int ws = 0, wc = 0, nl = 1, q = 0, blk = 0;
+    {
+        int first = 0;
+        for (int x = 0; x < nw; ++x) {
+            const int n_l = __builtin_amdgcn_readlane(nlx, x), ex = __builtin_amdgcn_readlane(ent, x);
+            if (lane >= first && lane < first + n_l) {
+                ws = ex & 0x7FF;
+                wc = (ex >> 11) & 0xFFF;
+                blk = (unsigned)ex >> 23;
+                nl = n_l;
+                q = lane - first;
+            }
+            first += n_l;
         }
-        // dF/dR_j = a d_jk - bj d_ij, dF/dR_k = -(a d_jk + bk d_ik)
-        const R a = fcik * dfcjk * fcij;
-        const R bjj = fcik * dfcij * fcjk, bkk = fcij * dfcik * fcjk;
-        const R S2F = S2 * F, S3F = S3 * F;
-        const R beta = S1 * a - S2F;
-        const R cjk = beta - S3F * inv;
-        const R cjj = S2F - S1 * bjj + S3F * cs * irj * irj - beta;
-        const R ckk = S2F - S1 * bkk + S3F * cs * irk * irk - beta;
-        __hip_atomic_fetch_add(&L.gx[j], (double)(cjj * aj.x + cjk * ak.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&L.gy[j], (double)(cjj * aj.y + cjk * ak.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&L.gz[j], (double)(cjj * aj.z + cjk * ak.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&L.gx[k], (double)(ckk * ak.x + cjk * aj.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&L.gy[k], (double)(ckk * ak.y + cjk * aj.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&L.gz[k], (double)(ckk * ak.z + cjk * aj.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    const I2 bt = L.btab[blk];
+    const int u0 = bt.x, cnt = wc > 0 ? bt.y : 0;
+    Vec4<float>::T pp[kRowFns];
+    float zm[kRowFns], q3[kRowFns];
+#pragma unroll
+    for (int t = 0; t < kRowFns; ++t) {
+        pp[t] = t < cnt ? L.sp0[u0 + t] : Vec4<float>::T{};
+        zm[t] = t < cnt ? __int_as_float(L.smeta[u0 + t]) : 0.0f;
+        q3[t] = t < cnt ? L.sp1[u0 + t] : 0.0f;
+    }
+    int pk = q < wc ? tl[ws + q] : 0;
+    for (int st = 0; st < T; ++st) {
+        const int idx = q + st * nl;
+        const int cur = pk;
+        if (idx + nl < wc) pk = tl[ws + idx + nl];   // the next step's triplet
+        if (idx >= wc) continue;
+        g4_grad_triplet<float, CT>(m, L, cur & 0x3FF, (cur >> 10) & 0x3FF, rrc,
+                                   [&](const Vec4<float>::T &aj, const Vec4<float>::T &ak, float, float r2, float &S1,
+                                       float &S2, float &S3) {
+                                       float bp, bm;
+                                       one_pm_cos(aj, ak, bp, bm);
+                                       const float Lp = log2_base(bp), Lm = log2_base(bm);
+#pragma unroll
+                                       for (int t = 0; t < kRowFns; ++t) {
+                                           if (t >= cnt) break;
+                                           logf_terms(pp[t], zm[t], q3[t], r2, bp, bm, Lp, Lm, S1, S2, S3);
+                                       }
+                                       for (int t = kRowFns; t < cnt; ++t)   // beyond the registers'
+                                           logf_terms(L.sp0[u0 + t], __int_as_float(L.smeta[u0 + t]), L.sp1[u0 + t],
+                                                      r2, bp, bm, Lp, Lm, S1, S2, S3);
+                                   });
     }
 }
 
@@ -1312,7 +1475,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         __syncthreads();
         const R rce2 = rce * rce, rrc = R(1) / rce;
         const int tc = w.trip ? uni(w.tcount[i]) : -1;
-        if (tc >= 0 && tc <= w.tcap) {
+        bool done = false;
+        if constexpr (sizeof(R) == 4) {
+            if (tc >= 0 && tc <= w.tcap && w.tblk && m.g4_lampm) {   // the block-ordered list of the FP32 pass
+                const int *tl = w.trip + (size_t)i * w.tcap, *tb = w.tblk + (size_t)i * kTripBlk;
+                if (m.cutoff_type == 2) g4_grad_blocks<2>(m, L, tl, tb, tc, rrc);
+                else g4_grad_blocks<-1>(m, L, tl, tb, tc, rrc);
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (tc >= 0 && tc <= w.tcap) {
             // the triplets pass A compacted
             const int *tl = w.trip + (size_t)i * w.tcap;
             for (int cb = 0; cb < tc; cb += kPairChunk<R>) {
@@ -1358,38 +1531,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
     }
 }
 
-// ---------------------------------------------------------------- FP32 row-sweep values pass
+// ---------------------------------------------------------------- FP32 block-sweep values pass
 // Pass A in FP32 with one atom per wavefront when every G4 lambda is +-1 (so
-// 1 + lambda cos >= 0) and every zeta >= 1. It takes the unordered pairs of
-// the staged neighbours in circulant order, as rows: the pair (j, j + d mod
-// n) belongs to neighbour j for d = 1..(n-1)/2, and for even n also (j, j +
-// n/2) for j < n/2. A lane owns a neighbour j and sweeps its d (for more
-// than 64 lanes' worth of pairs the d-range is cut into chunks, item c*n +
-// j); at step d the lanes read rows k = j + d, distinct and consecutive, so
-// the LDS reads have no bank conflicts, and j's own row stays in registers.
-// Pairs with r_jk >= rc, or of a block without G4 functions, add nothing.
-// The staged neighbours are grouped by element, so along a sweep the block
-// (el j, el k) changes at most nel + 1 times: the lane keeps the block's
-// parameters (up to kRowFns functions) and its term sums in registers, and
-// adds the sums to the FP64 function sums (LDS atomics) when the block
-// changes -- not one LDS update per term. The angular and radial factors of
-// a function are one exponential:
+// 1 + lambda cos >= 0) and every zeta >= 1, in two phases over the atom's
+// element-pair blocks (staged neighbours are grouped by element, so a block
+// is a segment pair).
+// Phase 1 sweeps each block's candidate pairs as rows (row a of the smaller
+// segment against the columns of the other, for one element the circulant
+// rows of its segment; the row cut into chunks, a lane per (row, chunk)):
+// the lane keeps its row's vector in registers, and at each step the lanes
+// read distinct columns (no LDS bank conflicts). The pairs with r_jk < rc
+// (about half of the candidates) go to the atom's triplet list, compacted by
+// ballot, in block order; the block ranges go to a small table.
+// Phase 2 deals the lanes out to the blocks in proportion to their triplets
+// (deal_lanes): a lane keeps its block's functions (up to kRowFns) and its
+// term sums in registers, sums kSumRun terms in FP32 and then in FP64, and
+// adds them to the FP64 function sums (LDS atomics) once at the end -- not
+// one LDS update per term. The angular and radial factors of a function are
+// one exponential:
 //   2^(1-zeta) (1 + lambda cos)^zeta exp(-eta R2) = exp2(zeta L + (1 - zeta) + ea R2)
 // with L = log2(1 + lambda cos) (two logarithms per triplet, one per sign of
-// lambda; L >= -126, so b = 0 gives 0 for zeta > 1), R2 = r_ij^2 + r_ik^2 +
-// r_jk^2 and ea = -eta log2(e): no branch on zeta, and any real zeta >= 1.
-// Functions of a block beyond kRowFns read their parameters from LDS per
-// triplet. The triplets with r_jk < rc go to the atom's list for pass B (in
-// the sweep's order). The wavefront of atom b runs on XCD b % 8: xcd_block
-// gives each XCD a contiguous range of atoms, whose neighbours overlap in
-// its L2.
-constexpr int kRowFns = 4;
+// lambda; 1 +- cos from the unit vectors, one_pm_cos; L >= -126, so b = 0
+// gives 0 for zeta > 1), R2 = r_ij^2 + r_ik^2 + r_jk^2 and ea = -eta
+// log2(e): no branch on zeta, and any real zeta >= 1. Functions of a block
+// beyond kRowFns read their parameters from LDS per triplet.
+// The list holds tcap entries per atom; an atom with more triplets is
+// evaluated in windows of the list (phase 1 until the window is full, then
+// phase 2), and pass B then compacts its triplets itself. Pass B reads the
+// list and the table of a complete list (g4_grad_blocks). The wavefront of
+// atom b runs on XCD b % 8: xcd_block gives each XCD a contiguous range of
+// atoms, whose neighbours overlap in its L2.
 
 struct DirectOffsets {
-    size_t a, b, el, ent, seg, accd, tab0, btab, total;
+    size_t a, b, el, ent, seg, accd, tab0, blk, win, total;
 };
 
-__host__ __device__ __forceinline__ DirectOffsets direct_offsets(int nmax, int nsfmax, int g4max, int nel)
+__host__ __device__ __forceinline__ DirectOffsets direct_offsets(int nmax, int nsfmax, int g4max, int nblk)
 {
     DirectOffsets o{};
     size_t off = 0;
@@ -1405,12 +1582,13 @@ __host__ __device__ __forceinline__ DirectOffsets direct_offsets(int nmax, int n
     o.seg = take(4 * (kMaxEl + 1));
     o.accd = take(8 * (size_t)nsfmax);   // G sums (FP64)
     o.tab0 = take(16 * (size_t)g4max);
-    o.btab = take(8 * (size_t)nel * (size_t)nel);
+    o.blk = take(16 * (size_t)nblk);     // element-pair blocks with G4 functions
+    o.win = take(8 * (size_t)nblk);      // the blocks of a window of the triplet list
     o.total = off;
     return o;
 }
 
-// the stage() view of the row-sweep pass's LDS
+// the stage() view of the block-sweep pass's LDS
 __device__ __forceinline__ Lds<float> direct_lds(const DirectOffsets &o, LDS char *base)
 {
     Lds<float> L{};
@@ -1419,7 +1597,6 @@ __device__ __forceinline__ Lds<float> direct_lds(const DirectOffsets &o, LDS cha
     L.el = (LDS unsigned char *)(base + o.el);
     L.ent = (LDS unsigned short *)(base + o.ent);
     L.seg = (LDS int *)(base + o.seg);
-    L.btab = (LDS I2 *)(base + o.btab);
     L.accd = (LDS double *)(base + o.accd);
     return L;
 }
@@ -1429,50 +1606,6 @@ __device__ __forceinline__ Lds<float> direct_lds(const DirectOffsets &o, LDS cha
 __device__ __forceinline__ float log2_base(float b)
 {
     return __builtin_fmaxf(__builtin_amdgcn_logf(__builtin_fmaxf(b, 0.0f)), -126.0f);
-}
-
-// The sweep of n staged neighbours: Rt rows of d (the last, for even n, only
-// for j < n/2), cut into C chunks of len rows, items = C n, in passes of 64.
-// C (1..8) gives the fewest passes * len.
-struct Sweep {
-    int n, Rt, C, len, items;
-    float inv_n;
-};
-__device__ __forceinline__ Sweep make_sweep(int n)
-{
-    Sweep s;
-    s.n = n;
-    s.Rt = n / 2;   // (n-1)/2 rows, plus the n/2 row for even n
-    s.C = 1;
-    int best = 1 << 30;
-    for (int c = 1; c <= 8; ++c) {
-        const int len = (s.Rt + c - 1) / c;
-        const int cost = ((n * c + 63) / 64) * len;
-        if (cost < best) {
-            best = cost;
-            s.C = c;
-        }
-    }
-    s.len = (s.Rt + s.C - 1) / s.C;
-    s.items = n * s.C;
-    s.inv_n = 1.0f / (float)max(n, 1);
-    return s;
-}
-// item -> (j, first d, last d)
-__device__ __forceinline__ void sweep_item(const Sweep &s, int item, int &j, int &d0, int &d1)
-{
-    const int c = row_of(item, s.inv_n);
-    j = item - __mul24(c, s.n);
-    d0 = 1 + c * s.len;
-    d1 = min(s.Rt, d0 + s.len - 1);
-    // the row d = n/2 of even n holds (j, j + n/2) for j < n/2 only
-    if ((s.n & 1) == 0 && d1 == s.Rt && j >= s.n / 2) --d1;
-}
-
-// block (el j, el k): (first local function, count)
-__device__ __forceinline__ I2 row_block(const Lds<float> &L, int ej, int ek, int nel)
-{
-    return L.btab[min(ej, ek) * nel + max(ej, ek)];
 }
 
 template <int NMAX, int CT>
@@ -1486,7 +1619,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
     const ElemInfo ei = cld_einfo(m.einfo, e);
     const int nsf = ei.nsf, g4s = ei.g4_start, g4n = ei.g4_count;
     const int ct = CT >= 0 ? CT : m.cutoff_type;
-    const DirectOffsets o = direct_offsets(NMAX, m.nsfmax, m.g4_max, m.nel);
+    const DirectOffsets o = direct_offsets(NMAX, m.nsfmax, m.g4_max, m.g4_nblk);
     const Lds<float> L = direct_lds(o, (LDS char *)smem);
     LDS Vec4<float>::T *tab = (LDS Vec4<float>::T *)((LDS char *)smem + o.tab0);
     for (int s = lane; s < nsf; s += 64) L.accd[s] = 0;
@@ -1496,7 +1629,6 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
         tab[u] = Vec4<float>::T{cld(m.g4f_ea, x), cld(m.g4f_zeta, x), cld(m.g4f_lam, x),
                                 __int_as_float(cld(m.g4_s, x))};
     }
-    block_table<float>(m, L, e, g4s);
     stage<float, 64, NMAX, true>(m, g, w, i, wave_max(w.count[i]), ei.rc, L, (float *)nullptr);
     __syncthreads();
     const float rce = (float)ei.rc;
@@ -1515,69 +1647,186 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
         L.accd[cld(m.g2_s, x)] += (double)sum;
     }
     __syncthreads();
-    // G4, symmetryFunctions.f90:231-305, over the rows of the staged neighbours
-    const Sweep sw = make_sweep(uni(L.seg[kMaxEl]));
+    // G4, symmetryFunctions.f90:231-305
+    LDS I4 *bt = (LDS I4 *)((LDS char *)smem + o.blk);
+    LDS I2 *wt = (LDS I2 *)((LDS char *)smem + o.win);
+    const int nel = m.nel;
+    // The element-pair blocks (f <= h) with G4 functions, in (f, h) order,
+    // each swept as rows: row a of segment (sa, na) against the columns of
+    // segment (sb, nb), the row cut into C chunks of len columns, na C items
+    // in passes of 64 lanes. Two elements: (sa, na) is the smaller segment
+    // and row a takes the columns (a + o) mod nb, o < nb (diagonals: the
+    // lanes of a chunk read different columns). One element (n = na = nb):
+    // row a takes (a + 1 + o) mod n, o < n/2, and for even n the last (o =
+    // n/2 - 1) only for a < n/2. C fills the 64 lanes: min(R, 64 / na).
+    // Entry: (sa | na << 16, sb | nb << 16, first function | count << 16,
+    // f nel + h | one element << 8 | C << 9 | len << 16)
+    int nbk = 0;
+    for (int t0 = 0; t0 < nel * nel; t0 += 64) {
+        const int t = t0 + lane;
+        const int f = t / nel, h = t - f * nel;
+        int bx = 0, by = 0;
+        if (t < nel * nel && f <= h) cld2(m.g4_tab, (e * kMaxEl + f) * kMaxEl + h, bx, by);
+        const unsigned long long mk = __ballot(by > 0);
+        if (by > 0) {
+            const int sf = L.seg[f], Lf = L.seg[f + 1] - sf, sh = L.seg[h], Lh = L.seg[h + 1] - sh;
+            const bool same = f == h;
+            const int sa = same || Lf <= Lh ? sf : sh, na = min(Lf, Lh);
+            const int sb = same || Lf <= Lh ? sh : sf, nb = max(Lf, Lh);
+            const int R = same ? na / 2 : nb;
+            const int C = max(1, na >= 64 ? 1 : min(R, 64 / max(na, 1)));
+            const int len = (R + C - 1) / C;
+            bt[nbk + mbcnt(mk)] = I4{sa | na << 16, sb | nb << 16, (bx - g4s) | by << 16,
+                                     t | (int)same << 8 | C << 9 | len << 16};
+        }
+        nbk += __popcll(mk);
+    }
+    __syncthreads();
     const float rrc = 1.0f / rce, rc2 = rce * rce;
-    int *tl = w.trip ? w.trip + (size_t)i * w.tcap : nullptr;   // the triplets, for pass B
-    int tbase = 0;
-    for (int p0 = 0; p0 < sw.items; p0 += 64) {
-        const int item = p0 + lane;
-        int j = 0, d0 = 1, d1 = 0;
-        if (item < sw.items) sweep_item(sw, item, j, d0, d1);
-        const Vec4<float>::T aj = L.a[j], bj = L.b[j];
-        const int ej = L.el[j];
-        int hcur = -1, cnt = 0, u0 = 0, blk = 0;
+    int *tl = w.trip + (size_t)i * w.tcap;
+    const int cap = w.tcap;
+    int b = 0, P = 0, sp = 0, tot = 0, nw = 0;   // the sweep's block, pass and step
+    bool over = false;
+    while (b < nbk) {
+        // phase 1: the triplets with r_jk < rc of the blocks, in block order,
+        // into the list (a window of it: the list holds cap entries)
+        bool full = false;
+        tot = 0;
+        nw = 0;
+        for (; b < nbk; ++b, P = 0, sp = 0) {
+            const I4 d = bt[b];
+            const int dx = uni(d.x), dy = uni(d.y), dw = uni(d.w);
+            const int sa = dx & 0xFFFF, na = dx >> 16, sb = dy & 0xFFFF, nb = dy >> 16;
+            const bool same = (dw >> 8) & 1;
+            const int C = (dw >> 9) & 0x7F, len = dw >> 16, code = (dw & 0xFF) << 20;
+            const int R = same ? na / 2 : nb, items = na * C;
+            const float inv_na = __builtin_amdgcn_rcpf((float)max(na, 1));
+            const int bstart = tot;
+            for (; P * 64 < items; ++P, sp = 0) {
+                const int it = P * 64 + lane;
+                const int c = row_of(it, inv_na), a = it - __mul24(c, na);
+                const int o0 = c * len;
+                // columns o0 + sp .. oend - 1 of row a (kk, the column, stays
+                // in [0, nb) also on lanes without columns: their reads stay
+                // in the segment)
+                int oend = it < items ? min(R, o0 + len) : 0;
+                if (same && (na & 1) == 0 && a >= na / 2) oend = min(oend, R - 1);
+                const int j = sa + a;
+                const Vec4<float>::T aj = L.a[j];
+                int kk = a + o0 + sp + (int)same;
+                if (kk >= nb) kk -= nb;
+                if (kk >= nb) kk -= nb;
+                if (kk >= nb) kk = 0;
+                const int jc = j | code;
+                LDS const Vec4<float>::T *colp = L.a + sb;
+                // the pass writes at most (its lanes) x (its steps left)
+                // entries: when they fit, its steps skip the capacity test
+                if (tot + min(64, items - P * 64) * (len - sp) <= cap) {
+                    for (int o = o0 + sp; sp < len; ++sp, ++o) {
+                        const Vec4<float>::T ak = colp[kk];
+                        const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
+                        const bool valid = (o < oend) & (fmaR(ez, ez, fmaR(ey, ey, ex * ex)) < rc2);
+                        const unsigned long long mk = __ballot(valid);
+                        if (valid) tl[tot + mbcnt(mk)] = jc | ((sb + kk) << 10);
+                        tot += __popcll(mk);
+                        kk = kk + 1 == nb ? 0 : kk + 1;
+                    }
+                } else {
+                    for (int o = o0 + sp; sp < len; ++sp, ++o) {
+                        const Vec4<float>::T ak = colp[kk];
+                        const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
+                        const bool valid = (o < oend) & (fmaR(ez, ez, fmaR(ey, ey, ex * ex)) < rc2);
+                        const unsigned long long mk = __ballot(valid);
+                        const int cv = __popcll(mk);
+                        if (tot > 0 && tot + cv > cap) {   // the window is full: evaluate it, then go on
+                            full = true;
+                            break;
+                        }
+                        const int pos = tot + mbcnt(mk);
+                        if (valid && pos < cap) tl[pos] = jc | ((sb + kk) << 10);
+                        tot += cv;
+                        kk = kk + 1 == nb ? 0 : kk + 1;
+                    }
+                    if (full) break;
+                }
+            }
+            const int te = min(tot, cap);
+            if (te > bstart) {
+                if (lane == 0) wt[nw] = I2{bstart, (te - bstart) | (b << 16)};
+                ++nw;
+            }
+            if (full) break;
+        }
+        __syncthreads();
+        // phase 2: the lanes are dealt out to the window's blocks in
+        // proportion to their triplets; a lane keeps its block's functions
+        // and its term sums in registers
+        const int V = min(tot, cap);
+        int T;
+        {   // lane x < nw: the lanes of entry x
+            const int cx = lane < nw ? ((int)wt[lane].y & 0xFFFF) : 0;
+            int nlx;
+            deal_lanes(V, nw, cx, T, nlx);
+            if (lane < nw) wt[lane].x |= nlx << 16;
+        }
+        __syncthreads();
+        int ws = 0, wc = 0, nl = 1, q = 0, u0 = 0, cnt = 0;
+        {
+            int first = 0;
+            for (int x = 0; x < nw; ++x) {
+                const I2 wx = wt[x];
+                const int wx0 = uni((int)wx.x), wx1 = uni((int)wx.y);
+                const int n_l = wx0 >> 16;
+                if (lane >= first && lane < first + n_l) {
+                    ws = wx0 & 0xFFFF;
+                    wc = wx1 & 0xFFFF;
+                    nl = n_l;
+                    q = lane - first;
+                    const int bz = bt[wx1 >> 16].z;
+                    u0 = bz & 0xFFFF;
+                    cnt = bz >> 16;
+                }
+                first += n_l;
+            }
+        }
+        // the term sums: FP32 over (at most) kSumRun steps, then FP64
         float ea[kRowFns], ze[kRowFns], cz[kRowFns], lam[kRowFns], acc[kRowFns];
+        double accd[kRowFns];
         int slot[kRowFns];
 #pragma unroll
-        for (int t = 0; t < kRowFns; ++t) ea[t] = ze[t] = cz[t] = lam[t] = acc[t] = 0, slot[t] = 0;
-        for (int d = d0; d <= d1; ++d) {
-            int k = j + d;
-            if (k >= sw.n) k -= sw.n;
-            const int ek = L.el[k];
-            if (ek != hcur) {   // a new block: flush the sums, load its functions
-#pragma unroll
-                for (int t = 0; t < kRowFns; ++t)
-                    if (t < cnt) __hip_atomic_fetch_add(&L.accd[slot[t]], (double)acc[t], __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                hcur = ek;
-                blk = min(ej, ek) * m.nel + max(ej, ek);
-                const I2 bt = L.btab[blk];
-                u0 = bt.x;
-                cnt = bt.y;
+        for (int t = 0; t < kRowFns; ++t) {
+            const Vec4<float>::T P = t < cnt ? tab[u0 + t] : Vec4<float>::T{};
+            ea[t] = P.x;
+            ze[t] = P.y;
+            cz[t] = 1.0f - P.y;
+            lam[t] = P.z;
+            slot[t] = __float_as_int(P.w);
+            acc[t] = 0;
+            accd[t] = 0;
+        }
+        int pk = q < wc ? tl[ws + q] : 0;
+        for (int st = 0; st < T; ++st) {
+            if (st % kSumRun == 0 && st > 0) {
 #pragma unroll
                 for (int t = 0; t < kRowFns; ++t) {
-                    const Vec4<float>::T P = tab[min(u0 + t, g4n - 1)];
-                    ea[t] = P.x;
-                    ze[t] = P.y;
-                    cz[t] = 1.0f - P.y;
-                    lam[t] = P.z;
-                    slot[t] = __float_as_int(P.w);
+                    accd[t] += (double)acc[t];
                     acc[t] = 0;
                 }
             }
-            Vec4<float>::T ak{}, bk{};
-            float rjk2 = 0;
-            bool valid = false;
-            if (cnt > 0) {
-                ak = L.a[k];
-                bk = L.b[k];
-                const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
-                rjk2 = fmaR(ez, ez, fmaR(ey, ey, ex * ex));
-                valid = rjk2 < rc2;
-            }
-            if (tl) {   // in the order of the sweep's steps; lanes active here have all seen the same steps
-                const unsigned long long mk = __ballot(valid);
-                const int pos = tbase + mbcnt(mk);
-                if (valid && pos < w.tcap) tl[pos] = j | (k << 10) | (blk << 20);
-                tbase += __popcll(mk);
-            }
-            if (!valid) continue;
+            const int idx = q + st * nl;
+            const int cur = pk;
+            if (idx + nl < wc) pk = tl[ws + idx + nl];   // the next step's triplet
+            if (idx >= wc) continue;
+            const int j = cur & 0x3FF, k = (cur >> 10) & 0x3FF;
+            const Vec4<float>::T aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
+            const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
+            const float rjk2 = fmaR(ez, ez, fmaR(ey, ey, ex * ex));
             const float fcjk = cutoff_in<float>(ct, M<float>::sqrt(rjk2), rrc);
-            const float cs = (fmaR(aj.z, ak.z, fmaR(aj.y, ak.y, aj.x * ak.x)) * aj.w) * ak.w;
+            float bp, bm;
+            one_pm_cos(aj, ak, bp, bm);
             const float F = (bj.y * bk.y) * fcjk;
             const float r2 = (bj.w + bk.w) + rjk2;
-            const float Lp = log2_base(1.0f + cs), Lm = log2_base(1.0f - cs);
+            const float Lp = log2_base(bp), Lm = log2_base(bm);
 #pragma unroll
             for (int t = 0; t < kRowFns; ++t) {
                 if (t >= cnt) break;
@@ -1587,18 +1836,27 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
             for (int t = kRowFns; t < cnt; ++t) {   // functions beyond the registers' (not in common models)
                 const Vec4<float>::T P = tab[u0 + t];
                 const float Ls = P.z > 0.0f ? Lp : Lm;
-                const float T = __builtin_amdgcn_exp2f(fmaR(P.y, Ls, fmaR(P.x, r2, 1.0f - P.y))) * F;
-                __hip_atomic_fetch_add(&L.accd[__float_as_int(P.w)], (double)T, __ATOMIC_RELAXED,
+                const float Tv = __builtin_amdgcn_exp2f(fmaR(P.y, Ls, fmaR(P.x, r2, 1.0f - P.y))) * F;
+                __hip_atomic_fetch_add(&L.accd[__float_as_int(P.w)], (double)Tv, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
 #pragma unroll
         for (int t = 0; t < kRowFns; ++t)
-            if (t < cnt) __hip_atomic_fetch_add(&L.accd[slot[t]], (double)acc[t], __ATOMIC_RELAXED,
+            if (t < cnt) __hip_atomic_fetch_add(&L.accd[slot[t]], accd[t] + (double)acc[t], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-        tbase = wave_max(tbase);   // the lanes that stepped longest saw every step
+        over = over || full;
+        if (!over && w.tblk && lane < nw) {   // the blocks of the whole list, for pass B
+            const I2 wx = wt[lane];
+            w.tblk[(size_t)i * kTripBlk + 1 + lane] =
+                ((int)wx.x & 0xFFFF) | ((int)wx.y & 0xFFFF) << 11 | (bt[(int)wx.y >> 16].w & 0xFF) << 23;
+        }
+        __syncthreads();
     }
-    if (tl && lane == 0) w.tcount[i] = tbase;
+    if (lane == 0) {
+        w.tcount[i] = over ? cap + 1 : tot;   // cap + 1: pass B compacts its own
+        if (w.tblk) w.tblk[(size_t)i * kTripBlk] = over ? 0 : nw;
+    }
     __syncthreads();
     // centre and scale, hdnnps.f90:471-481
     const int so = ei.sf_off;
@@ -1617,8 +1875,8 @@ static bool sf_direct(const DevModel &m, int ga, int nmax)
         const char *v = std::getenv("HDNNP_SF_DIRECT");
         return !(v && std::strcmp(v, "0") == 0);
     }();
-    return on && ga == 1 && m.g4_lampm && sf_allpairs_enabled() &&
-           direct_offsets(nmax, m.nsfmax, m.g4_max, m.nel).total <= 65536;
+    return on && ga == 1 && m.g4_lampm && m.g4_nblk <= 48 && sf_allpairs_enabled() &&
+           direct_offsets(nmax, m.nsfmax, m.g4_max, m.g4_nblk).total <= 65536;
 }
 
 // wavefronts of a launch: tiles of GA atoms of one element
@@ -1638,10 +1896,13 @@ bool sf_allpairs_enabled()
 template <typename R>
 void launch_sf_values(const DevModel &m, const Geo &g, Work &w, int nmax, int ga, R *G, hipStream_t s)
 {
+    const Work wk = w;
+    w.tblk = nullptr;   // only the FP32 block-sweep pass writes the block tables
     if constexpr (sizeof(R) == 4) {
         const int nb = sf_bucket(nmax);
-        if (sf_direct(m, ga, nb)) {
-            const size_t lds = direct_offsets(nb, m.nsfmax, m.g4_max, m.nel).total;
+        if (sf_direct(m, ga, nb) && w.trip && w.tcount) {   // the pass writes its triplets to the list
+            w.tblk = wk.tblk;
+            const size_t lds = direct_offsets(nb, m.nsfmax, m.g4_max, m.g4_nblk).total;
 #define HDNNP_SFD(N)                                                                                     \
     do {                                                                                                 \
         if (m.cutoff_type == 2)                                                                          \
@@ -2133,7 +2394,7 @@ size_t sf_lds_bytes(bool fp64, int nmax, const DevModel &m, bool grad)
 {
     nmax = sf_bucket(nmax);
     const int nsfmax = m.nsfmax, g4max = m.g4_max, nel = m.nel;
-    if (!fp64 && !grad && sf_direct(m, 1, nmax)) return direct_offsets(nmax, nsfmax, g4max, nel).total;
+    if (!fp64 && !grad && sf_direct(m, 1, nmax)) return direct_offsets(nmax, nsfmax, g4max, m.g4_nblk).total;
     if (!grad && sf_allpairs_enabled())
         return fp64 ? lds_bytes<double>(nmax, 1, nsfmax, g4max, nel, false, true)
                     : lds_bytes<float>(nmax, 1, nsfmax, g4max, nel, false, true);
