@@ -1102,6 +1102,7 @@ __device__ __forceinline__ void one_pm_cos(const Vec4<float>::T &aj, const Vec4<
 // that deal the lanes out to the blocks
 constexpr int kRowFns = 4;
 constexpr int kSumRun = 4;   // terms summed in FP32 before they go to the FP64 sums
+constexpr int kFlatLen = 4;  // columns per item of the values pass's flat schedule
 
 // ceil(c / T) for 0 <= c < 2^22 and T >= 1, with rT about 1 / T (the
 // quotient of the product is off by at most one, which the remainder fixes)
@@ -1127,6 +1128,17 @@ __device__ __forceinline__ int wave_sum7(int v)
 #pragma unroll
     for (int bit = 0; bit < 7; ++bit) sum += __popcll(__ballot((v >> bit) & 1)) << bit;
     return sum;
+}
+
+// inclusive prefix sum over the lanes of the wavefront
+__device__ __forceinline__ int wave_scan_incl(int v)
+{
+    const int lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
+    return v;
 }
 
 // Steps T of a wavefront dealt out to nw (< 64) blocks of a list of V
@@ -1687,12 +1699,94 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
     const int cap = w.tcap;
     int b = 0, P = 0, sp = 0, tot = 0, nw = 0;   // the sweep's block, pass and step
     bool over = false;
+    // Flat schedule, when the list has a place for every candidate pair and
+    // pass B reads it through the block table: block b gets the list region
+    // [base_b, base_b + np_b), and the items (row, chunk of at most
+    // kFlatLen columns) of all blocks, in block order, run in passes of 64
+    // lanes (passes mix blocks; no block ends a pass early). The triplets of
+    // a step go to their blocks' regions, ranked within the block's lanes.
+    // wt[b] = (base | first item << 16, triplets so far | b << 16); bt[b].w
+    // takes the flat chunks and length.
+    bool flat = false;
+    int items_all = 0;
+    if (w.tblk) {
+        int items_l = 0, np_l = 0, Cf = 1, lenf = 0;
+        if (lane < nbk) {
+            const I4 d = bt[lane];
+            const int na = d.x >> 16, nb = d.y >> 16;
+            const bool same = (d.w >> 8) & 1;
+            const int R = same ? na / 2 : nb;
+            Cf = max(1, (R + kFlatLen - 1) / kFlatLen);
+            lenf = (R + Cf - 1) / Cf;
+            items_l = na > 0 && R > 0 ? na * Cf : 0;
+            np_l = same ? na * (na - 1) / 2 : na * nb;
+        }
+        const int items_in = wave_scan_incl(items_l), np_in = wave_scan_incl(np_l);
+        items_all = __builtin_amdgcn_readlane(items_in, 63);
+        flat = __builtin_amdgcn_readlane(np_in, 63) <= cap;
+        if (flat && lane < nbk) {
+            bt[lane].w = (bt[lane].w & 0x1FF) | Cf << 9 | lenf << 16;
+            wt[lane] = I2{(np_in - np_l) | (items_in - items_l) << 16, lane << 16};
+        }
+        __syncthreads();
+    }
     while (b < nbk) {
         // phase 1: the triplets with r_jk < rc of the blocks, in block order,
         // into the list (a window of it: the list holds cap entries)
         bool full = false;
         tot = 0;
         nw = 0;
+        if (flat) {
+            for (int P0 = 0; P0 < items_all; P0 += 64) {
+                const int it = P0 + lane;
+                int bb = 0;   // the item's block: the last with first item <= it
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (bb + st < nbk && ((int)wt[bb + st].x >> 16) <= it) bb += st;
+                const I4 d = bt[bb];
+                const I2 wx = wt[bb];
+                const int sa = d.x & 0xFFFF, na = d.x >> 16, sb = d.y & 0xFFFF, nb = d.y >> 16;
+                const bool same = (d.w >> 8) & 1;
+                const int Cf = (d.w >> 9) & 0x7F, lenf = d.w >> 16, code = (d.w & 0xFF) << 20;
+                const int first = (int)wx.x >> 16, base = (int)wx.x & 0xFFFF;
+                const int R = same ? na / 2 : nb, nitems = na * Cf;
+                const bool live = it < items_all;
+                const int loc = it - first;
+                const int c = row_of(loc, __builtin_amdgcn_rcpf((float)max(na, 1))), a = loc - __mul24(c, na);
+                const int o0 = c * lenf;
+                int oend = live ? min(R, o0 + lenf) : 0;
+                if (same && (na & 1) == 0 && a >= na / 2) oend = min(oend, R - 1);
+                const int j = sa + a;
+                const Vec4<float>::T aj = L.a[j];
+                int kk = a + o0 + (int)same;
+                if (kk >= nb) kk -= nb;
+                if (kk >= nb) kk -= nb;
+                if (!live || kk >= nb) kk = 0;
+                // the lanes of this block in the pass
+                const int lo = max(first - P0, 0), hi = min(first + nitems - P0, 64);
+                const unsigned long long bmask =
+                    live ? (hi >= 64 ? ~0ull : (1ull << hi) - 1) & ~((1ull << lo) - 1) : 0ull;
+                int cnt = (int)wx.y & 0xFFFF;
+                const int jc = j | code;
+                LDS const Vec4<float>::T *colp = L.a + sb;
+#pragma unroll
+                for (int s = 0; s < kFlatLen; ++s) {
+                    const Vec4<float>::T ak = colp[kk];
+                    const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
+                    const bool valid = (o0 + s < oend) & (fmaR(ez, ez, fmaR(ey, ey, ex * ex)) < rc2);
+                    const unsigned long long mk = __ballot(valid) & bmask;
+                    if (valid) tl[base + cnt + mbcnt(mk)] = jc | ((sb + kk) << 10);
+                    cnt += __popcll(mk);
+                    kk = kk + 1 == nb ? 0 : kk + 1;
+                }
+                __syncthreads();
+                if (live && lane == hi - 1) wt[bb].y = cnt | bb << 16;
+                __syncthreads();
+            }
+            b = nbk;
+            nw = nbk;
+            tot = __builtin_amdgcn_readlane(wave_scan_incl(lane < nbk ? ((int)wt[lane].y & 0xFFFF) : 0), 63);
+        } else
         for (; b < nbk; ++b, P = 0, sp = 0) {
             const I4 d = bt[b];
             const int dx = uni(d.x), dy = uni(d.y), dw = uni(d.w);
@@ -1767,7 +1861,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
             const int cx = lane < nw ? ((int)wt[lane].y & 0xFFFF) : 0;
             int nlx;
             deal_lanes(V, nw, cx, T, nlx);
-            if (lane < nw) wt[lane].x |= nlx << 16;
+            if (lane < nw) wt[lane].x = ((int)wt[lane].x & 0xFFFF) | nlx << 16;
         }
         __syncthreads();
         int ws = 0, wc = 0, nl = 1, q = 0, u0 = 0, cnt = 0;
@@ -1849,7 +1943,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
         if (!over && w.tblk && lane < nw) {   // the blocks of the whole list, for pass B
             const I2 wx = wt[lane];
             w.tblk[(size_t)i * kTripBlk + 1 + lane] =
-                ((int)wx.x & 0xFFFF) | ((int)wx.y & 0xFFFF) << 11 | (bt[(int)wx.y >> 16].w & 0xFF) << 23;
+                ((int)wx.x & 0x7FF) | ((int)wx.y & 0xFFFF) << 11 | (bt[(int)wx.y >> 16].w & 0xFF) << 23;
         }
         __syncthreads();
     }

@@ -78,13 +78,7 @@ def test_beyond_staging_capacity(pkg, wl, precision):
     assert np.isfinite(E) and np.all(np.isfinite(F))
 
 
-def test_triplet_list_fallback_fp64(pkg, wl, oracles):
-    """FP64 hands the value pass's triplets to the gradient pass, in per-atom
-    lists sized from the mean density. A dense 120-atom C/N/H cluster in a
-    60-bohr periodic box, with 40 dilute atoms around it, has ~100
-    neighbours per cluster atom at a mean of ~4: the cluster atoms overflow
-    their lists and are compacted by the gradient pass itself, while the
-    dilute atoms use theirs. Both against the oracle."""
+def _dense_in_dilute(wl):
     c = wl.cluster(120, seed=11, box=14.0, dmin=1.6)
     rng = np.random.default_rng(12)
     gas = []
@@ -94,7 +88,17 @@ def test_triplet_list_fallback_fp64(pkg, wl, oracles):
             gas.append(p)
     pos = np.vstack([c.pos + 23.0, np.array(gas)])
     Z = np.concatenate([c.Z, rng.choice(np.array([1, 6, 7], np.int32), size=len(gas))]).astype(np.int32)
-    s = wl.Structure(pos, Z, np.eye(3) * 60.0, True)
+    return wl.Structure(pos, Z, np.eye(3) * 60.0, True)
+
+
+def test_triplet_list_fallback_fp64(pkg, wl, oracles):
+    """FP64 hands the value pass's triplets to the gradient pass, in per-atom
+    lists sized from the mean density. A dense 120-atom C/N/H cluster in a
+    60-bohr periodic box, with 40 dilute atoms around it, has ~100
+    neighbours per cluster atom at a mean of ~4: the cluster atoms overflow
+    their lists and are compacted by the gradient pass itself, while the
+    dilute atoms use theirs. Both against the oracle."""
+    s = _dense_in_dilute(wl)
     h = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64")
     E, F, L = h.energy_and_forces(s)
     ent = h.neighbor_entries()
@@ -105,3 +109,26 @@ def test_triplet_list_fallback_fp64(pkg, wl, oracles):
     assert abs(E - r["energy"]) <= 1e-10 * max(1.0, abs(E))
     np.testing.assert_allclose(F, r["force"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(L, r["dEdLat"], rtol=0, atol=1e-9 * max(1.0, np.abs(r["dEdLat"]).max()))
+
+
+def test_triplet_list_windows_mixed(pkg, wl, oracles):
+    """The same structure in mixed precision. The FP32 value pass gives each
+    atom whose candidate pairs fit its list one region per element-pair block
+    (the dilute atoms); the cluster atoms (~5,000 candidate pairs against a
+    list of 2,048) are evaluated in windows of the list, and the gradient
+    pass compacts their triplets itself. Against the oracle within the mixed
+    bounds, and against the FP64 evaluation."""
+    s = _dense_in_dilute(wl)
+    hm = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed")
+    Em, Fm, Lm = hm.energy_and_forces(s)
+    cnt = np.bincount(np.asarray(hm.neighbor_entries())[:, 0], minlength=s.nat)
+    print(f"neighbours: most {cnt.max()}, atoms with more than 64: {(cnt > 64).sum()}", flush=True)
+    assert (cnt > 64).sum() >= 20 and cnt[120:].max() < 20   # pairs(65) = 2,080 > 2,048
+    E, F, L = pkg.parseRuNNerInput(PEROV_PARAMS, "fp64").energy_and_forces(s)
+    r = oracles["perovskite"].evaluate(s)
+    rmse = float(np.sqrt(np.mean((Fm - r["force"]) ** 2)))
+    print(f"mixed: dE/atom {abs(Em - r['energy']) / s.nat:.3e} F rmse {rmse:.3e} "
+          f"dEdLat max {np.abs(Lm - r['dEdLat']).max():.3e}", flush=True)
+    assert abs(Em - r["energy"]) / s.nat <= 1e-5 * max(1.0, abs(r["energy"]) / s.nat)
+    assert rmse <= 1e-5 * max(1.0, float(np.sqrt(np.mean(r["force"] ** 2))))
+    assert np.abs(Lm - L).max() <= 1e-5 * max(1.0, np.abs(L).max())
