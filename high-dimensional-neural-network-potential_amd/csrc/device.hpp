@@ -133,8 +133,9 @@ struct Work {
     int *trip, *tcount;
     int tcap;
     // Per atom kTripBlk ints: the element-pair blocks of its triplet list
-    // (which the FP32 values pass writes in block order): the count, then
-    // start | count << 11 | block << 23 each. null: not written.
+    // (which the FP32 values pass writes in block order): the count (| 1 <<
+    // 16 when the list holds 16-bit entries j | k << 8, one region per
+    // block), then start | count << 11 | block << 23 each. null: not written.
     int *tblk;
     int *ent_j;          // [nat*cap] neighbour atom
     int *ent_shift;      // [nat*cap] packed image shift (3 x int8)

@@ -1273,15 +1273,17 @@ __device__ __forceinline__ void g4_grad_list(const DevModel &m, const Lds<R> &L,
 }
 
 // Pass B (FP32, LOGF) over the atom's triplet list in block order, as the
-// values pass left it: tb = (entries, then start | count << 11 | block << 23
-// each). The lanes are dealt out to the blocks in proportion to their
+// values pass left it: tb = (entries | 16-bit list << 16, then start | count
+// << 11 | block << 23 each). The lanes are dealt out to the blocks in proportion to their
 // triplets; a lane keeps its block's function weights in registers.
 template <int CT>
 __device__ __forceinline__ void g4_grad_blocks(const DevModel &m, const Lds<float> &L, const int *tl, const int *tb,
                                                int V, float rrc)
 {
     const int lane = lane_id();
-    const int nw = uni(tb[0]);
+    const int nw = uni(tb[0]) & 0xFFFF;
+    const bool f16 = (uni(tb[0]) >> 16) & 1;   // 16-bit entries j | k << 8
+    const unsigned short *tl16 = (const unsigned short *)tl;
     const int ent = lane < nw ? tb[1 + lane] : 0;
     const int cx = (ent >> 11) & 0xFFF;
     int T, nlx;
@@ -1311,13 +1313,14 @@ __device__ __forceinline__ void g4_grad_blocks(const DevModel &m, const Lds<floa
         zm[t] = t < cnt ? __int_as_float(L.smeta[u0 + t]) : 0.0f;
         q3[t] = t < cnt ? L.sp1[u0 + t] : 0.0f;
     }
-    int pk = q < wc ? tl[ws + q] : 0;
+    auto entry = [&](int x) { return f16 ? (int)tl16[x] : tl[x]; };
+    int pk = q < wc ? entry(ws + q) : 0;
     for (int st = 0; st < T; ++st) {
         const int idx = q + st * nl;
         const int cur = pk;
-        if (idx + nl < wc) pk = tl[ws + idx + nl];   // the next step's triplet
+        if (idx + nl < wc) pk = entry(ws + idx + nl);   // the next step's triplet
         if (idx >= wc) continue;
-        g4_grad_triplet<float, CT>(m, L, cur & 0x3FF, (cur >> 10) & 0x3FF, rrc,
+        g4_grad_triplet<float, CT>(m, L, f16 ? cur & 0xFF : cur & 0x3FF, f16 ? cur >> 8 : (cur >> 10) & 0x3FF, rrc,
                                    [&](const Vec4<float>::T &aj, const Vec4<float>::T &ak, float, float r2, float &S1,
                                        float &S2, float &S3) {
                                        float bp, bm;
@@ -1696,6 +1699,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
     __syncthreads();
     const float rrc = 1.0f / rce, rc2 = rce * rce;
     int *tl = w.trip + (size_t)i * w.tcap;
+    unsigned short *tl16 = (unsigned short *)tl;   // the flat schedule's entries
     const int cap = w.tcap;
     int b = 0, P = 0, sp = 0, tot = 0, nw = 0;   // the sweep's block, pass and step
     bool over = false;
@@ -1705,6 +1709,9 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
     // kFlatLen columns) of all blocks, in block order, run in passes of 64
     // lanes (passes mix blocks; no block ends a pass early). The triplets of
     // a step go to their blocks' regions, ranked within the block's lanes.
+    // The entries are 16-bit, j | k << 8 (the block is the region's): the
+    // list an atom writes and reads back is half the size, and more of it is
+    // still in L2 when phase 2 reads it.
     // wt[b] = (base | first item << 16, triplets so far | b << 16); bt[b].w
     // takes the flat chunks and length.
     bool flat = false;
@@ -1723,7 +1730,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
         }
         const int items_in = wave_scan_incl(items_l), np_in = wave_scan_incl(np_l);
         items_all = __builtin_amdgcn_readlane(items_in, 63);
-        flat = __builtin_amdgcn_readlane(np_in, 63) <= cap;
+        flat = __builtin_amdgcn_readlane(np_in, 63) <= cap && uni(L.seg[kMaxEl]) <= 256;   // 8-bit j and k
         if (flat && lane < nbk) {
             bt[lane].w = (bt[lane].w & 0x1FF) | Cf << 9 | lenf << 16;
             wt[lane] = I2{(np_in - np_l) | (items_in - items_l) << 16, lane << 16};
@@ -1747,7 +1754,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
                 const I2 wx = wt[bb];
                 const int sa = d.x & 0xFFFF, na = d.x >> 16, sb = d.y & 0xFFFF, nb = d.y >> 16;
                 const bool same = (d.w >> 8) & 1;
-                const int Cf = (d.w >> 9) & 0x7F, lenf = d.w >> 16, code = (d.w & 0xFF) << 20;
+                const int Cf = (d.w >> 9) & 0x7F, lenf = d.w >> 16;
                 const int first = (int)wx.x >> 16, base = (int)wx.x & 0xFFFF;
                 const int R = same ? na / 2 : nb, nitems = na * Cf;
                 const bool live = it < items_all;
@@ -1767,7 +1774,6 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
                 const unsigned long long bmask =
                     live ? (hi >= 64 ? ~0ull : (1ull << hi) - 1) & ~((1ull << lo) - 1) : 0ull;
                 int cnt = (int)wx.y & 0xFFFF;
-                const int jc = j | code;
                 LDS const Vec4<float>::T *colp = L.a + sb;
 #pragma unroll
                 for (int s = 0; s < kFlatLen; ++s) {
@@ -1775,7 +1781,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
                     const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
                     const bool valid = (o0 + s < oend) & (fmaR(ez, ez, fmaR(ey, ey, ex * ex)) < rc2);
                     const unsigned long long mk = __ballot(valid) & bmask;
-                    if (valid) tl[base + cnt + mbcnt(mk)] = jc | ((sb + kk) << 10);
+                    if (valid) tl16[base + cnt + mbcnt(mk)] = (unsigned short)(j | (sb + kk) << 8);
                     cnt += __popcll(mk);
                     kk = kk + 1 == nb ? 0 : kk + 1;
                 }
@@ -1786,6 +1792,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
             b = nbk;
             nw = nbk;
             tot = __builtin_amdgcn_readlane(wave_scan_incl(lane < nbk ? ((int)wt[lane].y & 0xFFFF) : 0), 63);
+
         } else
         for (; b < nbk; ++b, P = 0, sp = 0) {
             const I4 d = bt[b];
@@ -1898,7 +1905,8 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
             acc[t] = 0;
             accd[t] = 0;
         }
-        int pk = q < wc ? tl[ws + q] : 0;
+        auto entry = [&](int x) { return flat ? (int)tl16[x] : tl[x]; };
+        int pk = q < wc ? entry(ws + q) : 0;
         for (int st = 0; st < T; ++st) {
             if (st % kSumRun == 0 && st > 0) {
 #pragma unroll
@@ -1909,9 +1917,9 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
             }
             const int idx = q + st * nl;
             const int cur = pk;
-            if (idx + nl < wc) pk = tl[ws + idx + nl];   // the next step's triplet
+            if (idx + nl < wc) pk = entry(ws + idx + nl);   // the next step's triplet
             if (idx >= wc) continue;
-            const int j = cur & 0x3FF, k = (cur >> 10) & 0x3FF;
+            const int j = flat ? cur & 0xFF : cur & 0x3FF, k = flat ? cur >> 8 : (cur >> 10) & 0x3FF;
             const Vec4<float>::T aj = L.a[j], ak = L.a[k], bj = L.b[j], bk = L.b[k];
             const float ex = ak.x - aj.x, ey = ak.y - aj.y, ez = ak.z - aj.z;
             const float rjk2 = fmaR(ez, ez, fmaR(ey, ey, ex * ex));
@@ -1949,7 +1957,7 @@ __global__ __launch_bounds__(64) void k_sf_values_direct(DevModel m, Geo g, Work
     }
     if (lane == 0) {
         w.tcount[i] = over ? cap + 1 : tot;   // cap + 1: pass B compacts its own
-        if (w.tblk) w.tblk[(size_t)i * kTripBlk] = over ? 0 : nw;
+        if (w.tblk) w.tblk[(size_t)i * kTripBlk] = over ? 0 : nw | (int)flat << 16;
     }
     __syncthreads();
     // centre and scale, hdnnps.f90:471-481
