@@ -103,6 +103,8 @@ struct MlpShape {
 };
 
 constexpr int kKU = 8;   // matrix-core steps whose operands are loaded together
+// output column groups of 16 computed together (their loads go out at once)
+template <typename R> constexpr int kJG = sizeof(R) == 4 ? 2 : 1;
 
 // Ei_out[i] = E_i of this network; dEdG = scale * dE_i/dG, or added to dEdG
 // when accumulate is set (committee mean over several networks). BF16 (FP32
@@ -167,10 +169,11 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         R *c = Cb + sh.coff[l];
         const int ldc = sh.ldc[l];
         const char act = sh.act[l];
+        if constexpr (BF16 && sizeof(R) == 4) {
         for (int j0 = 0; j0 < nout; j0 += 16) {
             V acc = {0, 0, 0, 0};
             const int j = j0 + ri;
-            if constexpr (BF16 && sizeof(R) == 4) {
+            {
                 // 16 k per instruction, two instructions per step
                 for (int k0 = 0; k0 < nin; k0 += 32) {
                     Bf4 a[2], b[2];
@@ -187,20 +190,6 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
 #pragma unroll
                     for (int u = 0; u < 2; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
                 }
-            } else {
-                // kKU steps at a time, operands loaded first; steps past nin
-                // multiply zeros, which leaves the accumulator as it is
-                for (int k0 = 0; k0 < nin; k0 += 4 * kKU) {
-                    R a[kKU], b[kKU];
-#pragma unroll
-                    for (int u = 0; u < kKU; ++u) {
-                        const int k = k0 + 4 * u + kq;
-                        a[u] = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
-                        b[u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
-                }
             }
             if (j < nout) {
                 const R bj = bias[j];
@@ -212,6 +201,49 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
                     out[row * ld + j] = act_f(act, cv);
                 }
             }
+        }
+        } else {
+        // kJG column groups of 16 at a time, kKU steps at a time: the
+        // operands of a step are loaded once for all the groups, and the
+        // loads of all (groups x steps) go out before their MFMAs; steps past
+        // nin multiply zeros, which leaves the accumulators as they are
+        for (int j00 = 0; j00 < nout; j00 += 16 * kJG<R>) {
+            V acc[kJG<R>];
+#pragma unroll
+            for (int g = 0; g < kJG<R>; ++g) acc[g] = V{0, 0, 0, 0};
+            for (int k0 = 0; k0 < nin; k0 += 4 * kKU) {
+                R a[kKU], b[kJG<R>][kKU];
+#pragma unroll
+                for (int u = 0; u < kKU; ++u) {
+                    const int k = k0 + 4 * u + kq;
+                    a[u] = k < nin ? (l == 0 ? (live ? Grow[k] : R(0)) : in[ri * ld + k]) : R(0);
+#pragma unroll
+                    for (int g = 0; g < kJG<R>; ++g) {
+                        const int j = j00 + 16 * g + ri;
+                        b[g][u] = (k < nin && j < nout) ? wp[(size_t)k * nout + j] : R(0);
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < kJG<R>; ++g)
+                    if (j00 + 16 * g < nout)
+#pragma unroll
+                        for (int u = 0; u < kKU; ++u) acc[g] = Mma<R>::op(a[u], b[g][u], acc[g]);
+            }
+#pragma unroll
+            for (int g = 0; g < kJG<R>; ++g) {
+                const int j = j00 + 16 * g + ri;
+                if (j < nout) {
+                    const R bj = bias[j];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = Mma<R>::row(kq, r);
+                        const R cv = acc[g][r] + bj;
+                        c[row * ldc + j] = cv;
+                        out[row * ld + j] = act_f(act, cv);
+                    }
+                }
+            }
+        }
         }
         __syncthreads();
         wp = bias + nout;
@@ -243,30 +275,42 @@ __global__ __launch_bounds__(64) void k_mlp(DevModel m, Work w, MlpShape sh, con
         __syncthreads();
         R *gn = (g == P0) ? P1 : P0;
         const R *wl = Wl[l];
-        for (int j0 = 0; j0 < ni; j0 += 16) {
-            V acc = {0, 0, 0, 0};
-            const int j = j0 + ri;
+        for (int j00 = 0; j00 < ni; j00 += 16 * kJG<R>) {
+            V acc[kJG<R>];
+#pragma unroll
+            for (int gi = 0; gi < kJG<R>; ++gi) acc[gi] = V{0, 0, 0, 0};
             for (int k0 = 0; k0 < nout; k0 += 4 * kKU) {
-                R a[kKU], b[kKU];
+                R a[kKU], b[kJG<R>][kKU];
 #pragma unroll
                 for (int u = 0; u < kKU; ++u) {
                     const int k = k0 + 4 * u + kq;
                     a[u] = k < nout ? g[ri * ld + k] : R(0);
-                    b[u] = (k < nout && j < ni) ? wl[(size_t)j * nout + k] : R(0);
+#pragma unroll
+                    for (int gi = 0; gi < kJG<R>; ++gi) {
+                        const int j = j00 + 16 * gi + ri;
+                        b[gi][u] = (k < nout && j < ni) ? wl[(size_t)j * nout + k] : R(0);
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < kKU; ++u) acc = Mma<R>::op(a[u], b[u], acc);
-            }
-            if (j < ni) {
+                for (int gi = 0; gi < kJG<R>; ++gi)
+                    if (j00 + 16 * gi < ni)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = Mma<R>::row(kq, r);
-                    if (l > 0) {
-                        gn[row * ld + j] = acc[r];
-                    } else if (row < tn) {
-                        R *d = dEdG + (size_t)rows[row] * ldg + j;
-                        const R v = scale * acc[r];
-                        *d = accumulate ? *d + v : v;
+                        for (int u = 0; u < kKU; ++u) acc[gi] = Mma<R>::op(a[u], b[gi][u], acc[gi]);
+            }
+#pragma unroll
+            for (int gi = 0; gi < kJG<R>; ++gi) {
+                const int j = j00 + 16 * gi + ri;
+                if (j < ni) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = Mma<R>::row(kq, r);
+                        if (l > 0) {
+                            gn[row * ld + j] = acc[gi][r];
+                        } else if (row < tn) {
+                            R *d = dEdG + (size_t)rows[row] * ldg + j;
+                            const R v = scale * acc[gi][r];
+                            *d = accumulate ? *d + v : v;
+                        }
                     }
                 }
             }
