@@ -52,6 +52,11 @@ namespace hdnnp {
 // way, the larger one for fewer rounds
 template <typename R> constexpr int kPairChunk = sizeof(R) == 4 ? 256 : 512;
 
+// The compact triplet lists the passes hand over hold 16-bit entries j | k
+// << 8 when the atom has at most 256 staged neighbours (both passes know n),
+// else the 32-bit packing j | k << 10 | block << 20
+__device__ __forceinline__ bool trip16(int n) { return n <= 256; }
+
 // ---------------------------------------------------------------- math
 template <typename R> struct M;
 template <> struct M<double> {
@@ -892,9 +897,13 @@ __device__ __forceinline__ int g4_blocks(const DevModel &m, const Lds<R> &L, int
                 __syncthreads();
                 if (!GRAD && GL == 64 && sizeof(R) == 8 && tl) {   // FP64 only (the engine passes none in FP32)
                     const int blk = f * m.nel + h;
+                    const bool t16 = trip16(uni(L.seg[kMaxEl]));
                     for (int q = lane_id(); q < nvw; q += 64) {
                         const int pk = L.pairs[q];
-                        if (tbase + q < tcap) tl[tbase + q] = (pk & 0xFFFF) | ((pk >> 16) << 10) | (blk << 20);
+                        if (tbase + q < tcap) {
+                            if (t16) ((unsigned short *)tl)[tbase + q] = (unsigned short)((pk & 0xFF) | ((pk >> 16) << 8));
+                            else tl[tbase + q] = (pk & 0xFFFF) | ((pk >> 16) << 10) | (blk << 20);
+                        }
                     }
                     tbase += nvw;
                 }
@@ -1355,12 +1364,15 @@ __device__ __forceinline__ void g4_grad_list_ct(const DevModel &m, const Lds<R> 
 
 // ---------------------------------------------------------------- pass A
 // Waves per SIMD the FP64 passes at the 96-neighbour staging, one atom per
-// wavefront, are compiled for: their LDS allows 4 (values) and 3
-// (gradients), and without the bound the compiler keeps exp_core's 13
-// constants in VGPRs and lands at 2. With it, a few values spill to scratch
-// outside the triplet loops.
+// wavefront, are compiled for: 3. Without the bound the compiler keeps
+// exp_core's 13 constants in VGPRs and lands at 2. The values pass fits in
+// the 168 VGPRs of 3 waves with no scratch (at 4, which its LDS allows on
+// three SIMDs of four, it spilled 40 bytes per lane, 145 MB of writes per
+// 49,152 atoms, for 6% less time); the gradient pass spills 60 bytes per
+// lane outside the triplet loops (at 2 waves it spills nothing but takes
+// 20% longer).
 template <typename R, int NMAX, int GA, bool GRAD>
-constexpr int kSfWaves = sizeof(R) == 8 && NMAX <= 96 && GA == 1 ? (GRAD ? 3 : 4) : 1;
+constexpr int kSfWaves = sizeof(R) == 8 && NMAX <= 96 && GA == 1 ? 3 : 1;
 
 // ALLP (one atom per wavefront): G4 over all pairs of the atom, in pass B's
 // order, with the function sums in FP64 (g4_values_list); the compacted
@@ -1417,8 +1429,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         for (int p0 = 0; p0 < np;) {
             const int nv = uni(compact_circ_fill<R>(L, p0, np, kAllPairsChunk, n, inv_n, rce * rce, m.nel));
             __syncthreads();
-            if (tl)
-                for (int q = gl; q < nv && tbase + q < w.tcap; q += 64) tl[tbase + q] = L.pairs[q];
+            if (tl) {
+                const bool t16 = trip16(n);
+                for (int q = gl; q < nv && tbase + q < w.tcap; q += 64) {
+                    const int pk = L.pairs[q];
+                    if (t16) ((unsigned short *)tl)[tbase + q] = (unsigned short)((pk & 0xFF) | ((pk >> 10) & 0xFF) << 8);
+                    else tl[tbase + q] = pk;
+                }
+            }
             tbase += nv;
             if (m.cutoff_type == 2) g4_values_list<R, 2>(m, L, nv, ei.g4_start, rrc);   // tanh^3 inlined
             else g4_values_list<R, -1>(m, L, nv, ei.g4_start, rrc);
@@ -1503,9 +1521,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSfWaves<R, 
         } else if (tc >= 0 && tc <= w.tcap) {
             // the triplets pass A compacted
             const int *tl = w.trip + (size_t)i * w.tcap;
+            const bool t16 = trip16(n);
             for (int cb = 0; cb < tc; cb += kPairChunk<R>) {
                 const int nv = min(kPairChunk<R>, tc - cb);
-                for (int q = gl; q < nv; q += 64) L.pairs[q] = tl[cb + q];
+                for (int q = gl; q < nv; q += 64) {
+                    if (t16) {   // the block from the two neighbours' elements
+                        const int e16 = ((const unsigned short *)tl)[cb + q];
+                        const int j = e16 & 0xFF, k = e16 >> 8, ej = L.el[j], ek = L.el[k];
+                        L.pairs[q] = j | k << 10 | (min(ej, ek) * m.nel + max(ej, ek)) << 20;
+                    } else {
+                        L.pairs[q] = tl[cb + q];
+                    }
+                }
                 __syncthreads();
                 g4_grad_list_ct<R>(m, L, nv, ei.g4_start, rrc);
                 __syncthreads();
