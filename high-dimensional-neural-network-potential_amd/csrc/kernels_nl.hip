@@ -483,8 +483,11 @@ __device__ inline bool stencil_cell(const StructGeo &g, int key, int s, StencilC
     return true;
 }
 
-// Neighbour lists of the atoms of one grid cell per wave (one-wave blocks).
-// All atoms of a cell share the stencil, so the candidates are enumerated
+// Neighbour lists of the atoms of one grid cell per wave (one-wave blocks),
+// or, on grids of few cells, of every split-th atom of the cell (wave p
+// takes atoms p, p + split, ...; the waves of a cell run on one XCD,
+// xcd_block): more waves in flight where the cells alone leave SIMDs idle.
+// The atoms of a wave share the stencil, so the candidates are enumerated
 // once and tested against each of them.
 //
 // Per chunk of 64 stencil cells each lane resolves one cell (wrapped id,
@@ -497,14 +500,15 @@ __device__ inline bool stencil_cell(const StructGeo &g, int key, int s, StencilC
 // the slot of that cell's first entry; the partner search uses it.
 // Entries beyond `cap` are counted but not stored; flags[1] then receives
 // the number needed so that the host can repeat the evaluation.
-__global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
+__global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w, int split)
 {
     __shared__ int pref[64], q0[64], info[64], ia[64];
     __shared__ double dl[3][64];
     __shared__ double4 pa[64];
     const int lane = threadIdx.x;
-    const int c = blockIdx.x;
-    const int a0 = w.cell_start[c], A = w.cell_end[c] - a0;
+    const int lb = split > 1 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int c = lb / split, part = lb - c * split;
+    const int a0 = w.cell_start[c], A = (w.cell_end[c] - a0 - part + split - 1) / split;   // this wave's atoms
     if (A <= 0) return;
     const StructGeo &G = g.sg[w.csid[c]];
     const int S = G.S;
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
         const int Ag = min(64, A - g0);
         __syncthreads();                      // previous group done with ia / pa
         if (lane < Ag) {
-            const int t = w.cell_atoms[a0 + g0 + lane];
+            const int t = w.cell_atoms[a0 + part + split * (g0 + lane)];
             ia[lane] = t;
             pa[lane] = w.wpos[t];
         }
@@ -604,7 +608,9 @@ __global__ __launch_bounds__(64) void k_nl_build(Geo g, Work w)
 
 void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nl_build, dim3(ncell), dim3(64), 0, s, g, w);
+    // waves per cell: up to 4 while the grid has fewer than 8,192 cells
+    const int split = ncell >= 8192 ? 1 : ncell >= 4096 ? 2 : 4;
+    if (ncell > 0) hipLaunchKernelGGL(k_nl_build, dim3(ncell * split), dim3(64), 0, s, g, w, split);
 }
 
 // Partner entries: entry (i -> j, image L) found in stencil cell s of i pairs
