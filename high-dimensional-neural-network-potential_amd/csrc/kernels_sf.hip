@@ -1111,7 +1111,7 @@ __device__ __forceinline__ void one_pm_cos(const Vec4<float>::T &aj, const Vec4<
 // that deal the lanes out to the blocks
 constexpr int kRowFns = 4;
 constexpr int kSumRun = 4;   // terms summed in FP32 before they go to the FP64 sums
-constexpr int kFlatLen = 4;  // columns per item of the values pass's flat schedule
+constexpr int kFlatLen = 8;  // columns per item of the values pass's flat schedule
 
 // ceil(c / T) for 0 <= c < 2^22 and T >= 1, with rT about 1 / T (the
 // quotient of the product is off by at most one, which the remainder fixes)
