@@ -6,6 +6,8 @@ set (as a potential trained with it would), so that the mixed-precision
 bound measures the arithmetic, not an input far outside the scaling."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -90,3 +92,24 @@ def test_g5_per_function_path(pkg, wl, oracle, tmp_path):
     E0, F0, _ = h.energy_and_forces(s)
     E1, F1, _ = pkg.hdnnpCalcNNs(h, s, nl, single)
     assert abs(E1 - E0) <= 1e-10 * abs(E0) and np.abs(F1 - F0).max() <= 1e-9
+
+
+def test_blocks_with_many_g4_functions(pkg, wl, oracle, tmp_path):
+    """The G4 functions of every centre element spread over three element
+    pairs only, ~13 per block (the shipped potentials have at most 4): the
+    FP32 passes keep 4 functions of a block in registers and take the others
+    from LDS per triplet, which this case runs. Against the oracle in FP64 and
+    mixed precision."""
+    d = potential_copy(tmp_path)
+    pairs = [("H", "H"), ("H", "C"), ("N", "I")]
+    out, s = [], 0
+    for ln in open(os.path.join(d, "input.nn")).read().splitlines():
+        w = ln.split()
+        if len(w) >= 8 and w[0] == "symfunction_short" and w[2] == "3":
+            a, b = pairs[s % len(pairs)]
+            s += 1
+            ln = " ".join([w[0], w[1], w[2], a, b] + w[5:])
+        out.append(ln)
+    assert s >= 3 * 12
+    open(os.path.join(d, "input.nn"), "w").write("\n".join(out) + "\n")
+    _compare(pkg, oracle, wl, rescale(d, oracle, wl))
