@@ -113,3 +113,22 @@ def test_blocks_with_many_g4_functions(pkg, wl, oracle, tmp_path):
     assert s >= 3 * 12
     open(os.path.join(d, "input.nn"), "w").write("\n".join(out) + "\n")
     _compare(pkg, oracle, wl, rescale(d, oracle, wl))
+
+
+def test_general_lambda_model(pkg, wl, oracle, tmp_path):
+    """A third of the G4 functions with lambda = +-0.5: the FP32 passes then
+    take the general path (per-term powers, the compacted triplet list that
+    pass A hands pass B in 16-bit entries) instead of the block sweep.
+    Against the oracle in FP64 and mixed precision."""
+    d = potential_copy(tmp_path)
+    out, s = [], 0
+    for ln in open(os.path.join(d, "input.nn")).read().splitlines():
+        w = ln.split()
+        if len(w) >= 8 and w[0] == "symfunction_short" and w[2] == "3":
+            if s % 3 == 0:
+                w[6] = "+0.5" if float(w[6]) > 0 else "-0.5"
+                ln = " ".join(w)
+            s += 1
+        out.append(ln)
+    open(os.path.join(d, "input.nn"), "w").write("\n".join(out) + "\n")
+    _compare(pkg, oracle, wl, rescale(d, oracle, wl))
