@@ -725,7 +725,7 @@ size_t Engine::footprint_bytes(const Footprint &f, bool slack) const
     const size_t nat = (size_t)f.nat;
     size_t b = nat * (9 * 4 + 32 + 8 * nm + 72 + 4 + 4 + 24 + 24 + 2 * R * nsfmax + 16);   // per atom
     b += nat * cap * (16 + 3 * R);                                                           // list slots
-    b += nat * (4 * (size_t)triplet_capacity(entry_capacity(f.est, cap_hint_, 1024)) + 4);   // triplets
+    b += nat * (4 * (size_t)triplet_capacity(entry_capacity(f.est, cap_hint_, 1024)) + 4 * (1 + kTripBlk));   // triplets, block tables
     b += (size_t)f.ncell * 12;                                                               // cells
     if ((size_t)f.ps * 2 <= kPosStartBytes) b += (size_t)f.ps * 2;                          // slot table
     b += (size_t)f.nstruct * (sizeof(StructGeo) + 80 * nm + 64 + 48);                       // per structure

@@ -111,8 +111,9 @@ int hdnnp_evaluate_batch_device(hdnnp_ctx *ctx, int32_t nstruct, const int32_t *
  * copy of the potential must exist already: hdnnp_create sets it up when a
  * GPU is visible (otherwise the call fails with HDNNP_ERR_DEVICE). Inputs
  * needing more scratch than the array holds are refused with
- * HDNNP_ERR_CAPACITY before anything is launched (about 25,000 atoms of the
- * 80-SF perovskite potential in mixed precision, 8,000 in FP64). All calls on
+ * HDNNP_ERR_CAPACITY before anything is launched (a cell of the 80-SF
+ * perovskite potential fits up to about 10,000 atoms in mixed precision,
+ * 8,400 in FP64). All calls on
  * one GPU share the array, which keeps the last call's contents; the library
  * serialises such calls (any contexts, any threads), so overlapping calls
  * wait for each other instead of racing. The context's own work buffers are
