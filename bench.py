@@ -208,9 +208,16 @@ def run_mode(pkg, torch, s, precision, steps, warmup, params=PARAMS):
         a = time.perf_counter()
         E, _, L = h.evaluate_torch(Z, pos, True, s.lat, force)
         lat_ms.append((time.perf_counter() - a) * 1e3)
-        stage.append(h.timings())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # the stage breakdown (and the roofline's kernel times): the same steps
+    # again with HIP events between the stages, which the rate above runs
+    # without (they are packets of their own on the stream)
+    h.set_stage_timings(True)
+    for _ in range(steps):
+        h.evaluate_torch(Z, pos, True, s.lat, force)
+        stage.append(h.timings())
+    h.set_stage_timings(False)
     return h, dict(E=E, F=force.cpu().numpy(), dEdLat=L, wall=wall, lat_ms=np.array(lat_ms),
                    stage=np.array(stage))
 
@@ -266,10 +273,14 @@ def bench_batch(args):
             lat.append(time.perf_counter() - a)
         torch.cuda.synchronize()
         t = float(np.median(lat))
+        h.set_stage_timings(True)                    # one more evaluation, with its stage times
+        h.evaluate_batch_torch(nat_s, Z, pos, per, lats, force)
+        stage = h.timings()
+        h.set_stage_timings(False)
         res[mode] = {"atoms_per_s": nat / t, "structures_per_s": len(ss) / t, "ms_per_batch": t * 1e3,
                      "p99_ms": float(np.percentile(lat, 99)) * 1e3,
                      "autotune": {"atoms_per_wave": tile, "sf_ms_per_tile": tile_ms},
-                     "stage_ms": np.round(h.timings(), 4).tolist(), "F": force.cpu().numpy(), "E": E, "L": L}
+                     "stage_ms": np.round(stage, 4).tolist(), "F": force.cpu().numpy(), "E": E, "L": L}
         print(f"{mode}: {t * 1e3:.3f} ms per batch -> {nat / t:.4g} atoms/s, {len(ss) / t:.4g} structures/s; "
               f"stage ms {res[mode]['stage_ms']}", file=sys.stderr, flush=True)
     def kind_errors(mode):

@@ -38,7 +38,7 @@ EXPORTS = ("hdnnp_create", "hdnnp_evaluate", "hdnnp_evaluate_batch", "hdnnp_eval
            "hdnnp_md_create", "hdnnp_md_step", "hdnnp_md_get", "hdnnp_md_stats", "hdnnp_md_counters", "hdnnp_md_destroy", "hdnnp_neighbor_count",
            "hdnnp_get_neighbors", "hdnnp_get_symfunctions", "hdnnp_get_model_info",
            "hdnnp_get_sf_table", "hdnnp_get_weights", "hdnnp_dEdLat_to_stress",
-           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_set_graphs", "hdnnp_graph_counts", "hdnnp_destroy",
+           "hdnnp_last_error", "hdnnp_last_timings", "hdnnp_set_stage_timings", "hdnnp_set_graphs", "hdnnp_graph_counts", "hdnnp_destroy",
            "hdnnp_set_tile", "hdnnp_get_tile", "hdnnp_autotune",
            "hdnnp_server_create", "hdnnp_server_submit", "hdnnp_server_wait", "hdnnp_server_cancel",
            "hdnnp_server_md_open",
@@ -120,6 +120,7 @@ def load_library():
         lib.hdnnp_last_error.argtypes = [vp]
         lib.hdnnp_last_error.restype = ctypes.c_char_p
         lib.hdnnp_last_timings.argtypes = [vp, _dp]
+        lib.hdnnp_set_stage_timings.argtypes = [vp, ctypes.c_int32]
         lib.hdnnp_set_graphs.argtypes = [vp, ctypes.c_int32]
         lib.hdnnp_graph_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int64)]
@@ -412,9 +413,16 @@ class Hdnnp:
         return G, D, Ei
 
     def timings(self):
+        """stage times (ms) of the last evaluation; zeros unless it ran with
+        stage timings on (set_stage_timings)"""
         t = np.zeros(6)
         self._check(self.lib.hdnnp_last_timings(self.ctx, _p(t, _dp)))
         return t
+
+    def set_stage_timings(self, on: bool):
+        """hdnnp_set_stage_timings: HIP events between the stages (off by
+        default; they cost a small evaluation ~10-20 us)"""
+        self._check(self.lib.hdnnp_set_stage_timings(self.ctx, 1 if on else 0))
 
 
 def parseRuNNerInput(path: str, precision="fp64") -> Hdnnp:

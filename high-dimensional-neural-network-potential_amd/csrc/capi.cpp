@@ -441,6 +441,12 @@ int hdnnp_last_timings(hdnnp_ctx *ctx, double *ms6)
     return HDNNP_OK;
 }
 
+int hdnnp_set_stage_timings(hdnnp_ctx *ctx, int32_t on)
+{
+    if (!ctx || (on != 0 && on != 1)) return HDNNP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->eng->set_stage_timings(on != 0); });
+}
+
 int hdnnp_set_graphs(hdnnp_ctx *ctx, int32_t on)
 {
     if (!ctx || (on != 0 && on != 1)) return HDNNP_ERR_ARG;

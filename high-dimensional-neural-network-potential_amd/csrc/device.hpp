@@ -191,7 +191,9 @@ void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_
 // one workgroup (cell_atoms, cell_start, cell_end, perm, elem_off,
 // elem_count; slot and tmp are nat-int scratch)
 constexpr int kSmallSortThreads = 1024, kSmallSortAtoms = 8192, kSmallSortCells = 12288;
-void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, hipStream_t s);
+// also clears clear[0..nclear) (the evaluation's flags and counters) first
+void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, int *clear, int nclear,
+                       hipStream_t s);
 void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
 // perm (atoms by element, index order within one), elem_off, elem_count;
 // chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk
@@ -234,9 +236,10 @@ template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, hipStream_t s);
 // per-structure sums of Ei and vir: chunks [c.x, c.y) of atoms never cross a
 // structure; structure s owns chunks [sc[s], sc[s+1]); partial: 10 per chunk;
-// flag_out (or null) receives a copy of w.flags[0..8)
+// flag_out (or null) receives a copy of w.flags[0..8); done: a counter that
+// is zero between launches (the one-launch form for few structures)
 void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   int *flag_out, hipStream_t s);
+                   int *flag_out, unsigned *done, hipStream_t s);
 
 // ---- kernels_md.hip: velocity Verlet and Verlet-skin bookkeeping ----
 void launch_md_drift(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,

@@ -194,6 +194,7 @@ Engine::Engine(std::vector<HostModel> members, int precision) : precision_(preci
         throw Error(1, "precision must be HDNNP_FP64, HDNNP_MIXED or HDNNP_MIXED_BF16");
     if ((int)model_.el.size() > kMaxEl) throw Error(6, "too many elements for this build");
     if (const char *v = std::getenv("HDNNP_EVAL_GRAPHS")) eval_graphs_on_ = std::strcmp(v, "1") == 0;
+    if (const char *v = std::getenv("HDNNP_STAGE_TIMINGS")) stage_timing_ = std::strcmp(v, "1") == 0;
     // committee members (globalHdnnps.f90:31-70) reuse the first potential's
     // symmetry functions and scaling and differ in their networks and atom
     // energies: every element of the first needs a network with as many inputs
@@ -518,6 +519,7 @@ void Engine::init_device()
 
 // misc buffer layout
 static constexpr int kIntElemCount = 0, kIntElemOff = 16, kIntFlags = 40;   // ints
+static constexpr int kIntReduceDone = 56;   // the reduction's count of finished blocks
 static constexpr size_t kFlagBytes = 8 * sizeof(int);                          // w.flags[0..8)
 // host-array evaluations of at most this many atoms stage through pinned memory
 static constexpr long kSmallHostEval = 4096;
@@ -924,19 +926,22 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
 
 void Engine::build_launch(hipStream_t s, bool timed)
 {
+    timed = timed && stage_timing_;
+    ev_recorded_ = timed;
     if (prep_.nat == 0) return;
     if (!prep_.valid) throw Error(5, "no prepared evaluation");
     Work w = prep_.w;
     const Geo g = prep_.g;
     if (timed) HIPCHK(hipEventRecord(ev_[0], s));
-    // flags and element counts start at zero (a fill kernel: no memset node
-    // when the launches are captured)
-    launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
+    // flags and element counts start at zero (cleared by a kernel: no memset
+    // node when the launches are captured)
     if (prep_.small_sort) {
-        // one workgroup does the set-up and sorts by cell and by element; the
-        // radix sort's input and output keys serve as scratch
-        launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, s);
+        // one workgroup clears them, does the set-up and sorts by cell and by
+        // element; the radix sort's input and output keys serve as scratch
+        launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, misc_.as<int>(),
+                          (int)(kDblOffset / sizeof(int)), s);
     } else {
+        launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
         launch_prep(dm_, g, w, iota_.as<int>(), s);
         size_t bt = cub_tmp_.bytes;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
@@ -977,6 +982,7 @@ std::vector<char> Engine::launch_key() const
 
 void Engine::finish_launch(double *force_dev, hipStream_t s, bool timed)
 {
+    timed = timed && stage_timing_;
     if (prep_.nat == 0) return;
     if (!prep_.valid) throw Error(5, "no prepared evaluation");
     const bool fp64 = precision_ == 0;
@@ -995,7 +1001,8 @@ void Engine::finish_launch(double *force_dev, hipStream_t s, bool timed)
         wk.Ei = w.Ei + (size_t)k * nat;
         wk.result = w.result + 10 * (size_t)k * nstruct;
         int *flag_out = k == 0 ? (int *)(w.result + 10 * (size_t)nm * nstruct) : nullptr;
-        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), flag_out, s);
+        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), flag_out,
+                      (unsigned *)(misc_.as<int>() + kIntReduceDone), s);
     }
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[6], s));
@@ -1041,7 +1048,7 @@ bool Engine::finish_collect(hipStream_t s, double *energy, double *dEdLat)
     last_total_ = -1;
     for (int k = 0; k < 6; ++k) {
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
+        if (ev_recorded_) HIPCHK(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
         ms_[k] = ms;
     }
     // committee mean and population standard deviation of the energies
@@ -1136,9 +1143,11 @@ void Engine::launch_evaluation(double *force_dev, hipStream_t s)
     }
     hit->used = ++eval_tick_;
     // the replay's whole device time is the first stage's
-    HIPCHK(hipEventRecord(ev_[0], s));
+    ev_recorded_ = stage_timing_;
+    if (ev_recorded_) HIPCHK(hipEventRecord(ev_[0], s));
     HIPCHK(hipGraphLaunch(hit->exec, s));
-    for (int k = 1; k < 7; ++k) HIPCHK(hipEventRecord(ev_[k], s));
+    if (ev_recorded_)
+        for (int k = 1; k < 7; ++k) HIPCHK(hipEventRecord(ev_[k], s));
     ++eval_replays_;
 }
 
@@ -1286,8 +1295,9 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
     std::vector<double> E(nstruct), F(3 * (size_t)std::max<long>(total, 1));
     const int cand[3] = {1, 2, 4};
     const int keep = tile_;
-    const bool graphs = eval_graphs_on_;
+    const bool graphs = eval_graphs_on_, timing = stage_timing_;
     eval_graphs_on_ = false;   // the stage timings of direct launches
+    stage_timing_ = true;
     int best = 1;
     double best_ms = 0;
     try {
@@ -1311,10 +1321,12 @@ int Engine::autotune(int nstruct, const int *nat, const int *Z, const double *po
     } catch (...) {
         tile_ = keep;
         eval_graphs_on_ = graphs;
+        stage_timing_ = timing;
         throw;
     }
     tile_ = best;
     eval_graphs_on_ = graphs;
+    stage_timing_ = timing;
     return best;
 }
 

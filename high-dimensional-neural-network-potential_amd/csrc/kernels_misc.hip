@@ -95,7 +95,16 @@ __device__ inline void tree10(double (*red)[256], double *a)
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chunks, double *partial)
+__device__ void reduce_final_wave(const Work &w, const int *sc, const double *partial, int *flag_out, int s,
+                                  int lane);
+
+// FUSE (few structures): the block that finishes last, by an atomic count of
+// finished blocks in `done`, runs stage 2 itself (the same per-wave code in
+// the same order, its four waves over the structures) and sets the count back
+// to zero, so that one launch does both stages
+template <bool FUSE>
+__global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chunks, double *partial, const int *sc,
+                                                        int nstruct, int *flag_out, unsigned *done)
 {
     __shared__ double red[10][256];
     double a[10];
@@ -109,15 +118,27 @@ __global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chun
     }
     tree10(red, a);
     if (threadIdx.x < 10) partial[10 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if constexpr (FUSE) {
+        __shared__ int last;
+        __threadfence();   // this block's partials reach L2 before it is counted
+        __syncthreads();
+        if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+        __syncthreads();
+        if (!last) return;
+        __threadfence();   // and the other blocks' are read from there
+        for (int st = (int)threadIdx.x / 64; st < nstruct; st += 4)
+            reduce_final_wave(w, sc, partial, flag_out, st, (int)threadIdx.x % 64);
+        if (threadIdx.x == 0) *done = 0;
+    }
 }
 
 // one wave per structure: lane l adds chunks l, l+64, ... of the structure,
 // then a fixed butterfly over the lanes
 // flag_out (block 0, when given) receives a copy of the 8 flag words, so that
 // the host reads results and flags back in one copy
-__global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, const double *partial, int *flag_out)
+__device__ void reduce_final_wave(const Work &w, const int *sc, const double *partial, int *flag_out, int s,
+                                  int lane)
 {
-    const int s = blockIdx.x, lane = threadIdx.x;
     double a[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) a[k] = 0;
@@ -136,11 +157,25 @@ __global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, cons
     if (flag_out && s == 0 && lane >= 16 && lane < 24) flag_out[lane - 16] = w.flags[lane - 16];
 }
 
-void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   int *flag_out, hipStream_t s)
+__global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, const double *partial, int *flag_out)
 {
+    reduce_final_wave(w, sc, partial, flag_out, (int)blockIdx.x, (int)threadIdx.x);
+}
+
+// up to this many structures, one launch does both stages (FUSE)
+constexpr int kFuseReduceMax = 16;
+
+void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
+                   int *flag_out, unsigned *done, hipStream_t s)
+{
+    if (nchunk > 0 && nstruct <= kFuseReduceMax) {
+        hipLaunchKernelGGL(k_reduce_partial<true>, dim3(nchunk), dim3(256), 0, s, w, chunks, partial, sc, nstruct,
+                           flag_out, done);
+        return;
+    }
     if (nchunk > 0)
-        hipLaunchKernelGGL(k_reduce_partial, dim3(nchunk), dim3(256), 0, s, w, chunks, partial);
+        hipLaunchKernelGGL(k_reduce_partial<false>, dim3(nchunk), dim3(256), 0, s, w, chunks, partial, sc, nstruct,
+                           flag_out, done);
     hipLaunchKernelGGL(k_reduce_final, dim3(nstruct), dim3(64), 0, s, w, sc, (const double *)partial, flag_out);
 }
 

@@ -217,12 +217,18 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s)
 // thread whatever the distribution.
 constexpr int kScanRankMax = 256;
 __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Geo g, Work w, int nel, int *slot,
-                                                                  int *tmp)
+                                                                  int *tmp, int *clear, int nclear)
 {
     __shared__ int start[kSmallSortCells + 1];
     __shared__ int part[kSmallSortThreads];
     __shared__ int fullest;
     const int t = threadIdx.x, nat = w.nat, ncell = g.ncell;
+    // the flags and counters of the evaluation start at zero (in place of a
+    // fill launch): cleared before any thread of this workgroup, the only
+    // one, sets a flag; the barrier's fence has the stores reach L2, where
+    // the flag atomics act
+    for (int x = t; x < nclear; x += kSmallSortThreads) clear[x] = 0;
+    __syncthreads();
     for (int x = t; x < max(nat, ncell); x += kSmallSortThreads) prep_one(m, g, w, nullptr, x);
     if (t == 0) fullest = 0;
     __syncthreads();
@@ -339,10 +345,11 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Ge
     }
 }
 
-void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, hipStream_t s)
+void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, int *clear, int nclear,
+                       hipStream_t s)
 {
-    if (w.nat > 0 || g.ncell > 0)
-        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, m, g, w, m.nel, slot, tmp);
+    hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kSmallSortThreads), 0, s, m, g, w, m.nel, slot, tmp, clear,
+                       nclear);
 }
 
 // Atoms ordered by element, in index order within an element (the tiles of

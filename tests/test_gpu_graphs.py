@@ -156,3 +156,27 @@ def test_small_evaluation_latency(pkg, wl):
         out[("perov6", on)] = float(np.median(t)) * 1e6
     print("median us per call:", {f"{k[0]} graphs={k[1]}": round(v, 1) for k, v in out.items()}, flush=True)
     assert out[("ma", True)] < 2000 and out[("perov6", True)] < 5000
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_stage_timings_opt_in(pkg, wl, graphs):
+    """hdnnp_set_stage_timings: off by default (timings all zero, results the
+    same bit for bit), on: positive stage times (a replay puts its whole time
+    in the first), off again: zeros"""
+    s0 = wl.perovskite(2, 2, 1, seed=3)
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, "mixed")
+    h.graphs = graphs
+    ref = h.energy_and_forces(s0)
+    assert np.array_equal(h.timings(), np.zeros(6))
+    h.set_stage_timings(True)
+    for k in range(4):                            # a capture and replays when graphs are on
+        _same(h.energy_and_forces(s0), ref)
+        t = h.timings()
+        print(f"graphs={graphs} call {k}: stage ms {np.round(t, 4).tolist()}", flush=True)
+        assert np.all(t >= 0) and t[0] > 0
+        if not h.graph_counts()[1]:
+            assert np.all(t[1:5] > 0)
+    assert (h.graph_counts()[1] > 0) == graphs
+    h.set_stage_timings(False)
+    _same(h.energy_and_forces(s0), ref)
+    assert np.array_equal(h.timings(), np.zeros(6))
