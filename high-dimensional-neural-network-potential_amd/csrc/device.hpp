@@ -182,10 +182,16 @@ void launch_bbox(const double *pos, const int *off, const int *which, int nwhich
 void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_rev(const Geo &g, Work &w, hipStream_t s);
 void launch_fill_int(int *p, int v, long n, hipStream_t s);
+void launch_zero2(int *a, long na, int *b, long nb, hipStream_t s);
 // per-atom set-up of a build: element ids and flags, structure ids of atoms
-// and cells, wrapped positions and cell keys, the identity order
-void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s);
-// cell_start / cell_end from the key-sorted atoms
+// and cells, wrapped positions and cell keys, the identity order (iota, or
+// null); slot (or null): each atom's arrival slot in its cell, counted in
+// cell_end, which must be zero
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, hipStream_t s);
+// cell_start / cell_end / cell_atoms from the counts and slots of launch_prep
+// (grids of at most kCountSortCells cells; tmp: nat ints)
+constexpr int kCountSortCells = 1 << 18;
+void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, hipStream_t s);
 // set-up (as launch_prep, without iota) and cell and element sorts of
 // evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in
 // one workgroup (cell_atoms, cell_start, cell_end, perm, elem_off,
@@ -194,6 +200,7 @@ constexpr int kSmallSortThreads = 1024, kSmallSortAtoms = 8192, kSmallSortCells 
 // also clears clear[0..nclear) (the evaluation's flags and counters) first
 void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int *tmp, int *clear, int nclear,
                        hipStream_t s);
+// cell_start / cell_end from the key-sorted atoms
 void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
 // perm (atoms by element, index order within one), elem_off, elem_count;
 // chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk

@@ -877,11 +877,14 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
     const bool small_sort = nat <= kSmallSortAtoms && ncell <= kSmallSortCells;
+    const bool count_sort = !small_sort && ncell <= kCountSortCells;
     if (!small_sort) {   // the radix sort's temporaries and the element sort's chunks
-        size_t b1 = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
-                                                  w.cell_atoms, nat, 0, bits, s));
-        cub_tmp_.ensure(b1);
+        if (!count_sort) {
+            size_t b1 = 0;
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
+                                                      w.cell_atoms, nat, 0, bits, s));
+            cub_tmp_.ensure(b1);
+        }
         elem_chunks_.ensure(4 * (size_t)kMaxEl * (size_t)std::max(1, elem_sort_chunks(nat)));
     }
 
@@ -917,6 +920,7 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     prep_.ncell = ncell;
     prep_.bits = bits;
     prep_.small_sort = small_sort;
+    prep_.count_sort = count_sort;
     prep_.o_res = o_res;
     prep_.o_flags = o_flags;
     prep_.res_b = res_b;
@@ -940,9 +944,17 @@ void Engine::build_launch(hipStream_t s, bool timed)
         // element; the radix sort's input and output keys serve as scratch
         launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, misc_.as<int>(),
                           (int)(kDblOffset / sizeof(int)), s);
+    } else if (prep_.count_sort) {
+        // the flags and the cells' counts start at zero; the identity order's
+        // buffer holds the arrival slots, the sorted keys' buffer the
+        // placement by slot
+        launch_zero2(misc_.as<int>(), (long)(kDblOffset / sizeof(int)), w.cell_end, prep_.ncell, s);
+        launch_prep(dm_, g, w, nullptr, iota_.as<int>(), s);
+        launch_cell_count_sort(w, prep_.ncell, iota_.as<int>(), w.cellkey_sorted, s);
+        launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
     } else {
         launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
-        launch_prep(dm_, g, w, iota_.as<int>(), s);
+        launch_prep(dm_, g, w, iota_.as<int>(), nullptr, s);
         size_t bt = cub_tmp_.bytes;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                                   w.cell_atoms, prep_.nat, 0, prep_.bits, s));

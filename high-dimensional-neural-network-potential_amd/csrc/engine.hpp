@@ -280,6 +280,7 @@ class Engine {
         int nat = 0, nchunk = 0, lds_lim = 0, ncell = 0, bits = 1;
         size_t o_res = 0, o_flags = 0, res_b = 0;
         bool small_sort = false;
+        bool count_sort = false;   // cells by counting sort (else radix)
         bool valid = false;
     } prep_;
     unsigned long long build_gen_ = 0;

@@ -54,6 +54,20 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
+// a[0..na) = 0 and b[0..nb) = 0 in one launch
+__global__ void k_zero2(int *a, long na, int *b, long nb)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0;
+    if (i < nb) b[i] = 0;
+}
+
+void launch_zero2(int *a, long na, int *b, long nb, hipStream_t s)
+{
+    const long n = na > nb ? na : nb;
+    if (n > 0) hipLaunchKernelGGL(k_zero2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, na, b, nb);
+}
+
 // Bounding boxes of the raw positions of nonperiodic structures (artificial
 // lattice, neighborLists.f90:131-138): block b covers structure which[b],
 // atoms [off[s], off[s+1]); out[6b..6b+5] = min x, y, z, max x, y, z.
@@ -125,11 +139,12 @@ __device__ inline int segment_of(const StructGeo *sg, int nstruct, int x, bool c
 //   shifted by the artificial origin and wrapped once in the artificial box.
 //   Cell index int(s*ngrid) (neighborLists.f90:161-166), clamped into the
 //   grid, offset by the structure's first cell.
-// item x of the set-up (cell x, atom x); iota may be null
-__device__ __forceinline__ void prep_one(const DevModel &m, const Geo &g, const Work &w, int *iota, int x)
+// item x of the set-up (cell x, atom x); iota may be null. Returns atom x's
+// cell key (-1 past the atoms).
+__device__ __forceinline__ int prep_one(const DevModel &m, const Geo &g, const Work &w, int *iota, int x)
 {
     if (x < g.ncell) w.csid[x] = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, true);
-    if (x >= w.nat) return;
+    if (x >= w.nat) return -1;
     const int z = w.Z[x];
     const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
     if (e == -1) atomicMax(&w.flags[0], z > 0 ? z : 1);
@@ -156,18 +171,109 @@ __device__ __forceinline__ void prep_one(const DevModel &m, const Geo &g, const 
     c0 = min(max(c0, 0), G.ng[0] - 1);
     c1 = min(max(c1, 0), G.ng[1] - 1);
     c2 = min(max(c2, 0), G.ng[2] - 1);
-    w.cellkey[x] = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
+    const int key = G.cell_off + (c0 * G.ng[1] + c1) * G.ng[2] + c2;
+    w.cellkey[x] = key;
+    return key;
 }
 
-__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota)
+// slot (or null): atom x's arrival slot in its cell, counted in cell_end
+// (zero before the launch)
+__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota, int *slot)
 {
-    prep_one(m, g, w, iota, blockIdx.x * blockDim.x + threadIdx.x);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int key = prep_one(m, g, w, iota, x);
+    if (slot && key >= 0) slot[x] = atomicAdd(&w.cell_end[key], 1);
 }
 
-void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, hipStream_t s)
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, hipStream_t s)
 {
     const int n = w.nat > g.ncell ? w.nat : g.ncell;
-    if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, m, g, w, iota);
+    if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, m, g, w, iota, slot);
+}
+
+// Counting sort by cell, for grids above the one-workgroup sort's size: after
+// k_prep has counted each cell's atoms in cell_end and kept each atom's
+// arrival slot, k_cell_scan (one workgroup) turns the counts into each
+// cell's range, k_cell_place puts atom x at its arrival slot of its cell in
+// tmp, and k_cell_rank moves it to its rank in index order (the number of the
+// cell's atoms with a smaller index), which is the stable radix sort's order.
+// Three launches where the radix sort and the range search take eight.
+// Cells in rows of 1024, thread t taking column t (every access coalesced),
+// kScanRows rows per round, their loads all out at once: each row's wave
+// scans, then one wave scans the (row, wave) totals in cell order.
+constexpr int kScanRows = 32;
+__global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell)
+{
+    __shared__ int pre[kScanRows * 16];
+    __shared__ int round_total;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int carry = 0;   // the cells of the rounds before
+    for (int r0 = 0; r0 * 1024 < ncell; r0 += kScanRows) {
+        int v[kScanRows], inc[kScanRows];
+#pragma unroll
+        for (int u = 0; u < kScanRows; ++u) {
+            const int c = (r0 + u) * 1024 + t;
+            v[u] = c < ncell ? w.cell_end[c] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kScanRows; ++u) {
+            inc[u] = wave_incl_scan_int(v[u], lane);
+            if (lane == 63) pre[u * 16 + wv] = inc[u];
+        }
+        __syncthreads();
+        if (wv == 0) {   // exclusive scan of the kScanRows * 16 totals, 8 per lane
+            int a[8], sum = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                a[i] = pre[lane * 8 + i];
+                sum += a[i];
+            }
+            const int ps = wave_incl_scan_int(sum, lane);
+            int run = ps - sum;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                pre[lane * 8 + i] = run;
+                run += a[i];
+            }
+            if (lane == 63) round_total = ps;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kScanRows; ++u) {
+            const int c = (r0 + u) * 1024 + t;
+            const int st = carry + pre[u * 16 + wv] + inc[u] - v[u];
+            if (c < ncell) {
+                w.cell_start[c] = st;
+                w.cell_end[c] = st + v[u];
+            }
+        }
+        carry += round_total;
+        __syncthreads();   // pre and round_total are read before the next round writes them
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_place(Work w, const int *slot, int *tmp)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < w.nat) tmp[w.cell_start[w.cellkey[x]] + slot[x]] = x;
+}
+
+__global__ __launch_bounds__(256) void k_cell_rank(Work w, const int *tmp)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w.nat) return;
+    const int c = w.cellkey[x], s0 = w.cell_start[c], s1 = w.cell_end[c];
+    int r = 0;
+    for (int q = s0; q < s1; ++q) r += tmp[q] < x;
+    w.cell_atoms[s0 + r] = x;
+}
+
+void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, hipStream_t s)
+{
+    if (w.nat <= 0) return;
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, w, ncell);
+    hipLaunchKernelGGL(k_cell_place, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, slot, tmp);
+    hipLaunchKernelGGL(k_cell_rank, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, (const int *)tmp);
 }
 
 // first position q in [0, n) with keys[q] >= v
