@@ -189,8 +189,10 @@ void launch_zero2(int *a, long na, int *b, long nb, hipStream_t s);
 // cell_end, which must be zero
 void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, hipStream_t s);
 // cell_start / cell_end / cell_atoms from the counts and slots of launch_prep
-// (grids of at most kCountSortCells cells; tmp: nat ints)
-constexpr int kCountSortCells = 1 << 18;
+// (grids of at most kCountSortCells cells, one round of its one-workgroup
+// scan; larger grids keep the radix sort, which spreads over the chip;
+// tmp: nat ints)
+constexpr int kCountSortCells = 1 << 15;
 void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, hipStream_t s);
 // set-up (as launch_prep, without iota) and cell and element sorts of
 // evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in
