@@ -182,18 +182,20 @@ void launch_bbox(const double *pos, const int *off, const int *which, int nwhich
 void launch_nl_build(const Geo &g, Work &w, int ncell, hipStream_t s);
 void launch_nl_rev(const Geo &g, Work &w, hipStream_t s);
 void launch_fill_int(int *p, int v, long n, hipStream_t s);
-void launch_zero2(int *a, long na, int *b, long nb, hipStream_t s);
+void launch_zero3(int *a, long na, int *b, long nb, int *c, long nc, hipStream_t s);
 // per-atom set-up of a build: element ids and flags, structure ids of atoms
 // and cells, wrapped positions and cell keys, the identity order (iota, or
 // null); slot (or null): each atom's arrival slot in its cell, counted in
-// cell_end, which must be zero
-void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, hipStream_t s);
+// cell_end; chunk_cnt (or null): the element sort's per-chunk counts (both
+// must be zero)
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, int *chunk_cnt, hipStream_t s);
 // cell_start / cell_end / cell_atoms from the counts and slots of launch_prep
 // (grids of at most kCountSortCells cells, one round of its one-workgroup
 // scan; larger grids keep the radix sort, which spreads over the chip;
-// tmp: nat ints)
+// tmp: nat ints); its workgroup also turns chunk_cnt into the element sort's
+// chunk offsets, elem_off and elem_count
 constexpr int kCountSortCells = 1 << 15;
-void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, hipStream_t s);
+void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s);
 // set-up (as launch_prep, without iota) and cell and element sorts of
 // evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in
 // one workgroup (cell_atoms, cell_start, cell_end, perm, elem_off,
@@ -207,6 +209,8 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
 // perm (atoms by element, index order within one), elem_off, elem_count;
 // chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk
 int elem_sort_chunks(int nat);
+// the element sort's placement alone (counts and offsets made elsewhere)
+void launch_elem_place(Work &w, int nel, int *chunk_off, hipStream_t s);
 void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s);
 
 // ---- launchers (kernels_sf.hip); R = double or float ----

@@ -945,16 +945,18 @@ void Engine::build_launch(hipStream_t s, bool timed)
         launch_sort_small(dm_, g, w, iota_.as<int>(), w.cellkey_sorted, misc_.as<int>(),
                           (int)(kDblOffset / sizeof(int)), s);
     } else if (prep_.count_sort) {
-        // the flags and the cells' counts start at zero; the identity order's
-        // buffer holds the arrival slots, the sorted keys' buffer the
-        // placement by slot
-        launch_zero2(misc_.as<int>(), (long)(kDblOffset / sizeof(int)), w.cell_end, prep_.ncell, s);
-        launch_prep(dm_, g, w, nullptr, iota_.as<int>(), s);
-        launch_cell_count_sort(w, prep_.ncell, iota_.as<int>(), w.cellkey_sorted, s);
-        launch_elem_sort(w, dm_.nel, elem_chunks_.as<int>(), s);
+        // the flags, the cells' counts and the element sort's chunk counts
+        // start at zero; the identity order's buffer holds the arrival slots,
+        // the sorted keys' buffer the placement by slot
+        int *ecc = elem_chunks_.as<int>();
+        launch_zero3(misc_.as<int>(), (long)(kDblOffset / sizeof(int)), w.cell_end, prep_.ncell, ecc,
+                     (long)kMaxEl * elem_sort_chunks(prep_.nat), s);
+        launch_prep(dm_, g, w, nullptr, iota_.as<int>(), ecc, s);
+        launch_cell_count_sort(w, prep_.ncell, iota_.as<int>(), w.cellkey_sorted, ecc, s);
+        launch_elem_place(w, dm_.nel, ecc, s);
     } else {
         launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
-        launch_prep(dm_, g, w, iota_.as<int>(), nullptr, s);
+        launch_prep(dm_, g, w, iota_.as<int>(), nullptr, nullptr, s);
         size_t bt = cub_tmp_.bytes;
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp_.p, bt, w.cellkey, w.cellkey_sorted, iota_.as<int>(),
                                                   w.cell_atoms, prep_.nat, 0, prep_.bits, s));

@@ -54,18 +54,20 @@ void launch_fill_int(int *p, int v, long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
-// a[0..na) = 0 and b[0..nb) = 0 in one launch
-__global__ void k_zero2(int *a, long na, int *b, long nb)
+// a[0..na) = 0, b[0..nb) = 0 and c[0..nc) = 0 in one launch
+__global__ void k_zero3(int *a, long na, int *b, long nb, int *c, long nc)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < na) a[i] = 0;
     if (i < nb) b[i] = 0;
+    if (i < nc) c[i] = 0;
 }
 
-void launch_zero2(int *a, long na, int *b, long nb, hipStream_t s)
+void launch_zero3(int *a, long na, int *b, long nb, int *c, long nc, hipStream_t s)
 {
-    const long n = na > nb ? na : nb;
-    if (n > 0) hipLaunchKernelGGL(k_zero2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, na, b, nb);
+    const long n = na > nb ? (na > nc ? na : nc) : (nb > nc ? nb : nc);
+    if (n > 0)
+        hipLaunchKernelGGL(k_zero3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, na, b, nb, c, nc);
 }
 
 // Bounding boxes of the raw positions of nonperiodic structures (artificial
@@ -140,8 +142,8 @@ __device__ inline int segment_of(const StructGeo *sg, int nstruct, int x, bool c
 //   Cell index int(s*ngrid) (neighborLists.f90:161-166), clamped into the
 //   grid, offset by the structure's first cell.
 // item x of the set-up (cell x, atom x); iota may be null. Returns atom x's
-// cell key (-1 past the atoms).
-__device__ __forceinline__ int prep_one(const DevModel &m, const Geo &g, const Work &w, int *iota, int x)
+// cell key (-1 past the atoms) and sets el to its element rank.
+__device__ __forceinline__ int prep_one(const DevModel &m, const Geo &g, const Work &w, int *iota, int x, int &el)
 {
     if (x < g.ncell) w.csid[x] = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, true);
     if (x >= w.nat) return -1;
@@ -149,7 +151,8 @@ __device__ __forceinline__ int prep_one(const DevModel &m, const Geo &g, const W
     const int e = (z >= 0 && z < 128) ? m.zlut[z] : -1;
     if (e == -1) atomicMax(&w.flags[0], z > 0 ? z : 1);
     if (e <= -2) atomicMax(&w.flags[3], z);
-    w.eidx[x] = e < 0 ? m.fallback_el : e;
+    el = e < 0 ? m.fallback_el : e;
+    w.eidx[x] = el;
     const int sid = g.nstruct == 1 ? 0 : segment_of(g.sg, g.nstruct, x, false);
     w.sid[x] = sid;
     if (iota) iota[x] = x;
@@ -176,19 +179,33 @@ __device__ __forceinline__ int prep_one(const DevModel &m, const Geo &g, const W
     return key;
 }
 
-// slot (or null): atom x's arrival slot in its cell, counted in cell_end
-// (zero before the launch)
-__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota, int *slot)
+// slot (or null): atom x's arrival slot in its cell, counted in cell_end;
+// chunk_cnt (or null): the atoms of each element in each kSortChunk-atom
+// chunk, the element sort's counts (both zero before the launch)
+constexpr int kSortChunk = 1024;   // atoms per chunk of the element sort: 4 rounds of 256 threads
+__global__ __launch_bounds__(256) void k_prep(DevModel m, Geo g, Work w, int *iota, int *slot, int *chunk_cnt)
 {
+    __shared__ int ecnt[kMaxEl];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int key = prep_one(m, g, w, iota, x);
+    if (chunk_cnt && threadIdx.x < kMaxEl) ecnt[threadIdx.x] = 0;
+    int el = -1;
+    const int key = prep_one(m, g, w, iota, x, el);
     if (slot && key >= 0) slot[x] = atomicAdd(&w.cell_end[key], 1);
+    if (chunk_cnt) {   // this block's 256 atoms lie in one chunk
+        __syncthreads();
+        if (key >= 0) atomicAdd(&ecnt[el], 1);
+        __syncthreads();
+        const int c = (int)(blockIdx.x * blockDim.x) / kSortChunk;
+        if (threadIdx.x < kMaxEl && ecnt[threadIdx.x])
+            atomicAdd(&chunk_cnt[c * kMaxEl + threadIdx.x], ecnt[threadIdx.x]);
+    }
 }
 
-void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, hipStream_t s)
+void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot, int *chunk_cnt, hipStream_t s)
 {
     const int n = w.nat > g.ncell ? w.nat : g.ncell;
-    if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, m, g, w, iota, slot);
+    if (n > 0)
+        hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, m, g, w, iota, slot, chunk_cnt);
 }
 
 // Counting sort by cell, for grids above the one-workgroup sort's size: after
@@ -202,7 +219,9 @@ void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot,
 // kScanRows rows per round, their loads all out at once: each row's wave
 // scans, then one wave scans the (row, wave) totals in cell order.
 constexpr int kScanRows = 32;
-__global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell)
+__device__ void elem_offsets_block(Work &w, int *chunk_cnt, int nchunk);
+// then, as k_elem_offsets, the element sort's chunk offsets from chunk_cnt
+__global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell, int *chunk_cnt, int nchunk)
 {
     __shared__ int pre[kScanRows * 16];
     __shared__ int round_total;
@@ -250,6 +269,7 @@ __global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell)
         carry += round_total;
         __syncthreads();   // pre and round_total are read before the next round writes them
     }
+    elem_offsets_block(w, chunk_cnt, nchunk);
 }
 
 __global__ __launch_bounds__(256) void k_cell_place(Work w, const int *slot, int *tmp)
@@ -268,10 +288,10 @@ __global__ __launch_bounds__(256) void k_cell_rank(Work w, const int *tmp)
     w.cell_atoms[s0 + r] = x;
 }
 
-void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, hipStream_t s)
+void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s)
 {
     if (w.nat <= 0) return;
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, w, ncell);
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, w, ncell, chunk_cnt, elem_sort_chunks(w.nat));
     hipLaunchKernelGGL(k_cell_place, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, slot, tmp);
     hipLaunchKernelGGL(k_cell_rank, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, (const int *)tmp);
 }
@@ -335,7 +355,10 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Ge
     // the flag atomics act
     for (int x = t; x < nclear; x += kSmallSortThreads) clear[x] = 0;
     __syncthreads();
-    for (int x = t; x < max(nat, ncell); x += kSmallSortThreads) prep_one(m, g, w, nullptr, x);
+    for (int x = t; x < max(nat, ncell); x += kSmallSortThreads) {
+        int el;
+        prep_one(m, g, w, nullptr, x, el);
+    }
     if (t == 0) fullest = 0;
     __syncthreads();
     for (int c = t; c <= ncell; c += kSmallSortThreads) start[c] = 0;
@@ -461,8 +484,6 @@ void launch_sort_small(const DevModel &m, const Geo &g, Work &w, int *slot, int 
 // Atoms ordered by element, in index order within an element (the tiles of
 // the MLP), by a stable counting sort over chunks of kSortChunk atoms:
 // per-chunk counts, their offsets, then each chunk places its atoms.
-constexpr int kSortChunk = 1024;   // atoms per chunk: 4 rounds of 256 threads
-
 __global__ __launch_bounds__(256) void k_elem_count(Work w, int *chunk_cnt)
 {
     __shared__ int cnt[kMaxEl];
@@ -480,7 +501,8 @@ __global__ __launch_bounds__(256) void k_elem_count(Work w, int *chunk_cnt)
 // one block of kMaxEl wavefronts: wave e scans element e's chunk counts into
 // chunk offsets; then elem_count[e], and elem_off[e] = first position of
 // element e (elem_off[kMaxEl] = nat)
-__global__ __launch_bounds__(64 * kMaxEl) void k_elem_offsets(Work w, int *chunk_cnt, int nchunk)
+static_assert(64 * kMaxEl == 1024, "the element offsets run in the cell scan's workgroup");
+__device__ void elem_offsets_block(Work &w, int *chunk_cnt, int nchunk)
 {
     __shared__ int tot[kMaxEl];
     const int e = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -500,6 +522,11 @@ __global__ __launch_bounds__(64 * kMaxEl) void k_elem_offsets(Work w, int *chunk
         w.elem_off[threadIdx.x] = off;
         if (threadIdx.x < kMaxEl) w.elem_count[threadIdx.x] = tot[threadIdx.x];
     }
+}
+
+__global__ __launch_bounds__(64 * kMaxEl) void k_elem_offsets(Work w, int *chunk_cnt, int nchunk)
+{
+    elem_offsets_block(w, chunk_cnt, nchunk);
 }
 
 __global__ __launch_bounds__(256) void k_elem_place(Work w, const int *chunk_off, int nel)
@@ -540,6 +567,12 @@ void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s)
     if (nchunk > 0) hipLaunchKernelGGL(k_elem_count, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt);
     hipLaunchKernelGGL(k_elem_offsets, dim3(1), dim3(64 * kMaxEl), 0, s, w, chunk_cnt, nchunk);
     if (nchunk > 0) hipLaunchKernelGGL(k_elem_place, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt, nel);
+}
+
+void launch_elem_place(Work &w, int nel, int *chunk_off, hipStream_t s)
+{
+    const int nchunk = elem_sort_chunks(w.nat);
+    if (nchunk > 0) hipLaunchKernelGGL(k_elem_place, dim3(nchunk), dim3(256), 0, s, w, chunk_off, nel);
 }
 
 // One stencil cell of atom i: wrapped cell id (within the structure's grid)
