@@ -373,17 +373,16 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevModel m, Ge
         sum += start[c];
         most = max(most, start[c]);
     }
-    part[t] = sum;
     atomicMax(&fullest, most);
+    // the sums of the threads before: wave scans, then the totals of the
+    // waves before (one barrier)
+    __shared__ int wtot[kSmallSortThreads / 64];
+    const int incl = wave_incl_scan_int(sum, t & 63);
+    if ((t & 63) == 63) wtot[t >> 6] = incl;
     __syncthreads();
-    for (int o = 1; o < kSmallSortThreads; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
+    int run = incl - sum;
+    for (int k = 0; k < (t >> 6); ++k) run += wtot[k];
     const bool by_scan = fullest <= kScanRankMax;   // workgroup-uniform
-    int run = part[t] - sum;
     for (int c = c0; c < c1; ++c) {
         const int n = start[c];
         start[c] = run;
