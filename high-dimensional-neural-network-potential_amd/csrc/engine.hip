@@ -877,7 +877,9 @@ void Engine::build(const std::vector<StructIn> &st, const int *Z_dev, const doub
     w.cell_start = cell_start_.as<int>();
     w.cell_end = cell_end_.as<int>();
     const bool small_sort = nat <= kSmallSortAtoms && ncell <= kSmallSortCells;
-    const bool count_sort = !small_sort && ncell <= kCountSortCells;
+    // the counting sort ranks an atom by a scan of its cell (k^2 loads for a
+    // cell of k atoms): dense grids keep the radix sort
+    const bool count_sort = !small_sort && ncell <= kCountSortCells && (long)nat <= 64L * ncell;
     if (!small_sort) {   // the radix sort's temporaries and the element sort's chunks
         if (!count_sort) {
             size_t b1 = 0;
