@@ -100,3 +100,23 @@ def test_mixed_batch_force_error_bound(pkg, wl):
         print(f"periodic {periodic}: {len(ks)} structures, F rmse {rmse:.3e}, max dE/nat {de:.3e}, "
               f"stress diff / bound {ds:.3f}", flush=True)
         assert rmse <= 1e-5 and de <= 1e-5 and ds <= 1.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cell_counting_sort_equals_radix_sort(pkg, wl, precision):
+    """A 12,288-atom perovskite cell alone is sorted into grid cells by the
+    counting sort (above the one-workgroup sort's 8,192 atoms, under 32,768
+    grid cells); next to a sparse periodic box of 35^3 = 42,875 grid cells the
+    batch's grid exceeds 32,768 cells and takes the radix sort. Both give the
+    atoms of a grid cell in index order, so the cell's results are the same
+    bit for bit."""
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    big = wl.perovskite(16, 16, 4, seed=21)
+    sparse = wl.Structure(np.array([[10.0, 10.0, 10.0], [30.0, 10.0, 10.0]]), np.array([82, 53], np.int32),
+                          np.eye(3) * 250.0, True)
+    E1, F1, L1 = h.energy_and_forces(big)
+    E, F, L = h.energy_and_forces_batch([big, sparse])
+    print(f"{precision}: E alone {E1!r}, in the batch {E[0]!r}", flush=True)
+    assert E[0] == E1 and np.array_equal(F[0], F1) and np.array_equal(L[0], L1)
+    E2, F2, L2 = h.energy_and_forces(sparse)
+    assert E[1] == E2 and np.array_equal(F[1], F2) and np.array_equal(L[1], L2)
