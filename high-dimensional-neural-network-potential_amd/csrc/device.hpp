@@ -194,7 +194,7 @@ void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot,
 // scan; larger grids keep the radix sort, which spreads over the chip;
 // tmp: nat ints); its workgroup also turns chunk_cnt into the element sort's
 // chunk offsets, elem_off and elem_count
-constexpr int kCountSortCells = 1 << 15;
+constexpr int kCountSortCells = 24 * 1024;
 void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s);
 // set-up (as launch_prep, without iota) and cell and element sorts of
 // evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in

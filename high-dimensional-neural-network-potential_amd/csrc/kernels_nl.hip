@@ -216,9 +216,11 @@ void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot,
 // cell's atoms with a smaller index), which is the stable radix sort's order.
 // Three launches where the radix sort and the range search take eight.
 // Cells in rows of 1024, thread t taking column t (every access coalesced),
-// kScanRows rows per round, their loads all out at once: each row's wave
-// scans, then one wave scans the (row, wave) totals in cell order.
-constexpr int kScanRows = 32;
+// kScanRows rows per round (kCountSortCells cells: one round), their loads
+// all out at once: each row's wave scans, then one wave scans the
+// (row, wave) totals in cell order.
+constexpr int kScanRows = 24;
+static_assert(kCountSortCells == kScanRows * 1024, "the counting sort's grids take one round of the scan");
 __device__ void elem_offsets_block(Work &w, int *chunk_cnt, int nchunk);
 // then, as k_elem_offsets, the element sort's chunk offsets from chunk_cnt
 __global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell, int *chunk_cnt, int nchunk)
@@ -240,18 +242,20 @@ __global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell, int *chun
             if (lane == 63) pre[u * 16 + wv] = inc[u];
         }
         __syncthreads();
-        if (wv == 0) {   // exclusive scan of the kScanRows * 16 totals, 8 per lane
-            int a[8], sum = 0;
+        if (wv == 0) {   // exclusive scan of the kScanRows * 16 totals, kPerLane per lane
+            constexpr int kPerLane = kScanRows * 16 / 64;
+            static_assert(kScanRows % 4 == 0, "whole totals per lane");
+            int a[kPerLane], sum = 0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                a[i] = pre[lane * 8 + i];
+            for (int i = 0; i < kPerLane; ++i) {
+                a[i] = pre[lane * kPerLane + i];
                 sum += a[i];
             }
             const int ps = wave_incl_scan_int(sum, lane);
             int run = ps - sum;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                pre[lane * 8 + i] = run;
+            for (int i = 0; i < kPerLane; ++i) {
+                pre[lane * kPerLane + i] = run;
                 run += a[i];
             }
             if (lane == 63) round_total = ps;

@@ -105,9 +105,9 @@ def test_mixed_batch_force_error_bound(pkg, wl):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_cell_counting_sort_equals_radix_sort(pkg, wl, precision):
     """A 12,288-atom perovskite cell alone is sorted into grid cells by the
-    counting sort (above the one-workgroup sort's 8,192 atoms, under 32,768
+    counting sort (above the one-workgroup sort's 8,192 atoms, under 24,576
     grid cells); next to a sparse periodic box of 35^3 = 42,875 grid cells the
-    batch's grid exceeds 32,768 cells and takes the radix sort. Both give the
+    batch's grid exceeds 24,576 cells and takes the radix sort. Both give the
     atoms of a grid cell in index order, so the cell's results are the same
     bit for bit."""
     h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
