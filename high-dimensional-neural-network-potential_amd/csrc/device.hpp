@@ -195,7 +195,8 @@ void launch_prep(const DevModel &m, const Geo &g, Work &w, int *iota, int *slot,
 // tmp: nat ints); its workgroup also turns chunk_cnt into the element sort's
 // chunk offsets, elem_off and elem_count
 constexpr int kCountSortCells = 24 * 1024;
-void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s);
+// (and, in the placement's launch, the element sort's placement by nel elements)
+void launch_cell_count_sort(Work &w, int ncell, int nel, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s);
 // set-up (as launch_prep, without iota) and cell and element sorts of
 // evaluations of at most kSmallSortAtoms atoms in kSmallSortCells cells, in
 // one workgroup (cell_atoms, cell_start, cell_end, perm, elem_off,
@@ -209,8 +210,6 @@ void launch_cell_bounds(Work &w, int ncell, hipStream_t s);
 // perm (atoms by element, index order within one), elem_off, elem_count;
 // chunk_cnt holds kMaxEl ints per elem_sort_chunks(nat) chunk
 int elem_sort_chunks(int nat);
-// the element sort's placement alone (counts and offsets made elsewhere)
-void launch_elem_place(Work &w, int nel, int *chunk_off, hipStream_t s);
 void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s);
 
 // ---- launchers (kernels_sf.hip); R = double or float ----

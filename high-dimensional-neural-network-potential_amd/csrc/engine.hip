@@ -954,8 +954,7 @@ void Engine::build_launch(hipStream_t s, bool timed)
         launch_zero3(misc_.as<int>(), (long)(kDblOffset / sizeof(int)), w.cell_end, prep_.ncell, ecc,
                      (long)kMaxEl * elem_sort_chunks(prep_.nat), s);
         launch_prep(dm_, g, w, nullptr, iota_.as<int>(), ecc, s);
-        launch_cell_count_sort(w, prep_.ncell, iota_.as<int>(), w.cellkey_sorted, ecc, s);
-        launch_elem_place(w, dm_.nel, ecc, s);
+        launch_cell_count_sort(w, prep_.ncell, dm_.nel, iota_.as<int>(), w.cellkey_sorted, ecc, s);
     } else {
         launch_fill_int(misc_.as<int>(), 0, (long)(kDblOffset / sizeof(int)), s);
         launch_prep(dm_, g, w, iota_.as<int>(), nullptr, nullptr, s);

@@ -276,8 +276,18 @@ __global__ __launch_bounds__(1024) void k_cell_scan(Work w, int ncell, int *chun
     elem_offsets_block(w, chunk_cnt, nchunk);
 }
 
-__global__ __launch_bounds__(256) void k_cell_place(Work w, const int *slot, int *tmp)
+__device__ void elem_place_block(Work &w, const int *chunk_off, int nel, int chunk);
+
+// blocks [0, nplace): atom x to its arrival slot of its cell in tmp; the
+// blocks after: the element sort's placement, chunk by chunk (the two are
+// independent, so they share a launch)
+__global__ __launch_bounds__(256) void k_cell_place(Work w, const int *slot, int *tmp, const int *chunk_off, int nel,
+                                                    int nplace)
 {
+    if ((int)blockIdx.x >= nplace) {   // block-uniform
+        elem_place_block(w, chunk_off, nel, (int)blockIdx.x - nplace);
+        return;
+    }
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x < w.nat) tmp[w.cell_start[w.cellkey[x]] + slot[x]] = x;
 }
@@ -292,11 +302,13 @@ __global__ __launch_bounds__(256) void k_cell_rank(Work w, const int *tmp)
     w.cell_atoms[s0 + r] = x;
 }
 
-void launch_cell_count_sort(Work &w, int ncell, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s)
+void launch_cell_count_sort(Work &w, int ncell, int nel, const int *slot, int *tmp, int *chunk_cnt, hipStream_t s)
 {
     if (w.nat <= 0) return;
     hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, w, ncell, chunk_cnt, elem_sort_chunks(w.nat));
-    hipLaunchKernelGGL(k_cell_place, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, slot, tmp);
+    const int nplace = (w.nat + 255) / 256;
+    hipLaunchKernelGGL(k_cell_place, dim3(nplace + elem_sort_chunks(w.nat)), dim3(256), 0, s, w, slot, tmp,
+                       (const int *)chunk_cnt, nel, nplace);
     hipLaunchKernelGGL(k_cell_rank, dim3((w.nat + 255) / 256), dim3(256), 0, s, w, (const int *)tmp);
 }
 
@@ -532,14 +544,14 @@ __global__ __launch_bounds__(64 * kMaxEl) void k_elem_offsets(Work w, int *chunk
     elem_offsets_block(w, chunk_cnt, nchunk);
 }
 
-__global__ __launch_bounds__(256) void k_elem_place(Work w, const int *chunk_off, int nel)
+__device__ void elem_place_block(Work &w, const int *chunk_off, int nel, int chunk)
 {
     __shared__ int wcnt[4][kMaxEl];
     __shared__ int base[kMaxEl];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    if (threadIdx.x < kMaxEl) base[threadIdx.x] = w.elem_off[threadIdx.x] + chunk_off[blockIdx.x * kMaxEl + threadIdx.x];
+    if (threadIdx.x < kMaxEl) base[threadIdx.x] = w.elem_off[threadIdx.x] + chunk_off[chunk * kMaxEl + threadIdx.x];
     const unsigned long long below = (1ull << lane) - 1;
-    const int a0 = blockIdx.x * kSortChunk;
+    const int a0 = chunk * kSortChunk;
     for (int r = 0; r < kSortChunk; r += 256) {
         const int i = a0 + r + (int)threadIdx.x;
         const int e = i < w.nat ? w.eidx[i] : -1;
@@ -562,6 +574,11 @@ __global__ __launch_bounds__(256) void k_elem_place(Work w, const int *chunk_off
     }
 }
 
+__global__ __launch_bounds__(256) void k_elem_place(Work w, const int *chunk_off, int nel)
+{
+    elem_place_block(w, chunk_off, nel, (int)blockIdx.x);
+}
+
 int elem_sort_chunks(int nat) { return (nat + kSortChunk - 1) / kSortChunk; }
 
 void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s)
@@ -570,12 +587,6 @@ void launch_elem_sort(Work &w, int nel, int *chunk_cnt, hipStream_t s)
     if (nchunk > 0) hipLaunchKernelGGL(k_elem_count, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt);
     hipLaunchKernelGGL(k_elem_offsets, dim3(1), dim3(64 * kMaxEl), 0, s, w, chunk_cnt, nchunk);
     if (nchunk > 0) hipLaunchKernelGGL(k_elem_place, dim3(nchunk), dim3(256), 0, s, w, chunk_cnt, nel);
-}
-
-void launch_elem_place(Work &w, int nel, int *chunk_off, hipStream_t s)
-{
-    const int nchunk = elem_sort_chunks(w.nat);
-    if (nchunk > 0) hipLaunchKernelGGL(k_elem_place, dim3(nchunk), dim3(256), 0, s, w, chunk_off, nel);
 }
 
 // One stencil cell of atom i: wrapped cell id (within the structure's grid)
