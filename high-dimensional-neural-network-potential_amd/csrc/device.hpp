@@ -249,9 +249,12 @@ void launch_force_gather_t(Work &w, const R *gent, hipStream_t s);
 // per-structure sums of Ei and vir: chunks [c.x, c.y) of atoms never cross a
 // structure; structure s owns chunks [sc[s], sc[s+1]); partial: 10 per chunk;
 // flag_out (or null) receives a copy of w.flags[0..8); done: a counter that
-// is zero between launches (the one-launch form for few structures)
+// is zero between launches (the one-launch form for few structures);
+// orphans_g (or null): the gradient rows of the force gather, whose
+// partnerless entries the first stage's launch adds to w.force
+template <typename R>
 void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   int *flag_out, unsigned *done, hipStream_t s);
+                   int *flag_out, unsigned *done, const R *orphans_g, hipStream_t s);
 
 // ---- kernels_md.hip: velocity Verlet and Verlet-skin bookkeeping ----
 void launch_md_drift(double *pos, const double *mom, const double *F, double *Fold, const double *mass, double dt,

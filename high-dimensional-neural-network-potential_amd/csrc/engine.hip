@@ -1016,8 +1016,14 @@ void Engine::finish_launch(double *force_dev, hipStream_t s, bool timed)
         wk.Ei = w.Ei + (size_t)k * nat;
         wk.result = w.result + 10 * (size_t)k * nstruct;
         int *flag_out = k == 0 ? (int *)(w.result + 10 * (size_t)nm * nstruct) : nullptr;
-        launch_reduce(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(), flag_out,
-                      (unsigned *)(misc_.as<int>() + kIntReduceDone), s);
+        // the first member's reduction also adds the partnerless entries
+        unsigned *done = (unsigned *)(misc_.as<int>() + kIntReduceDone);
+        if (fp64)
+            launch_reduce<double>(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(),
+                                  flag_out, done, k == 0 ? gent_.as<double>() : nullptr, s);
+        else
+            launch_reduce<float>(wk, chunks_.as<int2>(), nchunk, sc_.as<int>(), nstruct, partial_.as<double>(),
+                                 flag_out, done, k == 0 ? gent_.as<float>() : nullptr, s);
     }
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(ev_[6], s));

@@ -51,9 +51,9 @@ __global__ __launch_bounds__(64) void k_force_gather(Work w, const R *g)
 // side only): add their contribution to the neighbour directly. One
 // wavefront per atom (four per block), its lanes over the atom's entries.
 template <typename R>
-__global__ __launch_bounds__(256) void k_orphans(Work w, const R *g)
+__device__ void orphans_block(const Work &w, const R *g, int blk)
 {
-    const int i = (int)blockIdx.x * 4 + (int)threadIdx.x / 64;
+    const int i = blk * 4 + (int)threadIdx.x / 64;
     if (i >= w.nat) return;
     const int cnt = w.count[i];
     for (int t = (int)threadIdx.x % 64; t < cnt; t += 64) {
@@ -66,11 +66,11 @@ __global__ __launch_bounds__(256) void k_orphans(Work w, const R *g)
     }
 }
 
+// the orphans run in the launch of the first reduction stage (launch_reduce)
 template <typename R>
 void launch_force_gather_t(Work &w, const R *gent, hipStream_t s)
 {
     hipLaunchKernelGGL(k_force_gather<R>, dim3(w.nat), dim3(64), 0, s, w, gent);
-    hipLaunchKernelGGL(k_orphans<R>, dim3((w.nat + 3) / 4), dim3(256), 0, s, w, gent);
 }
 template void launch_force_gather_t<double>(Work &, const double *, hipStream_t);
 template void launch_force_gather_t<float>(Work &, const float *, hipStream_t);
@@ -103,26 +103,26 @@ __device__ void reduce_final_wave(const Work &w, const int *sc, const double *pa
 // the same order, its four waves over the structures) and sets the count back
 // to zero, so that one launch does both stages
 template <bool FUSE>
-__global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chunks, double *partial, const int *sc,
-                                                        int nstruct, int *flag_out, unsigned *done)
+__device__ void reduce_partial_block(const Work &w, const int2 *chunks, double *partial, const int *sc, int nstruct,
+                                     int *flag_out, unsigned *done, int blk, int nchunk)
 {
     __shared__ double red[10][256];
     double a[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) a[k] = 0;
-    const int2 c = chunks[blockIdx.x];
+    const int2 c = chunks[blk];
     for (int i = c.x + threadIdx.x; i < c.y; i += 256) {
         a[0] += w.Ei[i];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[1 + k] += w.vir[9 * (size_t)i + k];
     }
     tree10(red, a);
-    if (threadIdx.x < 10) partial[10 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 10) partial[10 * (size_t)blk + threadIdx.x] = red[threadIdx.x][0];
     if constexpr (FUSE) {
         __shared__ int last;
         __threadfence();   // this block's partials reach L2 before it is counted
         __syncthreads();
-        if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+        if (threadIdx.x == 0) last = atomicAdd(done, 1u) == (unsigned)nchunk - 1;
         __syncthreads();
         if (!last) return;
         __threadfence();   // and the other blocks' are read from there
@@ -130,6 +130,28 @@ __global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chun
             reduce_final_wave(w, sc, partial, flag_out, st, (int)threadIdx.x % 64);
         if (threadIdx.x == 0) *done = 0;
     }
+}
+
+template <bool FUSE>
+__global__ __launch_bounds__(256) void k_reduce_partial(Work w, const int2 *chunks, double *partial, const int *sc,
+                                                        int nstruct, int *flag_out, unsigned *done)
+{
+    reduce_partial_block<FUSE>(w, chunks, partial, sc, nstruct, flag_out, done, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// blocks [0, norph): the orphans of four atoms each; the blocks after: the
+// first reduction stage, one chunk each (independent of the forces, so the
+// two share a launch)
+template <typename R, bool FUSE>
+__global__ __launch_bounds__(256) void k_orphans_reduce(Work w, const R *g, int norph, const int2 *chunks,
+                                                        double *partial, const int *sc, int nstruct, int *flag_out,
+                                                        unsigned *done, int nchunk)
+{
+    if ((int)blockIdx.x < norph) {   // block-uniform
+        orphans_block<R>(w, g, (int)blockIdx.x);
+        return;
+    }
+    reduce_partial_block<FUSE>(w, chunks, partial, sc, nstruct, flag_out, done, (int)blockIdx.x - norph, nchunk);
 }
 
 // one wave per structure: lane l adds chunks l, l+64, ... of the structure,
@@ -168,18 +190,34 @@ __global__ __launch_bounds__(64) void k_reduce_final(Work w, const int *sc, cons
 // 9.3 us for the two launches)
 constexpr int kFuseReduceMax = 16;
 
+template <typename R, bool FUSE>
+static void launch_stage1(Work &w, const R *og, const int2 *chunks, int nchunk, const int *sc, int nstruct,
+                          double *partial, int *flag_out, unsigned *done, hipStream_t s)
+{
+    if (og) {
+        const int norph = (w.nat + 3) / 4;
+        hipLaunchKernelGGL((k_orphans_reduce<R, FUSE>), dim3(norph + nchunk), dim3(256), 0, s, w, og, norph, chunks,
+                           partial, sc, nstruct, flag_out, done, nchunk);
+    } else if (nchunk > 0) {
+        hipLaunchKernelGGL(k_reduce_partial<FUSE>, dim3(nchunk), dim3(256), 0, s, w, chunks, partial, sc, nstruct,
+                           flag_out, done);
+    }
+}
+
+template <typename R>
 void launch_reduce(Work &w, const int2 *chunks, int nchunk, const int *sc, int nstruct, double *partial,
-                   int *flag_out, unsigned *done, hipStream_t s)
+                   int *flag_out, unsigned *done, const R *orphans_g, hipStream_t s)
 {
     if (nchunk > 0 && nstruct <= kFuseReduceMax && nchunk <= kFuseReduceMax) {
-        hipLaunchKernelGGL(k_reduce_partial<true>, dim3(nchunk), dim3(256), 0, s, w, chunks, partial, sc, nstruct,
-                           flag_out, done);
+        launch_stage1<R, true>(w, orphans_g, chunks, nchunk, sc, nstruct, partial, flag_out, done, s);
         return;
     }
-    if (nchunk > 0)
-        hipLaunchKernelGGL(k_reduce_partial<false>, dim3(nchunk), dim3(256), 0, s, w, chunks, partial, sc, nstruct,
-                           flag_out, done);
+    launch_stage1<R, false>(w, orphans_g, chunks, nchunk, sc, nstruct, partial, flag_out, done, s);
     hipLaunchKernelGGL(k_reduce_final, dim3(nstruct), dim3(64), 0, s, w, sc, (const double *)partial, flag_out);
 }
+template void launch_reduce<double>(Work &, const int2 *, int, const int *, int, double *, int *, unsigned *,
+                                    const double *, hipStream_t);
+template void launch_reduce<float>(Work &, const int2 *, int, const int *, int, double *, int *, unsigned *,
+                                   const float *, hipStream_t);
 
 }  // namespace hdnnp
