@@ -60,6 +60,24 @@ def test_noalloc_equals_batch_device(pkg, precision):
         assert np.array_equal(E2, E0) and np.array_equal(F2.cpu().numpy(), F0)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_noalloc_counting_sort_path(pkg, precision):
+    """A perovskite cell of 8,448 atoms (the largest both precisions take
+    here): above the one-workgroup sort's 8,192 atoms, so the no-allocation
+    call sorts by cell with the counting sort, from its fixed array; the
+    results equal the allocating entry point's bit for bit."""
+    import torch
+    wl = pkg.workloads
+    h = pkg.parseRuNNerInput(PEROV_PARAMS, precision)
+    nat, Z, pos, per, lat = _inputs(torch, [wl.perovskite(8, 8, 11, seed=5)])
+    print(f"{precision}: {int(nat.sum())} atoms", flush=True)
+    E0, F0, L0 = h.evaluate_batch_torch(nat, Z, pos, per, lat)
+    F0 = F0.cpu().numpy()
+    force = torch.empty((int(nat.sum()), 3), dtype=torch.float64, device=pos.device)
+    E1, F1, L1 = h.evaluate_batch_torch(nat, Z, pos, per, lat, force, noalloc=True)
+    assert np.array_equal(E1, E0) and np.array_equal(F1.cpu().numpy(), F0) and np.array_equal(L1, L0)
+
+
 def test_noalloc_refuses_what_does_not_fit(pkg):
     import torch
     wl = pkg.workloads
